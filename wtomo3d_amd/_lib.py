@@ -1,0 +1,94 @@
+"""Loader for libwtomo3d_hip.so and the C structs it shares with Python."""
+import ctypes
+import math
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libwtomo3d_hip.so")
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class WtGridParams(ctypes.Structure):
+    """Mirror of ``wt_grid_params`` (include/wtomo3d.h)."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("ipml", ctypes.c_int32), ("fst", ctypes.c_int32), ("usemin", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("pmld", ctypes.c_double), ("pmlf", ctypes.c_double),
+                ("omega_re", ctypes.c_double), ("omega_im", ctypes.c_double)]
+
+
+_LIB = None
+_vp = ctypes.c_void_p
+
+
+def _declare(L):
+    i32, i64 = ctypes.c_int, ctypes.c_int64
+    L.wt_graph_nnz.argtypes = [i32] * 3
+    L.wt_graph_nnz.restype = i64
+    L.wt_graph.argtypes = [i32] * 4 + [_vp, _vp]
+    L.wt_graph_dev.argtypes = [i32] * 4 + [_vp, _vp, _vp]
+    L.wt_assemble.argtypes = [ctypes.POINTER(WtGridParams), i32, _vp, _vp, _vp, _vp]
+    L.wt_assemble_dev.argtypes = [ctypes.POINTER(WtGridParams), i32, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.wt_asmble.argtypes = [i32] * 5 + [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp] + \
+        [ctypes.c_float] * 5 + [i32] * 3
+    L.wt_asmble.restype = None
+    L.wt_dev_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+    L.wt_dev_free.argtypes = [_vp]
+    L.wt_h2d.argtypes = [_vp, _vp, ctypes.c_size_t]
+    L.wt_d2h.argtypes = [_vp, _vp, ctypes.c_size_t]
+    L.wt_dev_sync.argtypes = []
+    L.wt_device_count.argtypes = []
+    for name in ("wt_graph", "wt_graph_dev", "wt_assemble", "wt_assemble_dev", "wt_dev_alloc",
+                 "wt_dev_free", "wt_h2d", "wt_d2h", "wt_dev_sync", "wt_device_count"):
+        getattr(L, name).restype = ctypes.c_int
+    try:
+        from . import _solver_decl
+        _solver_decl.declare(L)
+    except ImportError:
+        pass
+    return L
+
+
+def lib():
+    """The loaded library; raises LibraryMissing when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise LibraryMissing("%s is not built: run python -m wtomo3d_amd.build" % LIBPATH)
+        _LIB = _declare(ctypes.CDLL(LIBPATH))
+    return _LIB
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed with code %d" % (what, rc))
+
+
+def pml_params(dx, pmld=None, pmlr=1e-3, single=False):
+    """(ipml, pmlf) as READINI_FORWARD computes them (readini.f90:81-84).
+
+    ``single`` evaluates in float32 like the reference's default REAL."""
+    import numpy as np
+    if pmld is None:
+        pmld = 10.0 * dx
+    if single:
+        f = np.float32
+        dxf, pd, pr = f(dx), f(pmld), f(pmlr)
+        ipml = int(pd / dxf) + 1
+        pmlf = f(3.0) * f(np.log(f(1.0) / pr)) / (f(2.0) * pd * pd * pd)
+        return ipml, float(pmlf)
+    ipml = int(pmld / dx) + 1
+    pmlf = 3.0 * math.log(1.0 / pmlr) / (2.0 * pmld ** 3)
+    return ipml, pmlf
+
+
+def grid_params(nx, ny, nz, dx, omega, pmld=None, pmlr=1e-3, fst=True, usemin=True, single=False):
+    """Build a ``wt_grid_params`` for one frequency (omega may be complex)."""
+    if pmld is None:
+        pmld = 10.0 * dx
+    ipml, pmlf = pml_params(dx, pmld, pmlr, single)
+    om = complex(omega)
+    return WtGridParams(nx, ny, nz, ipml, int(fst), int(usemin), float(dx), float(pmld), float(pmlf),
+                        om.real, om.imag)

@@ -1,0 +1,520 @@
+// Elastic 27-point stencil + PML assembly into CSR on MI355X (gfx950).
+//
+// Replaces ASMBLE (asmble.f90:96-601) and the per-node physics it calls:
+// SETCOEFS3D (setcoefs3d.f:79-233), CALCWTS (calcwts.f:28-71), the region
+// dispatch of FDFD3D (fdfd3d.f:50-115), PFDFD3D (pfdfd3d.f:133-576),
+// PML3D (pml3d.f:167-800) and FIXED3D (fixed3d.f:41-343).  The CSR pattern
+// kernel reproduces SREL3D (srgraphs.f:638-734) followed by SPARSE_ADJ2CRS
+// (sparse.f:13-49) from closed forms.
+//
+// Design: one wave of 64 lanes per (x, y) column segment of 64 z-nodes; the
+// 3x3 column neighbourhood of lambda, mu, rho is staged in LDS; each lane owns
+// one node, forms the octant-cell moduli and stretch factors in registers and
+// walks the 27 neighbour blocks with a wave-uniform label, so the whole wave
+// evaluates one formula at a time (no divergence).  The interior stencil is
+// the stretched form with r1 = 1, r2 = 0 (identical values; the reference
+// notes the identity at fdfd3d.f:56-57).  Output rows are written straight to
+// their CSR offsets, which follow from closed forms, so no atomics or scans.
+#include "wt_common.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int ASM_WAVE = 64;
+
+// Number of existing neighbours along one axis and prefix sums of it.
+__host__ __device__ inline int64_t axis_count(int i, int n) { return 1 + (i > 1) + (i < n); }
+__host__ __device__ inline int64_t axis_prefix(int i, int n)
+{
+    // sum_{i' < i} axis_count(i', n), 1 <= i <= n + 1
+    int64_t a = i - 1, b = i - 2 > 0 ? i - 2 : 0, c = (i - 1 < n - 1) ? i - 1 : n - 1;
+    return a + b + c;
+}
+
+// Offset (0-based, in nonzeros) of the first row of node (iz, ix, iy).
+__host__ __device__ inline int64_t node_offset(int iz, int ix, int iy, int nx, int ny, int nz)
+{
+    int64_t Sx = axis_prefix(nx + 1, nx), Sz = axis_prefix(nz + 1, nz);
+    int64_t cy = axis_count(iy, ny), cx = axis_count(ix, nx);
+    return 9 * (axis_prefix(iy, ny) * Sx * Sz + cy * axis_prefix(ix, nx) * Sz +
+                cy * cx * axis_prefix(iz, nz));
+}
+
+template <typename R>
+struct NodeCoef {
+    cpx<R> u[9], d[9];                   // octant-cell mu and lambda, index 1..8
+    cpx<R> rx2, ry2, rz2, rxy, rxz, ryz;  // r1-derived scalings
+    cpx<R> rx1, ry1, rz1;                // r2 / (2 dx) / 4
+    R wm2o6, wm3o12, wm4o8;
+    cpx<R> mc;                           // centre mass term (complex)
+};
+
+template <typename R>
+__device__ inline R csqrt_re(cpx<R> z)
+{
+    if (z.im == R(0)) return z.re >= R(0) ? sqrt(z.re) : R(0);
+    R m = hypot(z.re, z.im);
+    return sqrt((m + z.re) / R(2));
+}
+
+// Stretch factors along one axis (pml3d.f:207-295 with kmax = 1, fdom = 0).
+template <typename R>
+__device__ inline bool pml_axis(int i, int n, int ipml, R pmld, R dx, R pmlf, R vpreal,
+                                cpx<R> eyeom, cpx<R>& r1, cpx<R>& r2)
+{
+    R dpml = 0;
+    int isn = 0;
+    if (i <= ipml) { dpml = pmld - R(i - 1) * dx; isn = 1; }
+    else if (n - i + 1 <= ipml) { dpml = pmld - R(n - i) * dx; isn = -1; }
+    if (dpml < 0) return false;
+    R dn = dpml * dpml;
+    dn = dn * vpreal;
+    dn = dn * pmlf;
+    r1 = cdiv(eyeom, cpx<R>(eyeom.re + dn, eyeom.im));
+    cpx<R> r1sq = r1 * r1;
+    cpx<R> t = cdiv(eyeom, eyeom * eyeom);
+    cpx<R> q = (R(isn) * R(2)) * r1sq;
+    q = q * r1;
+    q = dpml * q;
+    r2 = q * ((vpreal * pmlf) * t);
+    return true;
+}
+
+// The 3x3 block (row-major u,v,w x u,v,w) coupling a node to neighbour k,
+// k = 9(oy+1) + 3(ox+1) + (oz+1).  ro is the density at that neighbour as the
+// reference's mass lumping sees it.
+template <typename R>
+__device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, cpx<R> v[9])
+{
+    const cpx<R>* u = c.u;
+    auto lmu = [&](int i) { return c.d[i] - u[i]; };
+    auto lpu = [&](int i) { return c.d[i] + u[i]; };
+    auto lp2u = [&](int i) { return (c.d[i] + u[i]) + u[i]; };
+    const cpx<R> rx2 = c.rx2, ry2 = c.ry2, rz2 = c.rz2, rxy = c.rxy, rxz = c.rxz, ryz = c.ryz;
+    const cpx<R> rx1 = c.rx1, ry1 = c.ry1, rz1 = c.rz1;
+    // Diagonal of the 8 corner blocks: q1*L + qs*U +- ra*L +- rb*U + mass.
+    auto corner = [&](cpx<R> q1, cpx<R> qs, int sa, cpx<R> ra, int sb, cpx<R> rb, int cell, cpx<R> mf) {
+        cpx<R> L = lp2u(cell), U = u[cell];
+        cpx<R> s = q1 * L + qs * U;
+        cpx<R> ta = ra * L, tb = rb * U;
+        s = s + (sa > 0 ? ta : -ta);
+        s = s + (sb > 0 ? tb : -tb);
+        return s + mf;
+    };
+    auto put = [&](cpx<R> uu, cpx<R> vv, cpx<R> ww, cpx<R> uv, cpx<R> vu, cpx<R> uw, cpx<R> wu,
+                   cpx<R> vw, cpx<R> wv) {
+        v[0] = uu; v[1] = uv; v[2] = uw;
+        v[3] = vu; v[4] = vv; v[5] = vw;
+        v[6] = wu; v[7] = wv; v[8] = ww;
+    };
+    const cpx<R> rxpy = rx2 + ry2, rxpz = rx2 + rz2, rypz = ry2 + rz2;
+    const cpx<R> r2xpy = rx1 + ry1, r2xmy = rx1 - ry1, r2xpz = rx1 + rz1, r2xmz = rx1 - rz1;
+    const cpx<R> r2ypz = ry1 + rz1, r2ymz = ry1 - rz1;
+    const cpx<R> m4(c.wm4o8 * ro, R(0)), m3(c.wm3o12 * ro, R(0)), m2(c.wm2o6 * ro, R(0));
+    cpx<R> t0, t1, t2, t3, t4, t5;
+    switch (k) {
+    // ---------------- oy = -1 ----------------
+    case 0:  // a1m
+        t0 = rxy * lpu(1); t1 = rxz * lpu(1); t2 = ryz * lpu(1);
+        put(corner(rx2, rypz, -1, rx1, -1, r2ypz, 1, m4), corner(ry2, rxpz, -1, ry1, -1, r2xpz, 1, m4),
+            corner(rz2, rxpy, -1, rz1, -1, r2xpy, 1, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    case 1:  // a2m
+        t0 = -(rz2 * (u[1] + u[5])) + m3; t1 = -(rz2 * (lp2u(1) + lp2u(5))) + m3;
+        t2 = rxy * (lpu(1) + lpu(5)); t3 = -(rxz * (lmu(1) - lmu(5))); t4 = -(ryz * (lmu(1) - lmu(5)));
+        put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
+        break;
+    case 2:  // a3m
+        t0 = rxy * lpu(5); t1 = -(rxz * lpu(5)); t2 = -(ryz * lpu(5));
+        put(corner(rx2, rypz, -1, rx1, -1, r2ymz, 5, m4), corner(ry2, rxpz, -1, ry1, -1, r2xmz, 5, m4),
+            corner(rz2, rxpy, +1, rz1, -1, r2xmy, 5, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    case 3:  // a4m
+        t0 = -(rx2 * (lp2u(1) + lp2u(3))) + m3; t1 = -(rx2 * (u[1] + u[3])) + m3;
+        t2 = rxy * (lmu(1) - lmu(3)); t3 = rxz * (lmu(1) - lmu(3)); t4 = ryz * (lpu(1) + lpu(3));
+        put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
+        break;
+    case 4:  // a5m
+        t0 = ry2 * (((u[1] + u[7]) + u[5]) + u[3]) + m2;
+        t2 = rxy * (((lmu(1) - lmu(7)) + lmu(5)) - lmu(3));
+        t3 = -(rxz * (((lpu(1) + lpu(7)) - lpu(5)) - lpu(3)));
+        t4 = -(ryz * (((lmu(1) - lmu(7)) - lmu(5)) + lmu(3)));
+        put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
+        break;
+    case 5:  // a6m
+        t0 = -(rx2 * (lp2u(7) + lp2u(5))) + m3; t1 = -(rx2 * (u[7] + u[5])) + m3;
+        t2 = rxy * (lmu(5) - lmu(7)); t3 = rxz * (lmu(7) - lmu(5)); t4 = -(ryz * (lpu(7) + lpu(5)));
+        put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
+        break;
+    case 6:  // a7m
+        t0 = -(rxy * lpu(3)); t1 = -(rxz * lpu(3)); t2 = ryz * lpu(3);
+        put(corner(rx2, rypz, +1, rx1, -1, r2ypz, 3, m4), corner(ry2, rxpz, -1, ry1, +1, r2xmz, 3, m4),
+            corner(rz2, rxpy, -1, rz1, +1, r2xpy, 3, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    case 7:  // a8m
+        t0 = -(rz2 * (u[7] + u[3])) + m3; t1 = -(rz2 * (lp2u(7) + lp2u(3))) + m3;
+        t2 = -(rxy * (lpu(7) + lpu(3))); t3 = -(rxz * (lmu(7) - lmu(3))); t4 = -(ryz * (lmu(3) - lmu(7)));
+        put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
+        break;
+    case 8:  // a9m
+        t0 = -(rxy * lpu(7)); t1 = rxz * lpu(7); t2 = -(ryz * lpu(7));
+        put(corner(rx2, rypz, +1, rx1, -1, r2ymz, 7, m4), corner(ry2, rxpz, -1, ry1, +1, r2xpz, 7, m4),
+            corner(rz2, rxpy, +1, rz1, +1, r2xmy, 7, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    // ---------------- oy = 0 ----------------
+    case 9:  // a1n
+        t0 = -(ry2 * (u[1] + u[2])) + m3;
+        t2 = rxy * (lmu(2) - lmu(1)); t3 = rxz * (lpu(1) + lpu(2)); t4 = ryz * (lmu(1) - lmu(2));
+        put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
+        break;
+    case 10:  // a2n
+        t0 = rx2 * (((lp2u(1) + lp2u(6)) + lp2u(2)) + lp2u(5)) + m2;
+        t1 = rx2 * (((u[1] + u[6]) + u[2]) + u[5]) + m2;
+        t2 = rxy * (((lmu(2) - lmu(5)) + lmu(6)) - lmu(1));
+        t3 = -(rxz * (((lmu(2) - lmu(5)) + lmu(1)) - lmu(6)));
+        t4 = ryz * (((lpu(2) + lpu(5)) - lpu(1)) - lpu(6));
+        put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
+        break;
+    case 11:  // a3n
+        t0 = -(ry2 * (u[5] + u[6])) + m3;
+        t2 = -(rxy * (lmu(5) - lmu(6))); t3 = -(rxz * (lpu(5) + lpu(6))); t4 = -(ryz * (lmu(5) - lmu(6)));
+        put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
+        break;
+    case 12:  // a4n
+        t0 = rz2 * (((u[1] + u[4]) + u[2]) + u[3]) + m2;
+        t1 = rz2 * (((lp2u(1) + lp2u(4)) + lp2u(2)) + lp2u(3)) + m2;
+        t2 = -(rxy * (((lpu(1) + lpu(4)) - lpu(2)) - lpu(3)));
+        t3 = rxz * (((lmu(1) - lmu(4)) + lmu(2)) - lmu(3));
+        t4 = ryz * (((lmu(1) - lmu(4)) + lmu(3)) - lmu(2));
+        put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
+        break;
+    case 13: {  // a5n, the node itself
+        cpx<R> ls = lp2u(1), us = u[1];
+#pragma unroll
+        for (int i = 2; i <= 8; ++i) { ls = ls + lp2u(i); us = us + u[i]; }
+        cpx<R> ex = ((((((lp2u(1) - lp2u(8)) + lp2u(2)) - lp2u(7)) + lp2u(5)) - lp2u(4)) + lp2u(6)) - lp2u(3);
+        cpx<R> fa = ((u[1] - u[8]) + u[3]) - u[6];
+        cpx<R> fb = ((u[7] - u[2]) + u[5]) - u[4];
+        t0 = ((((-(rx2 * ls) - rypz * us) + rx1 * ex) + r2ypz * fa) + r2ymz * fb) + c.mc;
+        cpx<R> ey = ((((((lp2u(1) - lp2u(2)) + lp2u(5)) - lp2u(6)) + lp2u(7)) - lp2u(8)) + lp2u(3)) - lp2u(4);
+        fa = ((u[1] - u[8]) + u[2]) - u[7];
+        fb = ((u[6] - u[3]) + u[5]) - u[4];
+        t1 = ((((-(ry2 * ls) - rxpz * us) + ry1 * ey) + r2xpz * fa) + r2xmz * fb) + c.mc;
+        cpx<R> ez = ((((((lp2u(1) - lp2u(8)) + lp2u(2)) - lp2u(7)) + lp2u(4)) - lp2u(5)) + lp2u(3)) - lp2u(6);
+        fa = ((u[1] - u[8]) + u[5]) - u[4];
+        fb = ((u[2] - u[7]) + u[6]) - u[3];
+        t2 = ((((-(rz2 * ls) - rxpy * us) + rz1 * ez) + r2xpy * fa) + r2xmy * fb) + c.mc;
+        t3 = rxy * (((((((lpu(2) - lpu(1)) + lpu(7)) - lpu(8)) + lpu(3)) - lpu(4)) + lpu(6)) - lpu(5));
+        t4 = rxz * (((((((lpu(4) - lpu(1)) + lpu(5)) - lpu(8)) + lpu(3)) - lpu(2)) + lpu(6)) - lpu(7));
+        t5 = ryz * (((((((lpu(2) - lpu(1)) + lpu(7)) - lpu(8)) + lpu(4)) - lpu(3)) + lpu(5)) - lpu(6));
+        put(t0, t1, t2, t3, t3, t4, t4, t5, t5);
+        break;
+    }
+    case 14:  // a6n
+        t0 = rz2 * (((u[8] + u[5]) + u[7]) + u[6]) + m2;
+        t1 = rz2 * (((lp2u(8) + lp2u(5)) + lp2u(7)) + lp2u(6)) + m2;
+        t2 = -(rxy * (((lpu(8) + lpu(5)) - lpu(7)) - lpu(6)));
+        t3 = rxz * (((lmu(8) - lmu(5)) + lmu(7)) - lmu(6));
+        t4 = ryz * (((lmu(8) - lmu(5)) + lmu(6)) - lmu(7));
+        put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
+        break;
+    case 15:  // a7n
+        t0 = -(ry2 * (u[4] + u[3])) + m3;
+        t2 = -(rxy * (lmu(4) - lmu(3))); t3 = -(rxz * (lpu(4) + lpu(3))); t4 = -(ryz * (lmu(4) - lmu(3)));
+        put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
+        break;
+    case 16:  // a8n
+        t0 = rx2 * (((lp2u(8) + lp2u(3)) + lp2u(7)) + lp2u(4)) + m2;
+        t1 = rx2 * (((u[8] + u[3]) + u[7]) + u[4]) + m2;
+        t2 = rxy * (((lmu(7) - lmu(4)) + lmu(3)) - lmu(8));
+        t3 = -(rxz * (((lmu(7) - lmu(4)) + lmu(8)) - lmu(3)));
+        t4 = ryz * (((lpu(7) + lpu(4)) - lpu(8)) - lpu(3));
+        put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
+        break;
+    case 17:  // a9n
+        t0 = -(ry2 * (u[8] + u[7])) + m3;
+        t2 = rxy * (lmu(7) - lmu(8)); t3 = rxz * (lpu(8) + lpu(7)); t4 = ryz * (lmu(8) - lmu(7));
+        put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
+        break;
+    // ---------------- oy = +1 ----------------
+    case 18:  // a1p
+        t0 = -(rxy * lpu(2)); t1 = rxz * lpu(2); t2 = -(ryz * lpu(2));
+        put(corner(rx2, rypz, -1, rx1, +1, r2ymz, 2, m4), corner(ry2, rxpz, +1, ry1, -1, r2xpz, 2, m4),
+            corner(rz2, rxpy, -1, rz1, -1, r2xmy, 2, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    case 19:  // a2p
+        t0 = -(rz2 * (u[2] + u[6])) + m3; t1 = -(rz2 * (lp2u(2) + lp2u(6))) + m3;
+        t2 = -(rxy * (lpu(2) + lpu(6))); t3 = -(rxz * (lmu(2) - lmu(6))); t4 = -(ryz * (lmu(6) - lmu(2)));
+        put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
+        break;
+    case 20:  // a3p
+        t0 = -(rxy * lpu(6)); t1 = -(rxz * lpu(6)); t2 = ryz * lpu(6);
+        put(corner(rx2, rypz, -1, rx1, +1, r2ypz, 6, m4), corner(ry2, rxpz, +1, ry1, -1, r2xmz, 6, m4),
+            corner(rz2, rxpy, +1, rz1, -1, r2xpy, 6, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    case 21:  // a4p
+        t0 = -(rx2 * (lp2u(2) + lp2u(4))) + m3; t1 = -(rx2 * (u[2] + u[4])) + m3;
+        t2 = rxy * (lmu(4) - lmu(2)); t3 = rxz * (lmu(2) - lmu(4)); t4 = -(ryz * (lpu(2) + lpu(4)));
+        put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
+        break;
+    case 22:  // a5p
+        t0 = ry2 * (((u[8] + u[2]) + u[4]) + u[6]) + m2;
+        t2 = rxy * (((lmu(8) - lmu(2)) + lmu(4)) - lmu(6));
+        t3 = -(rxz * (((lpu(8) + lpu(2)) - lpu(4)) - lpu(6)));
+        t4 = -(ryz * (((lmu(8) - lmu(2)) - lmu(4)) + lmu(6)));
+        put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
+        break;
+    case 23:  // a6p
+        t0 = -(rx2 * (lp2u(8) + lp2u(6))) + m3; t1 = -(rx2 * (u[8] + u[6])) + m3;
+        t2 = rxy * (lmu(8) - lmu(6)); t3 = rxz * (lmu(8) - lmu(6)); t4 = ryz * (lpu(8) + lpu(6));
+        put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
+        break;
+    case 24:  // a7p
+        t0 = rxy * lpu(4); t1 = -(rxz * lpu(4)); t2 = -(ryz * lpu(4));
+        put(corner(rx2, rypz, +1, rx1, +1, r2ymz, 4, m4), corner(ry2, rxpz, +1, ry1, +1, r2xmz, 4, m4),
+            corner(rz2, rxpy, -1, rz1, +1, r2xmy, 4, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    case 25:  // a8p
+        t0 = -(rz2 * (u[8] + u[4])) + m3; t1 = -(rz2 * (lp2u(8) + lp2u(4))) + m3;
+        t2 = rxy * (lpu(8) + lpu(4)); t3 = -(rxz * (lmu(8) - lmu(4))); t4 = -(ryz * (lmu(8) - lmu(4)));
+        put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
+        break;
+    default:  // 26, a9p
+        t0 = rxy * lpu(8); t1 = rxz * lpu(8); t2 = ryz * lpu(8);
+        put(corner(rx2, rypz, +1, rx1, +1, r2ypz, 8, m4), corner(ry2, rxpz, +1, ry1, +1, r2xpz, 8, m4),
+            corner(rz2, rxpy, +1, rz1, +1, r2xpy, 8, m4), t0, t0, t1, t1, t2, t2);
+        break;
+    }
+}
+
+// LDS tile: 3 (x) * 3 (y) columns of ASM_WAVE + 2 z-nodes.
+constexpr int TZ = ASM_WAVE + 2;
+
+template <typename R>
+__global__ __launch_bounds__(ASM_WAVE) void assemble_values_kernel(
+    wt_grid_params p, const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ mu,
+    const R* __restrict__ rho, cpx<R>* __restrict__ out, int* __restrict__ err)
+{
+    __shared__ cpx<R> s_lam[9][TZ];
+    __shared__ cpx<R> s_mu[9][TZ];
+    __shared__ R s_rho[9][TZ];
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int z0 = blockIdx.x * ASM_WAVE;  // 0-based first z of this segment
+    const int ix = blockIdx.y + 1, iy = blockIdx.z + 1;
+    const int lane = threadIdx.x;
+
+    // Stage the clamped 3x3 column neighbourhood, z in [z0-1, z0+ASM_WAVE].
+    for (int e = lane; e < 9 * TZ; e += ASM_WAVE) {
+        int col = e / TZ, zz = e - col * TZ;
+        int xg = ix + (col / 3) - 1, yg = iy + (col % 3) - 1;
+        xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
+        yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
+        int zg = z0 + zz;  // 1-based z of tile slot zz is z0 + zz
+        zg = zg < 1 ? 1 : (zg > nz ? nz : zg);
+        int64_t g = (int64_t)(zg - 1) + (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz;
+        s_lam[col][zz] = lam[g];
+        s_mu[col][zz] = mu[g];
+        s_rho[col][zz] = rho[g];
+    }
+    __syncthreads();
+
+    const int iz = z0 + lane + 1;
+    if (iz > nz) return;
+    const int ip = p.ipml;
+    // Region class, fdfd3d.f:50-115.
+    const bool inx = ix > ip && ix <= nx - ip, iny = iy > ip && iy <= ny - ip;
+    int mode = 1, izp = iz;  // 0 interior, 1 PML, 2 Dirichlet
+    if (inx && iny && iz > ip && iz <= nz - ip) mode = 0;
+    else if (iz == nz || ix == 1 || ix == nx || iy == 1 || iy == ny) mode = 2;
+    else if (iz <= ip) {
+        if (iz == 1 && !p.fst) mode = 2;
+        else if (iz == 1 && !p.usemin) { atomicOr(err, 2); return; }
+        else if (!p.fst) mode = 1;
+        else if (inx && iny) mode = 0;
+        else { mode = 1; izp = ip + 1; }
+    }
+    const bool topfree = p.fst && iz == 1;
+    const int tz = lane + 1;  // tile z-slot of this node
+
+    NodeCoef<R> c;
+    cpx<R> cen_mc;
+    {
+        // setcoefs3d.f:108-174, tile column index = 3*(x offset) + (y offset)
+        auto cell = [&](cpx<R> (*a)[TZ], cpx<R>* o) {
+            cpx<R> km[3][4];
+#pragma unroll
+            for (int kz = 0; kz < 3; ++kz) {
+                int z = tz + kz - 1;
+                km[kz][0] = (((a[0][z] + a[3][z]) + a[1][z]) + a[4][z]) / R(4);
+                km[kz][1] = (((a[2][z] + a[5][z]) + a[1][z]) + a[4][z]) / R(4);
+                km[kz][2] = (((a[6][z] + a[3][z]) + a[7][z]) + a[4][z]) / R(4);
+                km[kz][3] = (((a[8][z] + a[5][z]) + a[7][z]) + a[4][z]) / R(4);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                o[1 + q] = (km[0][q] + km[1][q]) / R(2);
+                o[5 + q] = (km[2][q] + km[1][q]) / R(2);
+            }
+        };
+        cell(s_mu, c.u);
+        cell(s_lam, c.d);
+        if (topfree) {
+#pragma unroll
+            for (int q = 1; q <= 4; ++q) { c.u[q] = cpx<R>(0, 0); c.d[q] = cpx<R>(0, 0); }
+        }
+        // setcoefs3d.f:224-225, calcwts.f:28-71
+        cpx<R> lc = s_lam[4][tz], mcn = s_mu[4][tz];
+        R rc = s_rho[4][tz];
+        R vp = csqrt_re((lc + R(2) * mcn) / rc);
+        R vs = csqrt_re(mcn / rc);
+        double x = (double)(vp / vs);
+        double w1 = 0.2909366e+00 + x * (-0.1792402e-01 + x * (0.7329184e-02 + x * -0.1055034e-02));
+        double w2 = 0.0, w3 = 0.0;
+        if (w1 > 1.0) w1 = 1.0;
+        else w2 = 0.6367377e+00 + x * (-0.7982022e-02 + x * (0.3385164e-02 + x * -0.5065399e-03));
+        if ((w1 + w2) > 1.0) w2 = 0.0;
+        else w3 = 0.7244875e-01 + x * (0.2569959e-01 + x * (-0.1060011e-01 + x * 0.1540534e-02));
+        if ((w1 + w2 + w3) > 1.0) w3 = 1.0 - (w1 + w2);
+        double w4 = 1.0 - (w1 + w2 + w3);
+        const R wm1 = (R)w1, wm2 = (R)w2, wm3 = (R)w3, wm4 = (R)w4;
+
+        const cpx<R> om(p.omega_re, p.omega_im);
+        const cpx<R> eyeom(om.im, -om.re);
+        const cpx<R> osq = om * om;
+        c.wm2o6 = (osq.re * wm2) / R(6);
+        c.wm3o12 = (osq.re * wm3) / R(12);
+        c.wm4o8 = (osq.re * wm4) / R(8);
+        cpx<R> mcen = wm1 * osq;
+        c.mc = cpx<R>(mcen.re * rc, mcen.im * rc);
+        (void)cen_mc;
+
+        const R dx = (R)p.dx, dx2 = dx + dx, dxx = dx * dx, dxx4 = (dxx + dxx) + (dxx + dxx);
+        cpx<R> r1x(1, 0), r1y(1, 0), r1z(1, 0), r2x(0, 0), r2y(0, 0), r2z(0, 0);
+        if (mode == 1) {
+            const R pmld = (R)p.pmld, pmlf = (R)p.pmlf;
+            bool ok = pml_axis(ix, nx, ip, pmld, dx, pmlf, vp, eyeom, r1x, r2x);
+            ok = pml_axis(iy, ny, ip, pmld, dx, pmlf, vp, eyeom, r1y, r2y) && ok;
+            ok = pml_axis(izp, nz, ip, pmld, dx, pmlf, vp, eyeom, r1z, r2z) && ok;
+            if (!ok) atomicOr(err, 1);
+        }
+        c.rx1 = (r2x / dx2) / R(4);
+        c.ry1 = (r2y / dx2) / R(4);
+        c.rz1 = (r2z / dx2) / R(4);
+        c.rx2 = (((r1x * r1x) / dxx) / R(4)) / R(2);
+        c.ry2 = (((r1y * r1y) / dxx) / R(4)) / R(2);
+        c.rz2 = (((r1z * r1z) / dxx) / R(4)) / R(2);
+        c.rxy = ((r1x * r1y) / dxx4) / R(4);
+        c.rxz = ((r1x * r1z) / dxx4) / R(4);
+        c.ryz = ((r1y * r1z) / dxx4) / R(4);
+    }
+
+    // Row layout of this node: 3 rows of 3*nnb entries, neighbours ascending.
+    const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), czn = 1 + (iz > 1) + (iz < nz);
+    const int rowlen = 3 * cxn * cyn * czn;
+    cpx<R>* base = out + node_offset(iz, ix, iy, nx, ny, nz);
+
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
+        const bool exists = (iy + oy >= 1) && (iy + oy <= ny) && (ix + ox >= 1) && (ix + ox <= nx) &&
+                            (iz + oz >= 1) && (iz + oz <= nz);
+        // density the mass lumping uses at this neighbour (clamped gather,
+        // top layer zeroed at the free surface, setcoefs3d.f:187-191)
+        R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
+        if (topfree && oz == -1) ro = R(0);
+        cpx<R> v[9];
+        label_block<R>(k, c, ro, v);
+        if (mode == 2) {
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
+        }
+        if (exists) {
+            const int rank = ((oy > -1) ? (iy > 1) + (oy > 0) : 0) * cxn * czn +
+                             ((ox > -1) ? (ix > 1) + (ox > 0) : 0) * czn +
+                             ((oz > -1) ? (iz > 1) + (oz > 0) : 0);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                cpx<R>* dst = base + r * rowlen + 3 * rank;
+                dst[0] = v[3 * r + 0];
+                dst[1] = v[3 * r + 1];
+                dst[2] = v[3 * r + 2];
+            }
+        }
+    }
+}
+
+// CSR pattern: row pointers and ascending column indices (base 0 or 1).
+__global__ __launch_bounds__(ASM_WAVE) void assemble_pattern_kernel(
+    int nx, int ny, int nz, int base, int32_t* __restrict__ rowptr, int32_t* __restrict__ colidx)
+{
+    const int iz = blockIdx.x * ASM_WAVE + threadIdx.x + 1;
+    const int ix = blockIdx.y + 1, iy = blockIdx.z + 1;
+    if (iz > nz) return;
+    const int64_t g0 = (int64_t)(iz - 1) + (int64_t)(ix - 1) * nz + (int64_t)(iy - 1) * nx * nz;
+    const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), czn = 1 + (iz > 1) + (iz < nz);
+    const int rowlen = 3 * cxn * cyn * czn;
+    const int64_t off = node_offset(iz, ix, iy, nx, ny, nz);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) rowptr[3 * g0 + r] = (int32_t)(off + r * rowlen + base);
+    if (ix == nx && iy == ny && iz == nz) rowptr[3 * g0 + 3] = (int32_t)(off + 3 * rowlen + base);
+    int pos = 0;
+    for (int oy = -1; oy <= 1; ++oy) {
+        if (iy + oy < 1 || iy + oy > ny) continue;
+        for (int ox = -1; ox <= 1; ++ox) {
+            if (ix + ox < 1 || ix + ox > nx) continue;
+            for (int oz = -1; oz <= 1; ++oz) {
+                if (iz + oz < 1 || iz + oz > nz) continue;
+                const int64_t gn = g0 + oz + (int64_t)ox * nz + (int64_t)oy * nx * nz;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    int32_t* dst = colidx + off + r * rowlen + pos;
+                    dst[0] = (int32_t)(3 * gn + base);
+                    dst[1] = (int32_t)(3 * gn + 1 + base);
+                    dst[2] = (int32_t)(3 * gn + 2 + base);
+                }
+                pos += 3;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers (C++ linkage; the C ABI lives in wt_api.cpp)
+
+int64_t wt_nnz(int nx, int ny, int nz)
+{
+    return 9 * axis_prefix(nx + 1, nx) * axis_prefix(ny + 1, ny) * axis_prefix(nz + 1, nz);
+}
+
+int wt_launch_pattern(int nx, int ny, int nz, int base, int32_t* d_rowptr, int32_t* d_colidx,
+                      hipStream_t stream)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return WT_ERR_ARG;
+    dim3 grid((nz + ASM_WAVE - 1) / ASM_WAVE, nx, ny);
+    hipLaunchKernelGGL(assemble_pattern_kernel, grid, dim3(ASM_WAVE), 0, stream, nx, ny, nz, base,
+                       d_rowptr, d_colidx);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_launch_values(const wt_grid_params& p, const void* d_lam, const void* d_mu, const void* d_rho,
+                     void* d_vals, int* d_err, hipStream_t stream)
+{
+    if (p.nx < 2 || p.ny < 2 || p.nz < 2) return WT_ERR_ARG;
+    if (!p.usemin && p.fst) return WT_ERR_UNSUPPORTED;
+    dim3 grid((p.nz + ASM_WAVE - 1) / ASM_WAVE, p.nx, p.ny);
+    hipLaunchKernelGGL(assemble_values_kernel<R>, grid, dim3(ASM_WAVE), 0, stream, p,
+                       (const cpx<R>*)d_lam, (const cpx<R>*)d_mu, (const R*)d_rho, (cpx<R>*)d_vals, d_err);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template int wt_launch_values<float>(const wt_grid_params&, const void*, const void*, const void*, void*,
+                                     int*, hipStream_t);
+template int wt_launch_values<double>(const wt_grid_params&, const void*, const void*, const void*, void*,
+                                      int*, hipStream_t);
