@@ -22,6 +22,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
           # keep hipcc from emitting packed-FP32 instructions
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-Wno-unused-command-line-argument"]
 
 
