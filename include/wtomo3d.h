@@ -76,6 +76,71 @@ void wt_asmble(int lunlog, int nzero, int nx, int ny, int nz, const int32_t *irp
                const float *rho, float omega_re, float omega_im, float dx, float pmld,
                float pmlf, int ipml, int fst, int usemin);
 
+/* ---------------- multifrontal solver ---------------- */
+
+/* Opaque solver handle: one grid (nx, ny, nz), its nested-dissection
+ * elimination tree and the factor / solve buffers on the GPU. */
+typedef struct wt_solver wt_solver;
+
+typedef struct wt_solve_stats {
+    int32_t iterations;          /* solves with the factor (1 = no refinement) */
+    int32_t nrhs;
+    double rel_residual;         /* max_j ||b_j - A x_j|| / ||b_j|| (fp64) */
+    double rel_residual_first;   /* the same after the first solve */
+    double abs_residual;         /* max_j ||b_j - A x_j|| */
+    double ms;                   /* wall time of the call */
+} wt_solve_stats;
+
+/* Ordering + symbolic analysis (the work of STRUMPACK_reorder in job 1,
+ * strumpack_finter.c:24-66): geometric nested dissection of the grid down
+ * to boxes of at most leaf_nodes nodes (<= 0: 64), front sizes, update sets,
+ * assembly maps and arena offsets.  Builds the CSR pattern on the GPU. */
+int wt_solver_create(int nx, int ny, int nz, int leaf_nodes, wt_solver **out);
+int wt_solver_destroy(wt_solver *h);
+
+/* info[8] = {unknowns, nonzeros, fronts, tree levels, max pivots per front,
+ * max update size, factor entries, contribution-block arena entries};
+ * *flops = sum over fronts of 8 (p^3/3 + p^2 u + p u^2). */
+int wt_solver_info(const wt_solver *h, int64_t *info, double *flops);
+
+/* HBM bytes the solver needs for a factorization in factor_prec and a
+ * block solve of nrhs columns (the admission estimate). */
+int64_t wt_solver_bytes(const wt_solver *h, int factor_prec, int nrhs);
+
+/* lambda, mu: complex128[nxnynz]; rho: double[nxnynz] (host or device). */
+int wt_solver_set_model(wt_solver *h, const void *lam, const void *mu, const double *rho, int on_device);
+
+/* FP64 assembly of the operator at p->omega into the solver's CSR values. */
+int wt_solver_assemble(wt_solver *h, const wt_grid_params *p);
+
+/* CSR values in the library's pattern order (wt_graph), complex64
+ * (prec 32, widened) or complex128 (prec 64); host or device. */
+int wt_solver_set_values(wt_solver *h, const void *vals, int prec, int on_device);
+int wt_solver_get_values(wt_solver *h, void *vals);
+
+/* Multifrontal LU of the current values in complex64 (32) or complex128
+ * (64): replaces STRUMPACK_factor (strumpack_finter.c:84).  Returns
+ * WT_ERR_PIVOT when a diagonal tile had an exactly zero pivot. */
+int wt_solver_factor(wt_solver *h, int factor_prec);
+int wt_solver_release(wt_solver *h, int factor_prec);
+
+/* Solve A X = B for nrhs right-hand sides (row-major n x nrhs complex128,
+ * unknown-major: all columns of one unknown contiguous).  With the
+ * complex64 factor each block of <= 64 columns is refined with FP64
+ * residuals until max_j ||r_j||/||b_j|| <= tol or max_iter solves.
+ * Replaces STRUMPACK_solve (strumpack_finter.c:102) for a whole block. */
+int wt_solver_solve(wt_solver *h, int factor_prec, int nrhs, const void *b, void *x, int on_device,
+                    double tol, int max_iter, wt_solve_stats *stats);
+
+/* Drop-in for the reference's solver shim with the same argument list and
+ * job semantics (strumpack_finter.c:9-120, interface.f90:297-307):
+ * 1 analyse, 2 factor (complex64 values), 3 solve one right-hand side,
+ * -1 release.  irptr/jcptr are 1-based and left unchanged. */
+void strumpack_serial_finter(int *job, int *nrows, int *nnz, int *irptr, int *jcptr, void *a, void *rhs,
+                             void *sol, int *ierr);
+/* The reference's empty MPI stub (strumpack_finter.c:123-126). */
+int strumpack_parallel_finter(void);
+
 /* ---------------- device memory helpers ---------------- */
 int wt_dev_alloc(size_t bytes, void **d_ptr);
 int wt_dev_free(void *d_ptr);

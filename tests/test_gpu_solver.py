@@ -1,0 +1,132 @@
+"""GPU multifrontal factor + block solve against the CPU direct-solve oracle.
+
+The reference's own solve (STRUMPACK) is not available, so the solution is
+pinned by an independent FP64 SuperLU solve of the same system and by the
+FP64 residual; tolerances are stated per test."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import wtomo3d_amd as W
+from wtomo3d_amd.solver import Solver
+from solve_cases import problem, rhs_block
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import solve_oracle as SO  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, nz, leaf nodes): single-level, multi-level, multi-tile separators
+CASES = [(5, 4, 6, 64), (10, 9, 11, 64), (16, 15, 14, 64), (12, 12, 12, 8), (9, 22, 7, 27)]
+
+
+def _setup(nx, ny, nz, leaf, seed=1):
+    prm, lam, mu, rho, ir, jc, vals = problem(nx, ny, nz, seed=seed)
+    A = SO.csr(ir, jc, vals)
+    s = Solver(nx, ny, nz, leaf_nodes=leaf)
+    s.set_model(lam, mu, rho)
+    g = W._lib.WtGridParams(nx, ny, nz, prm["ipml"], 1, 1, prm["dx"], prm["pmld"], prm["pmlf"], prm["omega"], 0.0)
+    s.assemble(g)
+    return s, A, vals
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fp64_factor_solve_matches_superlu(case):
+    nx, ny, nz, leaf = case
+    s, A, vals = _setup(nx, ny, nz, leaf)
+    assert np.array_equal(s.values(), vals)  # FP64 assembly on the GPU == oracle
+    B, Xt = rhs_block(A, 3, 11)
+    Xo = SO.solve_superlu(A, B)
+    s.factor(64)
+    X, st = s.solve(B, prec=64, tol=0.0, max_iter=1)
+    assert st["iterations"] == 1
+    assert st["rel_residual"] <= 1e-11
+    assert SO.rel_residual(A, X, B) <= 1e-11
+    assert np.max(np.abs(X - Xo)) <= 1e-10 * np.max(np.abs(Xo))
+    s.close()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fp32_factor_with_fp64_refinement(case):
+    nx, ny, nz, leaf = case
+    s, A, vals = _setup(nx, ny, nz, leaf, seed=2)
+    B, Xt = rhs_block(A, 4, 12)
+    Xo = SO.solve_superlu(A, B)
+    s.factor(32)
+    X, st = s.solve(B, prec=32, tol=1e-10, max_iter=30)
+    assert st["rel_residual"] <= 1e-10
+    assert SO.rel_residual(A, X, B) <= 1e-10
+    assert np.max(np.abs(X - Xo)) <= 1e-8 * np.max(np.abs(Xo))
+    assert st["rel_residual_first"] < 1e-2  # the complex64 factor alone is a good preconditioner
+    s.close()
+
+
+def test_block_solve_equals_column_solves():
+    """Each right-hand side is solved the same alone and inside a block."""
+    s, A, vals = _setup(10, 9, 11, 64)
+    B, _ = rhs_block(A, 5, 13)
+    s.factor(32)
+    Xb, _ = s.solve(B, prec=32, tol=1e-10, max_iter=30)
+    for j in range(5):
+        xj, _ = s.solve(B[:, j], prec=32, tol=1e-10, max_iter=30)
+        assert np.array_equal(xj, Xb[:, j])
+    s.close()
+
+
+def test_refactor_same_handle_new_frequency():
+    nx, ny, nz = 10, 9, 11
+    prm, lam, mu, rho, ir, jc, vals = problem(nx, ny, nz, seed=4)
+    s = Solver(nx, ny, nz)
+    s.set_model(lam, mu, rho)
+    for f in (0.15, 0.4):
+        om = 2 * np.pi * f
+        g = W._lib.WtGridParams(nx, ny, nz, prm["ipml"], 1, 1, prm["dx"], prm["pmld"], prm["pmlf"], om, 0.0)
+        s.assemble(g)
+        A = SO.csr(ir, jc, s.values())
+        B, _ = rhs_block(A, 2, 14)
+        s.factor(64)
+        X, st = s.solve(B, prec=64, tol=0.0, max_iter=1)
+        assert SO.rel_residual(A, X, B) <= 1e-11
+    s.close()
+
+
+def test_strumpack_serial_finter_dropin():
+    """Jobs 1/2/3/-1 with the reference's argument list (strumpack_finter.c:9)."""
+    nx, ny, nz = 8, 7, 9
+    prm, lam, mu, rho, ir, jc, vals64 = problem(nx, ny, nz, seed=6)
+    vals = vals64.astype(np.complex64)
+    n = ir.size - 1
+    A = SO.csr(ir, jc, vals.astype(np.complex128))
+    b = np.asarray(A @ (np.ones(n) + 0.5j)).astype(np.complex64)
+    x = np.zeros(n, np.complex64)
+    L = W.lib()
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    ierr = ctypes.c_int(-1)
+    ir0, jc0 = ir.copy(), jc.copy()
+    # solve before init / factor: error, like the reference
+    L.strumpack_serial_finter(I(3), I(n), I(jc.size), P(ir), P(jc), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 1
+    L.strumpack_serial_finter(I(2), I(n), I(jc.size), P(ir), P(jc), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 1
+    L.strumpack_serial_finter(I(1), I(n), I(jc.size), P(ir), P(jc), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 0
+    L.strumpack_serial_finter(I(1), I(n), I(jc.size), P(ir), P(jc), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 1  # already initialized
+    L.strumpack_serial_finter(I(2), I(n), I(jc.size), P(ir), P(jc), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 0
+    L.strumpack_serial_finter(I(3), I(n), I(jc.size), P(ir), P(jc), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 0
+    assert np.array_equal(ir, ir0) and np.array_equal(jc, jc0)  # pattern left untouched
+    r = np.linalg.norm(b.astype(np.complex128) - A @ x.astype(np.complex128)) / np.linalg.norm(b)
+    assert r <= 1e-4  # rctol of strumpack_finter.c:39, on the complex64 output
+    L.strumpack_serial_finter(I(-1), I(n), I(jc.size), None, None, None, None, None, ctypes.byref(ierr))
+    assert ierr.value == 0
+    # a pattern that is not a 27-point grid is rejected at job 1
+    bad = jc.copy()
+    bad[5], bad[6] = bad[6], bad[5]
+    L.strumpack_serial_finter(I(1), I(n), I(jc.size), P(ir), P(bad), P(vals), P(b), P(x), ctypes.byref(ierr))
+    assert ierr.value == 1

@@ -19,6 +19,13 @@ class WtGridParams(ctypes.Structure):
                 ("omega_re", ctypes.c_double), ("omega_im", ctypes.c_double)]
 
 
+class WtSolveStats(ctypes.Structure):
+    """Mirror of ``wt_solve_stats``."""
+    _fields_ = [("iterations", ctypes.c_int32), ("nrhs", ctypes.c_int32),
+                ("rel_residual", ctypes.c_double), ("rel_residual_first", ctypes.c_double),
+                ("abs_residual", ctypes.c_double), ("ms", ctypes.c_double)]
+
+
 _LIB = None
 _vp = ctypes.c_void_p
 
@@ -43,11 +50,26 @@ def _declare(L):
     for name in ("wt_graph", "wt_graph_dev", "wt_assemble", "wt_assemble_dev", "wt_dev_alloc",
                  "wt_dev_free", "wt_h2d", "wt_d2h", "wt_dev_sync", "wt_device_count"):
         getattr(L, name).restype = ctypes.c_int
-    try:
-        from . import _solver_decl
-        _solver_decl.declare(L)
-    except ImportError:
-        pass
+    i32p = ctypes.POINTER(ctypes.c_int)
+    L.wt_solver_create.argtypes = [i32] * 4 + [ctypes.POINTER(ctypes.c_void_p)]
+    L.wt_solver_destroy.argtypes = [_vp]
+    L.wt_solver_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
+    L.wt_solver_bytes.argtypes = [_vp, i32, i32]
+    L.wt_solver_bytes.restype = i64
+    L.wt_solver_set_model.argtypes = [_vp, _vp, _vp, _vp, i32]
+    L.wt_solver_assemble.argtypes = [_vp, ctypes.POINTER(WtGridParams)]
+    L.wt_solver_set_values.argtypes = [_vp, _vp, i32, i32]
+    L.wt_solver_get_values.argtypes = [_vp, _vp]
+    L.wt_solver_factor.argtypes = [_vp, i32]
+    L.wt_solver_release.argtypes = [_vp, i32]
+    L.wt_solver_solve.argtypes = [_vp, i32, i32, _vp, _vp, i32, ctypes.c_double, i32,
+                                  ctypes.POINTER(WtSolveStats)]
+    for name in ("wt_solver_create", "wt_solver_destroy", "wt_solver_info", "wt_solver_set_model",
+                 "wt_solver_assemble", "wt_solver_set_values", "wt_solver_get_values", "wt_solver_factor",
+                 "wt_solver_release", "wt_solver_solve"):
+        getattr(L, name).restype = ctypes.c_int
+    L.strumpack_serial_finter.argtypes = [i32p, i32p, i32p, _vp, _vp, _vp, _vp, _vp, i32p]
+    L.strumpack_serial_finter.restype = None
     return L
 
 

@@ -1,0 +1,392 @@
+// C ABI of the solver: handle API, iterative refinement, and the drop-in
+// replacement of the reference's STRUMPACK shim (strumpack_finter.c:9-126).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "wt_solver.h"
+#include "wt_vec.h"
+
+int64_t wt_nnz(int nx, int ny, int nz);
+template <typename R>
+int wt_launch_values(const wt_grid_params& p, const void* d_lam, const void* d_mu, const void* d_rho, void* d_vals,
+                     int* d_err, hipStream_t stream);
+
+struct wt_solver {
+    wt_solver_impl s;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    if (hipMalloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return WT_ERR_NOMEM;
+    }
+    return WT_OK;
+}
+
+int ensure_ir(wt_solver_impl& s, int nrhs)
+{
+    if (s.ir_cap >= nrhs) return WT_OK;
+    (void)hipFree(s.xb); (void)hipFree(s.rb); (void)hipFree(s.bb); (void)hipFree(s.partial); (void)hipFree(s.norms);
+    s.xb = s.rb = s.bb = nullptr;
+    s.partial = s.norms = nullptr;
+    s.ir_cap = 0;
+    int64_t nb = wt_residual_blocks(s.n);
+    int rc;
+    if ((rc = dalloc(&s.xb, s.n * nrhs)) || (rc = dalloc(&s.rb, s.n * nrhs)) || (rc = dalloc(&s.bb, s.n * nrhs)) ||
+        (rc = dalloc(&s.partial, nb * nrhs)) || (rc = dalloc(&s.norms, 2 * nrhs)))
+        return rc;
+    s.ir_cap = nrhs;
+    return WT_OK;
+}
+
+// Refinement on a row-major block of <= 64 columns already on the device:
+// x = 0, r = b; repeat { d = A_f^{-1} r (factor precision); x += d;
+// r = b - A x (fp64) } until max_j |r_j| / |b_j| <= tol or max_iter solves.
+template <typename R>
+int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b, cpx<double>* x, double tol,
+           int max_iter, wt_solve_stats* st)
+{
+    hipStream_t q = s.stream;
+    const int64_t n = s.n;
+    int rc;
+    const int64_t ncb = 1024;
+    if ((rc = wt_colnorm2(b, n, nrhs, s.partial, ncb < wt_residual_blocks(n) ? ncb : wt_residual_blocks(n),
+                          s.norms + nrhs, q)))
+        return rc;
+    WT_HIP(hipMemsetAsync(x, 0, n * nrhs * sizeof(cpx<double>), q));
+    const cpx<double>* r = b;
+    std::vector<double> h(2 * nrhs);
+    int it = 0;
+    double rel = INFINITY, first = INFINITY;
+    while (it < max_iter) {
+        if ((rc = wt_solver_prepare_solve<R>(s, num, nrhs))) return rc;
+        if ((rc = wt_to_elim<double, R>(r, s.iperm, num.y, n, nrhs, q))) return rc;
+        if ((rc = wt_solver_solve_block<R>(s, num, nrhs, nullptr))) return rc;
+        if ((rc = wt_from_elim<R, double>(num.y, s.iperm, x, n, nrhs, 1, q))) return rc;
+        if ((rc = wt_residual<double>(s.rowptr, s.colidx, s.vals, x, b, s.rb, s.partial, s.norms, n, nrhs, q)))
+            return rc;
+        WT_HIP(hipMemcpyAsync(h.data(), s.norms, 2 * nrhs * sizeof(double), hipMemcpyDeviceToHost, q));
+        WT_HIP(hipStreamSynchronize(q));
+        ++it;
+        rel = 0.0;
+        double absmax = 0.0;
+        for (int c = 0; c < nrhs; ++c) {
+            double rr = std::sqrt(h[c]), bn = std::sqrt(h[nrhs + c]);
+            double v = bn > 0 ? rr / bn : rr;
+            if (!(v <= rel)) rel = v;  // NaN propagates
+            if (rr > absmax) absmax = rr;
+        }
+        if (it == 1) first = rel;
+        r = s.rb;
+        if (st) st->abs_residual = absmax;
+        if (rel <= tol) break;
+    }
+    if (st) {
+        st->iterations = it;
+        st->nrhs = nrhs;
+        st->rel_residual = rel;
+        st->rel_residual_first = first;
+    }
+    return std::isfinite(rel) ? WT_OK : WT_ERR_PIVOT;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int wt_solver_create(int nx, int ny, int nz, int leaf_nodes, wt_solver** out)
+{
+    if (!out) return WT_ERR_ARG;
+    *out = nullptr;
+    wt_solver* h = new (std::nothrow) wt_solver();
+    if (!h) return WT_ERR_NOMEM;
+    int rc = wt_solver_init(h->s, nx, ny, nz, leaf_nodes > 0 ? leaf_nodes : 64);
+    if (rc) {
+        wt_solver_free(h->s);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return WT_OK;
+}
+
+int wt_solver_destroy(wt_solver* h)
+{
+    if (!h) return WT_OK;
+    wt_solver_free(h->s);
+    delete h;
+    return WT_OK;
+}
+
+int wt_solver_info(const wt_solver* h, int64_t* info, double* flops)
+{
+    if (!h || !info) return WT_ERR_ARG;
+    const wt_symbolic& S = h->s.S;
+    info[0] = S.n;
+    info[1] = h->s.nnz;
+    info[2] = (int64_t)S.fronts.size();
+    info[3] = (int64_t)S.levels.size();
+    info[4] = S.max_p;
+    info[5] = S.max_u;
+    info[6] = S.factor_len;
+    info[7] = S.cb_len[0] + S.cb_len[1];
+    if (flops) *flops = S.flops;
+    return WT_OK;
+}
+
+int64_t wt_solver_bytes(const wt_solver* h, int factor_prec, int nrhs)
+{
+    if (!h) return -1;
+    const wt_symbolic& S = h->s.S;
+    const int64_t cs = factor_prec == WT_PREC_F32 ? 8 : 16;
+    int64_t b = 0;
+    b += (S.n + 1) * 4 + h->s.nnz * (4 + 8 + 16);                   // pattern, scatter map, fp64 values
+    b += S.n * 8 + (int64_t)S.upd_idx.size() * 8 + (int64_t)S.relmap.size() * 4;
+    b += (int64_t)S.fronts.size() * (int64_t)sizeof(wt_dfront) + 3 * S.n / 3 * 40;  // model
+    b += (S.factor_len + S.cb_len[0] + S.cb_len[1] + S.tile_len) * cs + S.piv_len * 4;
+    b += (S.n + S.w_len[0] + S.w_len[1]) * nrhs * cs;                 // solve blocks
+    b += 3 * S.n * (int64_t)(nrhs < 64 ? nrhs : 64) * 16;             // refinement blocks
+    return b;
+}
+
+int wt_solver_set_model(wt_solver* h, const void* lam, const void* mu, const double* rho, int on_device)
+{
+    if (!h || !lam || !mu || !rho) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    const int64_t nodes = s.n / 3;
+    int rc;
+    if (!s.lam && ((rc = dalloc(&s.lam, nodes)) || (rc = dalloc(&s.mu, nodes)) || (rc = dalloc(&s.rho, nodes))))
+        return rc;
+    hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    WT_HIP(hipMemcpyAsync(s.lam, lam, nodes * 16, k, s.stream));
+    WT_HIP(hipMemcpyAsync(s.mu, mu, nodes * 16, k, s.stream));
+    WT_HIP(hipMemcpyAsync(s.rho, rho, nodes * 8, k, s.stream));
+    WT_HIP(hipStreamSynchronize(s.stream));
+    return WT_OK;
+}
+
+int wt_solver_assemble(wt_solver* h, const wt_grid_params* p)
+{
+    if (!h || !p) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    if (!s.lam) return WT_ERR_STATE;
+    if (p->nx != s.S.nx || p->ny != s.S.ny || p->nz != s.S.nz) return WT_ERR_ARG;
+    int rc;
+    if (!s.vals && (rc = dalloc(&s.vals, s.nnz))) return rc;
+    WT_HIP(hipMemsetAsync(s.d_err, 0, sizeof(int), s.stream));
+    if ((rc = wt_launch_values<double>(*p, s.lam, s.mu, s.rho, s.vals, s.d_err, s.stream))) return rc;
+    int he = 0;
+    WT_HIP(hipMemcpyAsync(&he, s.d_err, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    WT_HIP(hipStreamSynchronize(s.stream));
+    if (he) return WT_ERR_ARG;
+    s.have_vals = true;
+    return WT_OK;
+}
+
+int wt_solver_set_values(wt_solver* h, const void* vals, int prec, int on_device)
+{
+    if (!h || !vals) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    int rc;
+    if (!s.vals && (rc = dalloc(&s.vals, s.nnz))) return rc;
+    hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (prec == WT_PREC_F64) {
+        WT_HIP(hipMemcpyAsync(s.vals, vals, s.nnz * 16, k, s.stream));
+    } else if (prec == WT_PREC_F32) {
+        cpx<float>* t = nullptr;
+        if ((rc = dalloc(&t, s.nnz))) return rc;
+        rc = hipMemcpyAsync(t, vals, s.nnz * 8, k, s.stream) == hipSuccess ? WT_OK : WT_ERR_HIP;
+        if (!rc) rc = wt_widen_c64(t, s.vals, s.nnz, s.stream);
+        if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
+        (void)hipFree(t);
+        if (rc) return rc;
+    } else {
+        return WT_ERR_ARG;
+    }
+    WT_HIP(hipStreamSynchronize(s.stream));
+    s.have_vals = true;
+    return WT_OK;
+}
+
+int wt_solver_get_values(wt_solver* h, void* vals)
+{
+    if (!h || !vals || !h->s.have_vals) return WT_ERR_ARG;
+    WT_HIP(hipMemcpy(vals, h->s.vals, h->s.nnz * 16, hipMemcpyDeviceToHost));
+    return WT_OK;
+}
+
+int wt_solver_factor(wt_solver* h, int factor_prec)
+{
+    if (!h) return WT_ERR_ARG;
+    if (factor_prec == WT_PREC_F32) return wt_solver_factor_run<float>(h->s, h->s.f32);
+    if (factor_prec == WT_PREC_F64) return wt_solver_factor_run<double>(h->s, h->s.f64);
+    return WT_ERR_ARG;
+}
+
+int wt_solver_release(wt_solver* h, int factor_prec)
+{
+    if (!h) return WT_ERR_ARG;
+    if (factor_prec == WT_PREC_F32) h->s.f32.release();
+    else if (factor_prec == WT_PREC_F64) h->s.f64.release();
+    else return WT_ERR_ARG;
+    return WT_OK;
+}
+
+int wt_solver_solve(wt_solver* h, int factor_prec, int nrhs, const void* b, void* x, int on_device, double tol,
+                    int max_iter, wt_solve_stats* st)
+{
+    if (!h || !b || !x || nrhs < 1 || max_iter < 1) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    if (!s.have_vals) return WT_ERR_STATE;
+    bool f32 = factor_prec == WT_PREC_F32;
+    if (!f32 && factor_prec != WT_PREC_F64) return WT_ERR_ARG;
+    if ((f32 && !s.f32.ready) || (!f32 && !s.f64.ready)) return WT_ERR_STATE;
+    const int chunk = nrhs < 64 ? nrhs : 64;
+    int rc;
+    if ((rc = ensure_ir(s, chunk))) return rc;
+    wt_solve_stats agg{};
+    agg.rel_residual = 0;
+    agg.rel_residual_first = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+        const int nc = nrhs - c0 < chunk ? nrhs - c0 : chunk;
+        // gather this chunk's columns (row-major n x nrhs) into a dense n x nc block
+        const cpx<double>* src = (const cpx<double>*)b;
+        cpx<double>* dstx = (cpx<double>*)x;
+        const cpx<double>* bblk;
+        cpx<double>* xblk;
+        if (on_device && nc == nrhs) {
+            bblk = src;
+            xblk = dstx;
+        } else {
+            WT_HIP(hipMemcpy2DAsync(s.bb, nc * 16, src + c0, (size_t)nrhs * 16, nc * 16, s.n,
+                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.stream));
+            bblk = s.bb;
+            xblk = s.xb;
+        }
+        wt_solve_stats cs{};
+        rc = f32 ? refine<float>(s, s.f32, nc, bblk, xblk, tol, max_iter, &cs)
+                 : refine<double>(s, s.f64, nc, bblk, xblk, tol, max_iter, &cs);
+        if (rc) return rc;
+        if (xblk == s.xb)
+            WT_HIP(hipMemcpy2DAsync(dstx + c0, (size_t)nrhs * 16, s.xb, nc * 16, nc * 16, s.n,
+                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s.stream));
+        WT_HIP(hipStreamSynchronize(s.stream));
+        agg.iterations = cs.iterations > agg.iterations ? cs.iterations : agg.iterations;
+        agg.rel_residual = cs.rel_residual > agg.rel_residual ? cs.rel_residual : agg.rel_residual;
+        agg.rel_residual_first =
+            cs.rel_residual_first > agg.rel_residual_first ? cs.rel_residual_first : agg.rel_residual_first;
+        agg.abs_residual = cs.abs_residual > agg.abs_residual ? cs.abs_residual : agg.abs_residual;
+    }
+    agg.nrhs = nrhs;
+    agg.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = agg;
+    return WT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in for strumpack_serial_finter (strumpack_finter.c:9-120): one static
+// solver; job 1 = analyse the CSR pattern (1-based, left untouched), job 2 =
+// numeric factorization of the complex64 values, job 3 = solve one
+// complex64 right-hand side with refinement to the reference's tolerances
+// (rctol 1e-4, actol 1e-10, strumpack_finter.c:39-40), job -1 = release.
+// The grid is recovered from the pattern; any other pattern is rejected.
+
+static wt_solver* g_legacy = nullptr;
+static bool g_factored = false;
+
+static int grid_from_pattern(int n, int nnz, const int* irptr, const int* jcptr, int* nx, int* ny, int* nz)
+{
+    if (n < 24 || n % 3 || irptr[0] != 1 || irptr[1] - irptr[0] != 24) return WT_ERR_PATTERN;
+    const int z = (jcptr[6] - 1) / 3, xz = (jcptr[12] - 1) / 3;
+    if (z < 2 || xz % z || xz / z < 2 || (n / 3) % xz) return WT_ERR_PATTERN;
+    *nz = z;
+    *nx = xz / z;
+    *ny = (n / 3) / xz;
+    if (*ny < 2 || wt_nnz(*nx, *ny, *nz) != nnz) return WT_ERR_PATTERN;
+    std::vector<int32_t> r(n + 1), c(nnz);
+    if (wt_graph(*nx, *ny, *nz, 1, r.data(), c.data()) != WT_OK) return WT_ERR_HIP;
+    if (std::memcmp(r.data(), irptr, (size_t)(n + 1) * 4) || std::memcmp(c.data(), jcptr, (size_t)nnz * 4))
+        return WT_ERR_PATTERN;
+    return WT_OK;
+}
+
+void strumpack_serial_finter(int* job, int* nrows, int* nnz, int* irptr, int* jcptr, void* a, void* rhs, void* sol,
+                             int* ierr)
+{
+    const char* fcnm = "strumpack_serial_finter";
+    *ierr = 0;
+    if (*job == 1) {
+        if (g_legacy) {
+            printf("%s: Error already initialized!\n", fcnm);
+            *ierr = 1;
+            return;
+        }
+        int nx, ny, nz;
+        if (grid_from_pattern(*nrows, *nnz, irptr, jcptr, &nx, &ny, &nz) != WT_OK) {
+            printf("%s: Error reordering matrix!\n", fcnm);
+            *ierr = 1;
+            return;
+        }
+        if (wt_solver_create(nx, ny, nz, 64, &g_legacy) != WT_OK) {
+            printf("%s: Strumpack error!\n", fcnm);
+            *ierr = 1;
+            return;
+        }
+        g_factored = false;
+    } else if (*job == 2) {
+        g_factored = false;
+        if (!g_legacy) {
+            printf("%s: Error not initialized!\n", fcnm);
+            *ierr = 1;
+            return;
+        }
+        if (wt_solver_set_values(g_legacy, a, WT_PREC_F32, 0) != WT_OK ||
+            wt_solver_factor(g_legacy, WT_PREC_F32) != WT_OK) {
+            printf("%s: Error factoring matrix!\n", fcnm);
+            *ierr = 1;
+            return;
+        }
+        g_factored = true;
+    } else if (*job == 3) {
+        if (!g_legacy || !g_factored) {
+            printf("%s: Error matrix not factorized!\n", fcnm);
+            *ierr = 1;
+            return;
+        }
+        const int64_t n = *nrows;
+        std::vector<cpx<double>> b(n), x(n);
+        const cpx<float>* r32 = (const cpx<float>*)rhs;
+        for (int64_t i = 0; i < n; ++i) b[i] = cpx<double>(r32[i].re, r32[i].im);
+        wt_solve_stats st{};
+        int rc = wt_solver_solve(g_legacy, WT_PREC_F32, 1, b.data(), x.data(), 0, 1e-4, 10, &st);
+        if (rc != WT_OK || (st.rel_residual > 1e-4 && st.abs_residual > 1e-10)) {
+            printf("%s: Error in solution phase1\n", fcnm);
+            *ierr = 1;
+            if (rc != WT_OK) return;
+        }
+        cpx<float>* s32 = (cpx<float>*)sol;
+        for (int64_t i = 0; i < n; ++i) s32[i] = cpx<float>((float)x[i].re, (float)x[i].im);
+    } else if (*job == -1) {
+        // the reference driver passes already-deallocated arrays here
+        // (elwave.f90:184-190): they are not touched
+        wt_solver_destroy(g_legacy);
+        g_legacy = nullptr;
+        g_factored = false;
+    }
+}
+
+int strumpack_parallel_finter(void) { return 0; }
+
+}  // extern "C"

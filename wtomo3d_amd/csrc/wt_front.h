@@ -1,0 +1,33 @@
+// Front-level kernels of the multifrontal factorization and solve.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+#include "wt_common.h"
+
+// Diagonal tile width: pivoting is restricted to rows of one tile.
+#define WT_NB 64
+
+// Device copy of the per-front offsets of wt_front (wt_symbolic.h).
+struct wt_dfront {
+    int64_t p, u, pstart;
+    int64_t Loff, Uoff, CBoff, moff, uoff, tileoff, pivoff, Woff;
+    int32_t parent, level, slot, pad;
+};
+
+template <typename R>
+int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s);
+template <typename Ri, typename R>
+int wt_scatter(const cpx<Ri>* vals, const int64_t* dst, int64_t nnz, cpx<R>* arena, hipStream_t s);
+template <typename R>
+int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
+                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, hipStream_t s);
+template <typename R>
+int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t kb, cpx<R>* factor, cpx<R>* tiles,
+                  int32_t* perm, int* status, hipStream_t s);
+template <typename R>
+int wt_solve_gather(const wt_dfront* fr, const int32_t* kids, int nkids, const int32_t* relmap,
+                    const cpx<R>* w_child, cpx<R>* w_parent, cpx<R>* y, int64_t n, int nrhs, hipStream_t s);
+template <typename R>
+int wt_solve_gather_upd(const wt_dfront* fr, const int32_t* list, int nlist, const int64_t* upd_idx,
+                        const cpx<R>* x, cpx<R>* work, int64_t n, int nrhs, hipStream_t s);

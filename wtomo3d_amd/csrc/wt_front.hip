@@ -1,0 +1,318 @@
+// Front-level kernels of the multifrontal factorization and solve.
+//
+// Replaces the numeric phase STRUMPACK runs in job 2 (strumpack_finter.c:
+// 68-92, STRUMPACK_factor) and the dense parts of job 3 (STRUMPACK_solve).
+// Each front stores [L11\U11; L21] as a (p+u) x p column block and U12 as a
+// p x u block; the u x u contribution block (CB) lives in an arena indexed by
+// tree level parity.  Partial pivoting is done inside each nb x nb diagonal
+// tile; the tile's unit-lower and upper inverses are kept so the panel
+// solves and the forward/back substitutions run as MFMA GEMMs.
+#include "wt_front.h"
+
+namespace {
+
+template <typename R>
+__global__ void fill_zero_kernel(cpx<R>* __restrict__ p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = cpx<R>(R(0), R(0));
+}
+
+template <typename Ri, typename R>
+__global__ void scatter_kernel(const cpx<Ri>* __restrict__ vals, const int64_t* __restrict__ dst, int64_t nnz,
+                               cpx<R>* __restrict__ arena)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < nnz; i += stride) {
+        cpx<Ri> v = vals[i];
+        arena[dst[i]] = cpx<R>((R)v.re, (R)v.im);
+    }
+}
+
+constexpr int EA_COLS = 8;  // CB columns per extend-add block
+
+// Adds the contribution blocks of the listed children into their parents.
+// At most one child per parent is listed, so the adds do not race.
+template <typename R>
+__global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __restrict__ fr,
+                                                         const int32_t* __restrict__ kids,
+                                                         const int32_t* __restrict__ relmap,
+                                                         const cpx<R>* __restrict__ cb_child,
+                                                         cpx<R>* __restrict__ factor,
+                                                         cpx<R>* __restrict__ cb_parent)
+{
+    const wt_dfront C = fr[kids[blockIdx.x]];
+    const int64_t uc = C.u;
+    const int64_t j0 = (int64_t)blockIdx.y * EA_COLS;
+    if (j0 >= uc) return;
+    const wt_dfront P = fr[C.parent];
+    const int32_t* map = relmap + C.moff;
+    const cpx<R>* cb = cb_child + C.CBoff;
+    cpx<R>* Lp = factor + P.Loff;
+    cpx<R>* Up = factor + P.Uoff;
+    cpx<R>* Cp = cb_parent + P.CBoff;
+    const int64_t pp = P.p, ld = P.p + P.u, pu = P.u;
+    const int64_t j1 = j0 + EA_COLS < uc ? j0 + EA_COLS : uc;
+    for (int64_t j = j0; j < j1; ++j) {
+        const int64_t s = map[j];
+        for (int64_t i = threadIdx.x; i < uc; i += blockDim.x) {
+            const int64_t r = map[i];
+            const cpx<R> v = cb[j * uc + i];
+            cpx<R>* t;
+            if (s < pp) t = Lp + s * ld + r;
+            else if (r < pp) t = Up + (s - pp) * pp + r;
+            else t = Cp + (s - pp) * pu + (r - pp);
+            *t = *t + v;
+        }
+    }
+}
+
+// LU with partial pivoting of the nb x nb diagonal tile starting at pivot kb
+// of every listed front, plus the inverses of its unit-lower and upper
+// factors.  perm[i] = tile row that ends at position i.
+template <typename R, int NB>
+__global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restrict__ fr,
+                                                        const int32_t* __restrict__ list, int64_t kb,
+                                                        cpx<R>* __restrict__ factor, cpx<R>* __restrict__ tiles,
+                                                        int32_t* __restrict__ perm, int* __restrict__ status)
+{
+    __shared__ cpx<R> T[NB][NB + 1];  // T[col][row]
+    __shared__ cpx<R> X[NB][NB + 1];
+    __shared__ int s_piv;
+    __shared__ int s_perm[NB];
+    const wt_dfront F = fr[list[blockIdx.x]];
+    if (kb >= F.p) return;
+    const int w = (int)(F.p - kb < NB ? F.p - kb : NB);
+    const int64_t ld = F.p + F.u;
+    cpx<R>* base = factor + F.Loff + kb * ld + kb;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < w * w; e += blockDim.x) {
+        int c = e / w, r = e - c * w;
+        T[c][r] = base[(int64_t)c * ld + r];
+    }
+    if (tid < NB) s_perm[tid] = tid;
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        if (tid < 64) {
+            // pivot: largest |re| + |im| in column j, rows j..w-1 (first on ties)
+            R best = R(-1);
+            int bi = j;
+            for (int i = j + tid; i < w; i += 64) {
+                R a = fabs(T[j][i].re) + fabs(T[j][i].im);
+                if (a > best) { best = a; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                R ob = __shfl_xor(best, off);
+                int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (tid == 0) {
+                s_piv = bi;
+                int t0 = s_perm[j];
+                s_perm[j] = s_perm[bi];
+                s_perm[bi] = t0;
+            }
+        }
+        __syncthreads();
+        const int pv = s_piv;
+        if (pv != j)
+            for (int c = tid; c < w; c += blockDim.x) {
+                cpx<R> t0 = T[c][j];
+                T[c][j] = T[c][pv];
+                T[c][pv] = t0;
+            }
+        __syncthreads();
+        cpx<R> d = T[j][j];
+        if (d.re == R(0) && d.im == R(0)) {
+            if (tid == 0) atomicOr(status, 1);
+            d = cpx<R>(R(1e-30), R(0));
+            if (tid == 0) T[j][j] = d;
+        }
+        const cpx<R> dinv = cdiv(cpx<R>(R(1), R(0)), d);
+        __syncthreads();
+        for (int i = j + 1 + tid; i < w; i += blockDim.x) T[j][i] = T[j][i] * dinv;
+        __syncthreads();
+        const int m = w - j - 1;
+        for (int e = tid; e < m * m; e += blockDim.x) {
+            int c = j + 1 + e / m, i = j + 1 + e % m;
+            T[c][i] = T[c][i] - T[j][i] * T[c][j];
+        }
+        __syncthreads();
+    }
+    // write the factors back and the row permutation
+    for (int e = tid; e < w * w; e += blockDim.x) {
+        int c = e / w, r = e - c * w;
+        base[(int64_t)c * ld + r] = T[c][r];
+    }
+    if (tid < w) perm[F.pivoff + kb + tid] = s_perm[tid];
+    // inverse of the unit-lower factor, one column per thread
+    cpx<R>* Li = tiles + F.tileoff + (kb / NB) * 2 * NB * NB;
+    cpx<R>* Ui = Li + NB * NB;
+    if (tid < w) {
+        const int c = tid;
+        for (int i = 0; i < c; ++i) X[c][i] = cpx<R>(R(0), R(0));
+        X[c][c] = cpx<R>(R(1), R(0));
+        for (int i = c + 1; i < w; ++i) {
+            cpx<R> s(R(0), R(0));
+            for (int k = c; k < i; ++k) s = s + T[k][i] * X[c][k];
+            X[c][i] = -s;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < w * w; e += blockDim.x) {
+        int c = e / w, r = e - c * w;
+        Li[(int64_t)c * NB + r] = X[c][r];
+    }
+    __syncthreads();
+    // inverse of the upper factor
+    if (tid < w) {
+        const int c = tid;
+        for (int i = c + 1; i < w; ++i) X[c][i] = cpx<R>(R(0), R(0));
+        X[c][c] = cdiv(cpx<R>(R(1), R(0)), T[c][c]);
+        for (int i = c - 1; i >= 0; --i) {
+            cpx<R> s(R(0), R(0));
+            for (int k = i + 1; k <= c; ++k) s = s + T[k][i] * X[c][k];
+            X[c][i] = -cdiv(s, T[i][i]);
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < w * w; e += blockDim.x) {
+        int c = e / w, r = e - c * w;
+        Ui[(int64_t)c * NB + r] = X[c][r];
+    }
+}
+
+// ---- solve-side gathers / scatters (block of nrhs columns) ----
+
+// y (elimination order, ld = n) += children's update rows that land on the
+// parent's pivots; the parent's own update block W gets the rest.
+template <typename R>
+__global__ __launch_bounds__(256) void solve_gather_kernel(const wt_dfront* __restrict__ fr,
+                                                           const int32_t* __restrict__ kids,
+                                                           const int32_t* __restrict__ relmap,
+                                                           const cpx<R>* __restrict__ w_child,
+                                                           cpx<R>* __restrict__ w_parent, cpx<R>* __restrict__ y,
+                                                           int64_t n, int nrhs)
+{
+    const wt_dfront C = fr[kids[blockIdx.x]];
+    const wt_dfront P = fr[C.parent];
+    const int32_t* map = relmap + C.moff;
+    const int64_t uc = C.u;
+    const int r = blockIdx.y;
+    if (r >= nrhs) return;
+    const cpx<R>* wc = w_child + C.Woff * nrhs + (int64_t)r * uc;
+    cpx<R>* wp = w_parent + P.Woff * nrhs + (int64_t)r * P.u;
+    for (int64_t i = threadIdx.x; i < uc; i += blockDim.x) {
+        int64_t pos = map[i];
+        cpx<R> v = wc[i];
+        if (pos < P.p) {
+            cpx<R>* t = y + (int64_t)r * n + P.pstart + pos;
+            *t = *t + v;
+        } else {
+            cpx<R>* t = wp + (pos - P.p);
+            *t = *t + v;
+        }
+    }
+}
+
+// X_U (u x nrhs work block) = x[upd_idx of the front] for the back solve.
+template <typename R>
+__global__ __launch_bounds__(256) void solve_gather_upd_kernel(const wt_dfront* __restrict__ fr,
+                                                               const int32_t* __restrict__ list,
+                                                               const int64_t* __restrict__ upd_idx,
+                                                               const cpx<R>* __restrict__ x,
+                                                               cpx<R>* __restrict__ work, int64_t n, int nrhs)
+{
+    const wt_dfront F = fr[list[blockIdx.x]];
+    const int r = blockIdx.y;
+    if (r >= nrhs) return;
+    const int64_t* idx = upd_idx + F.uoff;
+    cpx<R>* wk = work + F.Woff * nrhs + (int64_t)r * F.u;
+    for (int64_t i = threadIdx.x; i < F.u; i += blockDim.x) wk[i] = x[(int64_t)r * n + idx[i]];
+}
+
+}  // namespace
+
+template <typename R>
+int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_zero_kernel<R>, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename Ri, typename R>
+int wt_scatter(const cpx<Ri>* vals, const int64_t* dst, int64_t nnz, cpx<R>* arena, hipStream_t s)
+{
+    if (nnz <= 0) return WT_OK;
+    int64_t blocks = (nnz + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((scatter_kernel<Ri, R>), dim3((unsigned)blocks), dim3(256), 0, s, vals, dst, nnz, arena);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
+                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, hipStream_t s)
+{
+    if (nkids <= 0 || max_u <= 0) return WT_OK;
+    dim3 grid(nkids, (unsigned)((max_u + EA_COLS - 1) / EA_COLS));
+    hipLaunchKernelGGL(extend_add_kernel<R>, grid, dim3(256), 0, s, fr, kids, relmap, cb_child, factor, cb_parent);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t kb, cpx<R>* factor, cpx<R>* tiles,
+                  int32_t* perm, int* status, hipStream_t s)
+{
+    if (nlist <= 0) return WT_OK;
+    hipLaunchKernelGGL((diag_tile_kernel<R, WT_NB>), dim3(nlist), dim3(256), 0, s, fr, list, kb, factor, tiles,
+                       perm, status);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_solve_gather(const wt_dfront* fr, const int32_t* kids, int nkids, const int32_t* relmap,
+                    const cpx<R>* w_child, cpx<R>* w_parent, cpx<R>* y, int64_t n, int nrhs, hipStream_t s)
+{
+    if (nkids <= 0) return WT_OK;
+    hipLaunchKernelGGL(solve_gather_kernel<R>, dim3(nkids, nrhs), dim3(256), 0, s, fr, kids, relmap, w_child,
+                       w_parent, y, n, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_solve_gather_upd(const wt_dfront* fr, const int32_t* list, int nlist, const int64_t* upd_idx,
+                        const cpx<R>* x, cpx<R>* work, int64_t n, int nrhs, hipStream_t s)
+{
+    if (nlist <= 0) return WT_OK;
+    hipLaunchKernelGGL(solve_gather_upd_kernel<R>, dim3(nlist, nrhs), dim3(256), 0, s, fr, list, upd_idx, x, work,
+                       n, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+#define WT_INST(R)                                                                                              \
+    template int wt_fill_zero<R>(cpx<R>*, int64_t, hipStream_t);                                                \
+    template int wt_extend_add<R>(const wt_dfront*, const int32_t*, int, int64_t, const int32_t*, const cpx<R>*, \
+                                  cpx<R>*, cpx<R>*, hipStream_t);                                               \
+    template int wt_diag_tiles<R>(const wt_dfront*, const int32_t*, int, int64_t, cpx<R>*, cpx<R>*, int32_t*,   \
+                                  int*, hipStream_t);                                                           \
+    template int wt_solve_gather<R>(const wt_dfront*, const int32_t*, int, const int32_t*, const cpx<R>*,       \
+                                    cpx<R>*, cpx<R>*, int64_t, int, hipStream_t);                               \
+    template int wt_solve_gather_upd<R>(const wt_dfront*, const int32_t*, int, const int64_t*, const cpx<R>*,   \
+                                        cpx<R>*, int64_t, int, hipStream_t);
+WT_INST(float)
+WT_INST(double)
+template int wt_scatter<double, float>(const cpx<double>*, const int64_t*, int64_t, cpx<float>*, hipStream_t);
+template int wt_scatter<double, double>(const cpx<double>*, const int64_t*, int64_t, cpx<double>*, hipStream_t);
+template int wt_scatter<float, float>(const cpx<float>*, const int64_t*, int64_t, cpx<float>*, hipStream_t);

@@ -1,0 +1,234 @@
+// Batched complex GEMM for the dense front kernels, on CDNA4 matrix cores.
+//
+// C (op)= A * B for many independent problems in one launch (a flattened
+// list of output tiles).  Complex products use the 3M form: three real
+// MFMA products T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi) with fp32
+// (or fp64) accumulation, then Re = T1 - T2, Im = T3 - T1 - T2.
+//
+// The MFMA computes D = X Y with X taken from B^T and Y from A^T, so D is a
+// tile of C^T: each lane's accumulators then run along the rows of the
+// column-major C, and the epilogue stores are contiguous across lanes.
+// 256 threads = 4 waves in a 2 x 2 grid; fp32 uses v_mfma_f32_32x32x2f32
+// (block 128 x 128, wave 64 x 64), fp64 uses v_mfma_f64_16x16x4f64 (block
+// 128 x 64, wave 64 x 32).  K is consumed in slices of 8 complex values,
+// staged through double-buffered LDS as separate real and imaginary planes.
+#include "wt_gemm.h"
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct MF;
+template <>
+struct MF<float> {
+    static constexpr int MT = 32, KS = 2;
+    typedef f32x16 acc_t;
+    static constexpr int NACC = 16;
+    __device__ static acc_t mfma(float x, float y, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, c, 0, 0, 0);
+    }
+    // accumulator v of lane l holds D[drow][dcol]
+    __device__ static int drow(int l, int v) { return (v >> 2) * 8 + (l >> 5) * 4 + (v & 3); }
+    __device__ static int dcol(int l) { return l & 31; }
+    // operand element of lane l: row/col index and k index inside the step
+    __device__ static int orow(int l) { return l & 31; }
+    __device__ static int ok(int l) { return l >> 5; }
+};
+template <>
+struct MF<double> {
+    static constexpr int MT = 16, KS = 4;
+    typedef f64x4 acc_t;
+    static constexpr int NACC = 4;
+    __device__ static acc_t mfma(double x, double y, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+    __device__ static int drow(int l, int v) { return v * 4 + (l >> 4); }
+    __device__ static int dcol(int l) { return l & 15; }
+    __device__ static int orow(int l) { return l & 15; }
+    __device__ static int ok(int l) { return l >> 4; }
+};
+
+constexpr int NT = 256;  // threads per block
+constexpr int BK = 8;    // complex k per slice
+
+template <typename R>
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kernel(const wt_gemm_prob<R>* __restrict__ probs,
+                                                            const int64_t* __restrict__ tile_start,
+                                                            int nprob, int64_t ntiles)
+{
+    typedef MF<R> M_;
+    constexpr int BM = wt_gemm_tile<R>::BM, BN = wt_gemm_tile<R>::BN;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int SM = WM / M_::MT, SN = WN / M_::MT;
+    constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
+    constexpr int TPA = BM / EA, TPB = BK / EB;  // threads per column of A / B
+    constexpr int PAD = 4;
+
+    __shared__ R sAr[2][BK][BM + PAD], sAi[2][BK][BM + PAD];
+    __shared__ R sBr[2][BK][BN + PAD], sBi[2][BK][BN + PAD];
+
+    // tile id with an XCD-contiguous remap (blocks go round-robin to 8 XCDs)
+    int64_t b = blockIdx.x;
+    {
+        int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, i = b / 8;
+        b = x * q + (x < r ? x : r) + i;
+    }
+    // problem of this tile
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const wt_gemm_prob<R> P = probs[lo];
+    int64_t t = b - tile_start[lo];
+    const int tiles_m = (P.M + BM - 1) / BM, tiles_n = (P.N + BN - 1) / BN;
+    // grouped order: GM tile-rows at a time share B columns in cache
+    constexpr int GM = 8;
+    int64_t grp = t / ((int64_t)GM * tiles_n);
+    int first_m = (int)(grp * GM);
+    int gm = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+    int64_t tin = t - grp * (int64_t)GM * tiles_n;
+    const int tm = first_m + (int)(tin % gm), tn = (int)(tin / gm);
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+
+    typename M_::acc_t acc[3][SM][SN];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < SM; ++i)
+#pragma unroll
+            for (int j = 0; j < SN; ++j)
+#pragma unroll
+                for (int v = 0; v < M_::NACC; ++v) acc[c][i][j][v] = R(0);
+
+    const int K = P.K;
+    const int nkt = (K + BK - 1) / BK;
+    // per-thread load coordinates
+    const int ka = tid / TPA, ma = (tid % TPA) * EA;
+    const int nbb = tid / TPB, kb = (tid % TPB) * EB;
+    cpx<R> ra[EA], rb[EB];
+
+    auto load = [&](int kt) {
+        const int k0 = kt * BK;
+        const bool fullA = (m0 + BM <= P.M) && (k0 + BK <= K);
+        const int kk = k0 + ka;
+        const cpx<R>* pa = P.A + (int64_t)kk * P.lda + m0 + ma;
+        if (fullA) {
+#pragma unroll
+            for (int e = 0; e < EA; ++e) ra[e] = pa[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < EA; ++e)
+                ra[e] = (kk < K && m0 + ma + e < P.M) ? pa[e] : cpx<R>(R(0), R(0));
+        }
+        const int nn = n0 + nbb;
+        const bool fullB = (n0 + BN <= P.N) && (k0 + BK <= K) && !P.bperm;
+        if (fullB) {
+            const cpx<R>* pb = P.B + (int64_t)nn * P.ldb + k0 + kb;
+#pragma unroll
+            for (int e = 0; e < EB; ++e) rb[e] = pb[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < EB; ++e) {
+                int k = k0 + kb + e;
+                if (k < K && nn < P.N) {
+                    int kr = P.bperm ? P.bperm[k] : k;
+                    rb[e] = P.B[(int64_t)nn * P.ldb + kr];
+                } else {
+                    rb[e] = cpx<R>(R(0), R(0));
+                }
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) { sAr[buf][ka][ma + e] = ra[e].re; sAi[buf][ka][ma + e] = ra[e].im; }
+#pragma unroll
+        for (int e = 0; e < EB; ++e) { sBr[buf][kb + e][nbb] = rb[e].re; sBi[buf][kb + e][nbb] = rb[e].im; }
+    };
+
+    if (nkt > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    const int orow = M_::orow(lane), okk = M_::ok(lane);
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nkt) load(kt + 1);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += M_::KS) {
+            R xr[SN], xi[SN], xs[SN], yr[SM], yi[SM], ys[SM];
+#pragma unroll
+            for (int j = 0; j < SN; ++j) {
+                xr[j] = sBr[cur][ks + okk][wn0 + j * M_::MT + orow];
+                xi[j] = sBi[cur][ks + okk][wn0 + j * M_::MT + orow];
+                xs[j] = xr[j] + xi[j];
+            }
+#pragma unroll
+            for (int i = 0; i < SM; ++i) {
+                yr[i] = sAr[cur][ks + okk][wm0 + i * M_::MT + orow];
+                yi[i] = sAi[cur][ks + okk][wm0 + i * M_::MT + orow];
+                ys[i] = yr[i] + yi[i];
+            }
+#pragma unroll
+            for (int i = 0; i < SM; ++i)
+#pragma unroll
+                for (int j = 0; j < SN; ++j) {
+                    acc[0][i][j] = M_::mfma(xr[j], yr[i], acc[0][i][j]);
+                    acc[1][i][j] = M_::mfma(xi[j], yi[i], acc[1][i][j]);
+                    acc[2][i][j] = M_::mfma(xs[j], ys[i], acc[2][i][j]);
+                }
+        }
+        if (kt + 1 < nkt) store(cur ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue
+    const int mode = P.mode;
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+        for (int j = 0; j < SN; ++j)
+#pragma unroll
+            for (int v = 0; v < M_::NACC; ++v) {
+                const int m = m0 + wm0 + i * M_::MT + M_::dcol(lane);
+                const int n = n0 + wn0 + j * M_::MT + M_::drow(lane, v);
+                if (m < P.M && n < P.N) {
+                    R t1 = acc[0][i][j][v], t2 = acc[1][i][j][v], t3 = acc[2][i][j][v];
+                    R re = t1 - t2, im = (t3 - t1) - t2;
+                    cpx<R>* pc = P.C + (int64_t)n * P.ldc + m;
+                    if (mode == WT_GEMM_SET) {
+                        *pc = cpx<R>(re, im);
+                    } else {
+                        cpx<R> c = *pc;
+                        if (mode == WT_GEMM_SUB) *pc = cpx<R>(c.re - re, c.im - im);
+                        else *pc = cpx<R>(c.re + re, c.im + im);
+                    }
+                }
+            }
+}
+
+}  // namespace
+
+template <typename R>
+int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, int nprob, int64_t ntiles,
+                   hipStream_t stream)
+{
+    if (nprob <= 0 || ntiles <= 0) return WT_OK;
+    hipLaunchKernelGGL(gemm_batched_kernel<R>, dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                       d_tile_start, nprob, ntiles);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template int wt_gemm_launch<float>(const wt_gemm_prob<float>*, const int64_t*, int, int64_t, hipStream_t);
+template int wt_gemm_launch<double>(const wt_gemm_prob<double>*, const int64_t*, int, int64_t, hipStream_t);

@@ -1,0 +1,455 @@
+// Host orchestration of the multifrontal factorization and block solve.
+//
+// The elimination tree is processed one level at a time, bottom-up for the
+// factorization and the forward substitution, top-down for the back
+// substitution.  All fronts of a level advance together through their
+// nb-wide pivot tiles: one launch factors the diagonal tiles of every front,
+// one batched-GEMM launch forms the row and column panels, one more applies
+// the trailing update inside the pivot block, and a final launch per level
+// forms every contribution block CB = -L21 U12 (the large-K Schur GEMM).
+// Launch lists are built once per factorization buffer set and replayed.
+#include "wt_solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "wt_vec.h"
+
+int wt_launch_pattern(int nx, int ny, int nz, int base, int32_t* d_rowptr, int32_t* d_colidx, hipStream_t stream);
+int64_t wt_nnz(int nx, int ny, int nz);
+
+template <typename T>
+static int dalloc(T** p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    if (hipMalloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return WT_ERR_NOMEM;
+    }
+    return WT_OK;
+}
+template <typename T>
+static void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <typename R>
+int wt_gemm_plan<R>::upload()
+{
+    dfree(d_probs);
+    dfree(d_tiles);
+    if (dalloc(&d_probs, (int64_t)probs.size()) || dalloc(&d_tiles, (int64_t)tiles.size())) return WT_ERR_NOMEM;
+    if (!probs.empty())
+        WT_HIP(hipMemcpy(d_probs, probs.data(), probs.size() * sizeof(probs[0]), hipMemcpyHostToDevice));
+    if (!tiles.empty())
+        WT_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    return WT_OK;
+}
+template <typename R>
+void wt_gemm_plan<R>::release()
+{
+    dfree(d_probs);
+    dfree(d_tiles);
+    probs.clear();
+    tiles.clear();
+    launches.clear();
+}
+
+template <typename R>
+void wt_numeric<R>::release()
+{
+    dfree(factor);
+    dfree(cb[0]);
+    dfree(cb[1]);
+    dfree(tiles);
+    dfree(perm);
+    dfree(status);
+    dfree(y);
+    dfree(w[0]);
+    dfree(w[1]);
+    fplan.release();
+    splan.release();
+    fsteps.clear();
+    fwd.clear();
+    bwd.clear();
+    snrhs = 0;
+    ready = false;
+}
+
+int wt_solver_init(wt_solver_impl& s, int nx, int ny, int nz, int leaf_nodes)
+{
+    if (nx < 2 || ny < 2 || nz < 2) return WT_ERR_ARG;
+    WT_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.n = 3LL * nx * ny * nz;
+    s.nnz = wt_nnz(nx, ny, nz);
+    if (s.nnz >= (1LL << 31)) return WT_ERR_ARG;
+    int rc;
+    if ((rc = dalloc(&s.rowptr, s.n + 1)) || (rc = dalloc(&s.colidx, s.nnz))) return rc;
+    if ((rc = wt_launch_pattern(nx, ny, nz, 0, s.rowptr, s.colidx, s.stream))) return rc;
+    std::vector<int32_t> hr(s.n + 1), hc(s.nnz);
+    WT_HIP(hipMemcpyAsync(hr.data(), s.rowptr, (s.n + 1) * 4, hipMemcpyDeviceToHost, s.stream));
+    WT_HIP(hipMemcpyAsync(hc.data(), s.colidx, s.nnz * 4, hipMemcpyDeviceToHost, s.stream));
+    WT_HIP(hipStreamSynchronize(s.stream));
+    if ((rc = wt_symbolic_build(nx, ny, nz, leaf_nodes, WT_NB, hr.data(), hc.data(), s.S))) return rc;
+    hc.clear();
+    hc.shrink_to_fit();
+    const wt_symbolic& S = s.S;
+
+    // device front table
+    std::vector<wt_dfront> df(S.fronts.size());
+    for (size_t f = 0; f < S.fronts.size(); ++f) {
+        const wt_front& F = S.fronts[f];
+        wt_dfront& d = df[f];
+        d.p = F.p; d.u = F.u; d.pstart = F.pstart; d.Loff = F.Loff; d.Uoff = F.Uoff; d.CBoff = F.CBoff;
+        d.moff = F.moff; d.uoff = F.uoff; d.tileoff = F.tileoff; d.pivoff = F.pivoff; d.Woff = F.Woff;
+        d.parent = F.parent; d.level = F.level; d.slot = F.slot; d.pad = 0;
+    }
+    // per-level lists
+    const int nl = (int)S.levels.size();
+    s.lev_sorted_off.assign(nl, 0);
+    s.lev_sorted_p.assign(nl, {});
+    for (int k = 0; k < 2; ++k) {
+        s.lev_kids_off[k].assign(nl, 0);
+        s.lev_kids_cnt[k].assign(nl, 0);
+        s.lev_kids_maxu[k].assign(nl, 0);
+    }
+    s.lev_upd_cnt.assign(nl, 0);
+    s.lev_upd_off.assign(nl, 0);
+    s.hlists.clear();
+    for (int l = 0; l < nl; ++l) {
+        std::vector<int32_t> v = S.levels[l];
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return S.fronts[a].p > S.fronts[b].p; });
+        s.lev_sorted_off[l] = (int64_t)s.hlists.size();
+        for (int f : v) {
+            s.hlists.push_back(f);
+            s.lev_sorted_p[l].push_back(S.fronts[f].p);
+        }
+        for (int k = 0; k < 2; ++k) {
+            s.lev_kids_off[k][l] = (int64_t)s.hlists.size();
+            for (int f : S.levels[l])
+                if (S.fronts[f].parent >= 0 && S.fronts[f].slot == k && S.fronts[f].u > 0) {
+                    s.hlists.push_back(f);
+                    s.lev_kids_cnt[k][l]++;
+                    s.lev_kids_maxu[k][l] = std::max(s.lev_kids_maxu[k][l], S.fronts[f].u);
+                }
+        }
+        s.lev_upd_off[l] = (int64_t)s.hlists.size();
+        for (int f : S.levels[l])
+            if (S.fronts[f].u > 0) {
+                s.hlists.push_back(f);
+                s.lev_upd_cnt[l]++;
+            }
+    }
+    if ((rc = dalloc(&s.fronts, (int64_t)df.size())) || (rc = dalloc(&s.lists, (int64_t)s.hlists.size())) ||
+        (rc = dalloc(&s.dst, S.nnz)) || (rc = dalloc(&s.iperm, S.n)) || (rc = dalloc(&s.upd, (int64_t)S.upd_idx.size())) ||
+        (rc = dalloc(&s.relmap, (int64_t)S.relmap.size())) || (rc = dalloc(&s.d_err, 1)))
+        return rc;
+    WT_HIP(hipMemcpy(s.fronts, df.data(), df.size() * sizeof(wt_dfront), hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpy(s.lists, s.hlists.data(), s.hlists.size() * 4, hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpy(s.dst, S.dst.data(), S.nnz * 8, hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpy(s.iperm, S.iperm.data(), S.n * 8, hipMemcpyHostToDevice));
+    if (!S.upd_idx.empty())
+        WT_HIP(hipMemcpy(s.upd, S.upd_idx.data(), S.upd_idx.size() * 8, hipMemcpyHostToDevice));
+    if (!S.relmap.empty())
+        WT_HIP(hipMemcpy(s.relmap, S.relmap.data(), S.relmap.size() * 4, hipMemcpyHostToDevice));
+    // the dst map now lives on the device only
+    s.S.dst.clear();
+    s.S.dst.shrink_to_fit();
+    return WT_OK;
+}
+
+void wt_solver_free(wt_solver_impl& s)
+{
+    s.f32.release();
+    s.f64.release();
+    dfree(s.rowptr); dfree(s.colidx); dfree(s.dst); dfree(s.iperm); dfree(s.upd); dfree(s.relmap);
+    dfree(s.fronts); dfree(s.lists); dfree(s.lam); dfree(s.mu); dfree(s.rho); dfree(s.vals); dfree(s.d_err);
+    dfree(s.xb); dfree(s.rb); dfree(s.bb); dfree(s.partial); dfree(s.norms);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s.stream = nullptr;
+}
+
+// number of fronts of level l with p > kb (lists are sorted by p, desc)
+static int count_gt(const std::vector<int64_t>& ps, int64_t kb)
+{
+    int c = 0;
+    while (c < (int)ps.size() && ps[c] > kb) ++c;
+    return c;
+}
+
+template <typename R>
+static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    const wt_symbolic& S = s.S;
+    const int nl = (int)S.levels.size();
+    const int64_t nb = S.nb;
+    num.fplan.release();
+    num.fsteps.clear();
+    for (int l = nl - 1; l >= 0; --l) {
+        const int par = l & 1;
+        int64_t cbl = 0;
+        for (int f : S.levels[l]) cbl = std::max(cbl, S.fronts[f].CBoff + S.fronts[f].u * S.fronts[f].u);
+        if (cbl > 0) num.fsteps.push_back({WT_STEP_ZERO, 0, cbl, 0, par, -1});
+        if (l + 1 < nl)
+            for (int k = 0; k < 2; ++k)
+                if (s.lev_kids_cnt[k][l + 1] > 0)
+                    num.fsteps.push_back({WT_STEP_EXTADD, s.lev_kids_cnt[k][l + 1], s.lev_kids_maxu[k][l + 1],
+                                          s.lev_kids_off[k][l + 1], par, -1});
+        const std::vector<int64_t>& ps = s.lev_sorted_p[l];
+        const int32_t* lst = s.hlists.data() + s.lev_sorted_off[l];
+        for (int64_t kb = 0; kb < ps[0]; kb += nb) {
+            const int cnt = count_gt(ps, kb);
+            num.fsteps.push_back({WT_STEP_DIAG, cnt, kb, s.lev_sorted_off[l], par, -1});
+            num.fplan.begin();
+            for (int i = 0; i < cnt; ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
+                cpx<R>* L = num.factor + F.Loff;
+                cpx<R>* U = num.factor + F.Uoff;
+                const cpx<R>* Li = num.tiles + F.tileoff + (kb / nb) * 2 * nb * nb;
+                const cpx<R>* Ui = Li + nb * nb;
+                const int32_t* pr = num.perm + F.pivoff + kb;
+                // row panels: rows kb..kb+w of the remaining pivot columns and of U12
+                num.fplan.add(Li, nb, L + (kb + w) * ld + kb, ld, L + (kb + w) * ld + kb, ld, w, p - kb - w, w,
+                              WT_GEMM_SET, pr);
+                num.fplan.add(Li, nb, U + kb, p, U + kb, p, w, u, w, WT_GEMM_SET, pr);
+                // column panel below the tile (L11 and L21 rows)
+                num.fplan.add(L + kb * ld + kb + w, ld, Ui, nb, L + kb * ld + kb + w, ld, p + u - kb - w, w, w,
+                              WT_GEMM_SET);
+            }
+            int g = num.fplan.end();
+            if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            num.fplan.begin();
+            for (int i = 0; i < cnt; ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
+                cpx<R>* L = num.factor + F.Loff;
+                cpx<R>* U = num.factor + F.Uoff;
+                // trailing update of the pivot columns (all rows) and of U12
+                num.fplan.add(L + kb * ld + kb + w, ld, L + (kb + w) * ld + kb, ld, L + (kb + w) * ld + kb + w, ld,
+                              p + u - kb - w, p - kb - w, w, WT_GEMM_SUB);
+                num.fplan.add(L + kb * ld + kb + w, ld, U + kb, p, U + kb + w, p, p - kb - w, u, w, WT_GEMM_SUB);
+            }
+            g = num.fplan.end();
+            if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+        }
+        // contribution blocks: CB -= L21 U12
+        num.fplan.begin();
+        for (int f : S.levels[l]) {
+            const wt_front& F = S.fronts[f];
+            if (F.u == 0) continue;
+            const int64_t p = F.p, u = F.u, ld = p + u;
+            num.fplan.add(num.factor + F.Loff + p, ld, num.factor + F.Uoff, p, num.cb[par] + F.CBoff, u, u, u, p,
+                          WT_GEMM_SUB);
+        }
+        int g = num.fplan.end();
+        if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+    }
+    return num.fplan.upload();
+}
+
+template <typename R>
+static int alloc_numeric(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    if (num.ready) return WT_OK;
+    const wt_symbolic& S = s.S;
+    int rc;
+    if ((rc = dalloc(&num.factor, S.factor_len)) || (rc = dalloc(&num.cb[0], S.cb_len[0])) ||
+        (rc = dalloc(&num.cb[1], S.cb_len[1])) || (rc = dalloc(&num.tiles, S.tile_len)) ||
+        (rc = dalloc(&num.perm, S.piv_len)) || (rc = dalloc(&num.status, 1))) {
+        num.release();
+        return rc;
+    }
+    if ((rc = build_factor_plan(s, num))) {
+        num.release();
+        return rc;
+    }
+    num.ready = true;
+    return WT_OK;
+}
+
+template <typename R>
+int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    if (!s.have_vals) return WT_ERR_STATE;
+    int rc = alloc_numeric(s, num);
+    if (rc) return rc;
+    hipStream_t st = s.stream;
+    const wt_symbolic& S = s.S;
+    WT_HIP(hipMemsetAsync(num.status, 0, sizeof(int), st));
+    if ((rc = wt_fill_zero<R>(num.factor, S.factor_len, st))) return rc;
+    if ((rc = wt_scatter<double, R>(s.vals, s.dst, s.nnz, num.factor, st))) return rc;
+    for (const wt_step& t : num.fsteps) {
+        switch (t.kind) {
+        case WT_STEP_ZERO: rc = wt_fill_zero<R>(num.cb[t.buf], t.a, st); break;
+        case WT_STEP_EXTADD:
+            rc = wt_extend_add<R>(s.fronts, s.lists + t.list_off, t.count, t.a, s.relmap, num.cb[t.buf ^ 1],
+                                  num.factor, num.cb[t.buf], st);
+            break;
+        case WT_STEP_DIAG:
+            rc = wt_diag_tiles<R>(s.fronts, s.lists + t.list_off, t.count, t.a, num.factor, num.tiles, num.perm,
+                                  num.status, st);
+            break;
+        case WT_STEP_GEMM: rc = num.fplan.run(t.gemm, st); break;
+        default: rc = WT_ERR_STATE;
+        }
+        if (rc) return rc;
+    }
+    int hs = 0;
+    WT_HIP(hipMemcpyAsync(&hs, num.status, sizeof(int), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipStreamSynchronize(st));
+    s.factored_prec = sizeof(R) == 4 ? 32 : 64;
+    return hs ? WT_ERR_PIVOT : WT_OK;
+}
+
+template <typename R>
+static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
+{
+    const wt_symbolic& S = s.S;
+    const int nl = (int)S.levels.size();
+    const int64_t nb = S.nb, n = S.n;
+    dfree(num.y);
+    dfree(num.w[0]);
+    dfree(num.w[1]);
+    num.splan.release();
+    num.fwd.clear();
+    num.bwd.clear();
+    int rc;
+    if ((rc = dalloc(&num.y, n * nrhs)) || (rc = dalloc(&num.w[0], S.w_len[0] * nrhs)) ||
+        (rc = dalloc(&num.w[1], S.w_len[1] * nrhs)))
+        return rc;
+    cpx<R>* y = num.y;
+    // forward substitution, bottom-up
+    for (int l = nl - 1; l >= 0; --l) {
+        const int par = l & 1;
+        int64_t wl = 0;
+        for (int f : S.levels[l]) wl = std::max(wl, S.fronts[f].Woff + S.fronts[f].u);
+        if (wl > 0) num.fwd.push_back({WT_STEP_ZERO, 0, wl * nrhs, 0, par, -1});
+        if (l + 1 < nl)
+            for (int k = 0; k < 2; ++k)
+                if (s.lev_kids_cnt[k][l + 1] > 0)
+                    num.fwd.push_back({WT_STEP_SGATHER, s.lev_kids_cnt[k][l + 1], 0, s.lev_kids_off[k][l + 1], par, -1});
+        const std::vector<int64_t>& ps = s.lev_sorted_p[l];
+        const int32_t* lst = s.hlists.data() + s.lev_sorted_off[l];
+        for (int64_t kb = 0; kb < ps[0]; kb += nb) {
+            const int cnt = count_gt(ps, kb);
+            num.splan.begin();
+            for (int i = 0; i < cnt; ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t w = std::min(nb, F.p - kb);
+                const cpx<R>* Li = num.tiles + F.tileoff + (kb / nb) * 2 * nb * nb;
+                cpx<R>* yt = y + F.pstart + kb;
+                num.splan.add(Li, nb, yt, n, yt, n, w, nrhs, w, WT_GEMM_SET, num.perm + F.pivoff + kb);
+            }
+            int g = num.splan.end();
+            if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            num.splan.begin();
+            for (int i = 0; i < cnt; ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
+                const cpx<R>* L = num.factor + F.Loff;
+                cpx<R>* yt = y + F.pstart + kb;
+                num.splan.add(L + kb * ld + kb + w, ld, yt, n, yt + w, n, p - kb - w, nrhs, w, WT_GEMM_SUB);
+                num.splan.add(L + kb * ld + p, ld, yt, n, num.w[par] + F.Woff * nrhs, u, u, nrhs, w, WT_GEMM_SUB);
+            }
+            g = num.splan.end();
+            if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+        }
+    }
+    // back substitution, top-down
+    for (int l = 0; l < nl; ++l) {
+        const int par = l & 1;
+        if (s.lev_upd_cnt[l] > 0) {
+            num.bwd.push_back({WT_STEP_SGATHER_UPD, s.lev_upd_cnt[l], 0, s.lev_upd_off[l], par, -1});
+            num.splan.begin();
+            for (int f : S.levels[l]) {
+                const wt_front& F = S.fronts[f];
+                if (F.u == 0) continue;
+                num.splan.add(num.factor + F.Uoff, F.p, num.w[par] + F.Woff * nrhs, F.u, y + F.pstart, n, F.p, nrhs,
+                              F.u, WT_GEMM_SUB);
+            }
+            int g = num.splan.end();
+            if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+        }
+        const std::vector<int64_t>& ps = s.lev_sorted_p[l];
+        const int32_t* lst = s.hlists.data() + s.lev_sorted_off[l];
+        const int64_t nsteps = (ps[0] + nb - 1) / nb;
+        for (int64_t st = 0; st < nsteps; ++st) {
+            num.splan.begin();
+            for (int i = 0; i < (int)ps.size(); ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
+                if (kb < 0) continue;
+                const int64_t w = std::min(nb, F.p - kb);
+                const cpx<R>* Ui = num.tiles + F.tileoff + (kb / nb) * 2 * nb * nb + nb * nb;
+                cpx<R>* yt = y + F.pstart + kb;
+                num.splan.add(Ui, nb, yt, n, yt, n, w, nrhs, w, WT_GEMM_SET);
+            }
+            int g = num.splan.end();
+            if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            num.splan.begin();
+            for (int i = 0; i < (int)ps.size(); ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
+                if (kb <= 0) continue;
+                const int64_t p = F.p, ld = p + F.u, w = std::min(nb, p - kb);
+                const cpx<R>* L = num.factor + F.Loff;
+                num.splan.add(L + kb * ld, ld, y + F.pstart + kb, n, y + F.pstart, n, kb, nrhs, w, WT_GEMM_SUB);
+            }
+            g = num.splan.end();
+            if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+        }
+    }
+    if ((rc = num.splan.upload())) return rc;
+    num.snrhs = nrhs;
+    return WT_OK;
+}
+
+template <typename R>
+int wt_solver_prepare_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
+{
+    if (!num.ready) return WT_ERR_STATE;
+    if (num.snrhs == nrhs) return WT_OK;
+    return build_solve_plan(s, num, nrhs);
+}
+
+// In-place block solve of A x = y for nrhs columns in elimination order
+// (y column-major, ld = n).  Uses the solver's own y buffer when y == null.
+template <typename R>
+int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R>* /*unused*/)
+{
+    if (!num.ready) return WT_ERR_STATE;
+    int rc;
+    if (num.snrhs != nrhs && (rc = build_solve_plan(s, num, nrhs))) return rc;
+    hipStream_t st = s.stream;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (const wt_step& t : pass ? num.bwd : num.fwd) {
+            switch (t.kind) {
+            case WT_STEP_ZERO: rc = wt_fill_zero<R>(num.w[t.buf], t.a, st); break;
+            case WT_STEP_SGATHER:
+                rc = wt_solve_gather<R>(s.fronts, s.lists + t.list_off, t.count, s.relmap, num.w[t.buf ^ 1],
+                                        num.w[t.buf], num.y, s.S.n, nrhs, st);
+                break;
+            case WT_STEP_SGATHER_UPD:
+                rc = wt_solve_gather_upd<R>(s.fronts, s.lists + t.list_off, t.count, s.upd, num.y, num.w[t.buf],
+                                            s.S.n, nrhs, st);
+                break;
+            case WT_STEP_GEMM: rc = num.splan.run(t.gemm, st); break;
+            default: rc = WT_ERR_STATE;
+            }
+            if (rc) return rc;
+        }
+    }
+    return WT_OK;
+}
+
+template int wt_solver_factor_run<float>(wt_solver_impl&, wt_numeric<float>&);
+template int wt_solver_factor_run<double>(wt_solver_impl&, wt_numeric<double>&);
+template int wt_solver_prepare_solve<float>(wt_solver_impl&, wt_numeric<float>&, int);
+template int wt_solver_prepare_solve<double>(wt_solver_impl&, wt_numeric<double>&, int);
+template int wt_solver_solve_block<float>(wt_solver_impl&, wt_numeric<float>&, int, cpx<float>*);
+template int wt_solver_solve_block<double>(wt_solver_impl&, wt_numeric<double>&, int, cpx<double>*);

@@ -1,0 +1,144 @@
+// Multifrontal solver state: symbolic analysis, device arenas, and the
+// launch plans of the numeric factorization and the block solve.
+#pragma once
+#include <cstdint>
+#include <vector>
+#include <hip/hip_runtime.h>
+
+#include "wt_front.h"
+#include "wt_gemm.h"
+#include "wt_symbolic.h"
+
+// One launch of the batched GEMM: a contiguous range of the plan's problems.
+struct wt_gemm_launch_rec {
+    int64_t prob_off, tile_off, ntiles;
+    int32_t nprob;
+};
+
+template <typename R>
+struct wt_gemm_plan {
+    std::vector<wt_gemm_prob<R>> probs;
+    std::vector<int64_t> tiles;
+    std::vector<wt_gemm_launch_rec> launches;
+    wt_gemm_prob<R>* d_probs = nullptr;
+    int64_t* d_tiles = nullptr;
+    wt_gemm_launch_rec cur{};
+    void begin()
+    {
+        cur.prob_off = (int64_t)probs.size();
+        cur.tile_off = (int64_t)tiles.size();
+        cur.ntiles = 0;
+        cur.nprob = 0;
+        tiles.push_back(0);
+    }
+    void add(const cpx<R>* A, int64_t lda, const cpx<R>* B, int64_t ldb, cpx<R>* C, int64_t ldc, int64_t M,
+             int64_t N, int64_t K, int mode, const int32_t* bperm = nullptr)
+    {
+        if (M <= 0 || N <= 0 || K <= 0) return;
+        wt_gemm_prob<R> p;
+        p.A = A; p.B = B; p.C = C; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
+        p.M = (int32_t)M; p.N = (int32_t)N; p.K = (int32_t)K; p.mode = mode; p.bperm = bperm; p.pad = 0;
+        probs.push_back(p);
+        cur.ntiles += wt_gemm_tiles<R>(M, N);
+        tiles.push_back(cur.ntiles);
+        ++cur.nprob;
+    }
+    // returns the launch index, or -1 when nothing was added
+    int end()
+    {
+        if (cur.nprob == 0) {
+            tiles.pop_back();
+            return -1;
+        }
+        launches.push_back(cur);
+        return (int)launches.size() - 1;
+    }
+    int upload();
+    void release();
+    int run(int i, hipStream_t s) const
+    {
+        const wt_gemm_launch_rec& L = launches[i];
+        return wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.tile_off, L.nprob, L.ntiles, s);
+    }
+};
+
+enum wt_step_kind { WT_STEP_ZERO, WT_STEP_EXTADD, WT_STEP_DIAG, WT_STEP_GEMM, WT_STEP_SGATHER, WT_STEP_SGATHER_UPD };
+
+struct wt_step {
+    int kind;
+    int32_t count;       // list length (diag / extend-add / gathers)
+    int64_t a;           // ZERO: entries; DIAG: kb; EXTADD/SGATHER: max_u
+    int64_t list_off;    // offset into the device list array
+    int32_t buf;         // arena parity (ZERO: which arena; EXTADD: parent parity)
+    int32_t gemm;        // GEMM launch index
+};
+
+template <typename R>
+struct wt_numeric {
+    bool ready = false;
+    cpx<R>* factor = nullptr;
+    cpx<R>* cb[2] = {nullptr, nullptr};
+    cpx<R>* tiles = nullptr;
+    int32_t* perm = nullptr;
+    int* status = nullptr;
+    wt_gemm_plan<R> fplan;
+    std::vector<wt_step> fsteps;
+    // block solve
+    int snrhs = 0;
+    cpx<R>* y = nullptr;
+    cpx<R>* w[2] = {nullptr, nullptr};
+    wt_gemm_plan<R> splan;
+    std::vector<wt_step> fwd, bwd;
+    void release();
+};
+
+struct wt_solver_impl {
+    wt_symbolic S;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, nnz = 0;
+    // device copies of the symbolic data
+    int32_t* rowptr = nullptr;
+    int32_t* colidx = nullptr;
+    int64_t* dst = nullptr;
+    int64_t* iperm = nullptr;
+    int64_t* upd = nullptr;
+    int32_t* relmap = nullptr;
+    wt_dfront* fronts = nullptr;
+    int32_t* lists = nullptr;
+    std::vector<int32_t> hlists;
+    // per level: fronts sorted by p (desc), and children by slot
+    std::vector<int64_t> lev_sorted_off;
+    std::vector<std::vector<int64_t>> lev_sorted_p;
+    std::vector<int64_t> lev_kids_off[2];
+    std::vector<int32_t> lev_kids_cnt[2];
+    std::vector<int64_t> lev_kids_maxu[2];
+    std::vector<int32_t> lev_upd_cnt;
+    std::vector<int64_t> lev_upd_off;
+    // model and matrix
+    cpx<double>* lam = nullptr;
+    cpx<double>* mu = nullptr;
+    double* rho = nullptr;
+    cpx<double>* vals = nullptr;  // CSR values, complex128
+    bool have_vals = false;
+    int* d_err = nullptr;
+    // refinement buffers (row-major n x 64)
+    int ir_cap = 0;
+    cpx<double>* xb = nullptr;
+    cpx<double>* rb = nullptr;
+    cpx<double>* bb = nullptr;
+    double* partial = nullptr;
+    double* norms = nullptr;
+    wt_numeric<float> f32;
+    wt_numeric<double> f64;
+    int factored_prec = 0;
+    double last_factor_ms = 0;
+};
+
+int wt_solver_init(wt_solver_impl& s, int nx, int ny, int nz, int leaf_nodes);
+void wt_solver_free(wt_solver_impl& s);
+template <typename R>
+int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num);
+template <typename R>
+int wt_solver_prepare_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs);
+template <typename R>
+int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R>* y);

@@ -1,0 +1,23 @@
+// Vector kernels of the solve phase (layout changes, residual, norms).
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+#include "wt_common.h"
+
+template <typename Rs, typename R>
+int wt_to_elim(const cpx<Rs>* src, const int64_t* iperm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s);
+template <typename R, typename Rd>
+int wt_from_elim(const cpx<R>* y, const int64_t* iperm, cpx<Rd>* dst, int64_t n, int nrhs, int accumulate,
+                 hipStream_t s);
+int64_t wt_residual_blocks(int64_t n);
+template <typename Ra>
+int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* vals, const cpx<double>* x,
+                const cpx<double>* b, cpx<double>* r, double* partial, double* rnorm2, int64_t n, int nrhs,
+                hipStream_t s);
+int wt_colnorm2(const cpx<double>* v, int64_t n, int nrhs, double* partial, int64_t nblk, double* out,
+                hipStream_t s);
+template <typename R>
+int wt_copy(const cpx<R>* a, cpx<R>* b, int64_t n, hipStream_t s);
+int wt_convert_c128_c64(const cpx<double>* a, cpx<float>* b, int64_t n, hipStream_t s);
+int wt_widen_c64(const cpx<float>* a, cpx<double>* b, int64_t n, hipStream_t s);

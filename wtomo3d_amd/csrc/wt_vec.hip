@@ -1,0 +1,257 @@
+// Vector kernels of the solve phase: reordering between the caller's
+// layout and the elimination order, the FP64 CSR residual for iterative
+// refinement, and per-right-hand-side norms.
+//
+// Caller blocks are row-major n x nrhs (all right-hand sides of one unknown
+// contiguous), which makes the CSR residual read each matrix entry once for
+// the whole block.  Inside the factor/solve the block is column-major in
+// elimination order so the dense front solves are plain GEMMs.
+#include "wt_vec.h"
+
+namespace {
+
+constexpr int TP = 32;  // transpose tile
+
+// y[i' + c n] = (R) src[iperm[i'] * nrhs + c]
+template <typename Rs, typename R>
+__global__ __launch_bounds__(256) void to_elim_kernel(const cpx<Rs>* __restrict__ src, const int64_t* __restrict__ iperm,
+                                                      cpx<R>* __restrict__ y, int64_t n, int nrhs)
+{
+    __shared__ cpx<R> t[TP][TP + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * TP;
+    const int c0 = blockIdx.y * TP;
+    const int tx = threadIdx.x % TP, ty = threadIdx.x / TP;  // 32 x 8
+    for (int k = ty; k < TP; k += 8) {
+        int64_t i = i0 + k;
+        int c = c0 + tx;
+        if (i < n && c < nrhs) {
+            cpx<Rs> v = src[iperm[i] * nrhs + c];
+            t[k][tx] = cpx<R>((R)v.re, (R)v.im);
+        }
+    }
+    __syncthreads();
+    for (int k = ty; k < TP; k += 8) {
+        int c = c0 + k;
+        int64_t i = i0 + tx;
+        if (i < n && c < nrhs) y[(int64_t)c * n + i] = t[tx][k];
+    }
+}
+
+// dst[iperm[i'] * nrhs + c] (+)= y[i' + c n]
+template <typename R, typename Rd>
+__global__ __launch_bounds__(256) void from_elim_kernel(const cpx<R>* __restrict__ y, const int64_t* __restrict__ iperm,
+                                                        cpx<Rd>* __restrict__ dst, int64_t n, int nrhs, int accumulate)
+{
+    __shared__ cpx<R> t[TP][TP + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * TP;
+    const int c0 = blockIdx.y * TP;
+    const int tx = threadIdx.x % TP, ty = threadIdx.x / TP;
+    for (int k = ty; k < TP; k += 8) {
+        int c = c0 + k;
+        int64_t i = i0 + tx;
+        if (i < n && c < nrhs) t[tx][k] = y[(int64_t)c * n + i];
+    }
+    __syncthreads();
+    for (int k = ty; k < TP; k += 8) {
+        int64_t i = i0 + k;
+        int c = c0 + tx;
+        if (i < n && c < nrhs) {
+            cpx<Rd>* p = dst + iperm[i] * nrhs + c;
+            cpx<R> v = t[k][tx];
+            if (accumulate) *p = cpx<Rd>(p->re + (Rd)v.re, p->im + (Rd)v.im);
+            else *p = cpx<Rd>((Rd)v.re, (Rd)v.im);
+        }
+    }
+}
+
+// r = b - A x for a row-major block of up to 64 columns: one wave per
+// matrix row, lane = right-hand side, so each matrix entry is read once for
+// the whole block and the x rows are read as contiguous 64-wide segments.
+// Per-block partial sums of |r|^2 per column go to `partial`.
+constexpr int SP_ROWS = 4;  // rows per block (one per wave)
+
+template <typename Ra>
+__global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ colidx,
+                                                       const cpx<Ra>* __restrict__ vals,
+                                                       const cpx<double>* __restrict__ x,
+                                                       const cpx<double>* __restrict__ b, cpx<double>* __restrict__ r,
+                                                       double* __restrict__ partial, int64_t n, int nrhs)
+{
+    __shared__ double s_part[SP_ROWS][64];
+    const int wave = threadIdx.x >> 6, c = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * SP_ROWS + wave;
+    double nrm = 0.0;
+    if (row < n && c < nrhs) {
+        cpx<double> acc = b[row * nrhs + c];
+        const int32_t e1 = rowptr[row + 1];
+        for (int32_t e = rowptr[row]; e < e1; ++e) {
+            const cpx<Ra> a = vals[e];
+            const cpx<double> xv = x[(int64_t)colidx[e] * nrhs + c];
+            acc.re -= (double)a.re * xv.re - (double)a.im * xv.im;
+            acc.im -= (double)a.re * xv.im + (double)a.im * xv.re;
+        }
+        r[row * nrhs + c] = acc;
+        nrm = acc.re * acc.re + acc.im * acc.im;
+    }
+    s_part[wave][c] = nrm;
+    __syncthreads();
+    if (wave == 0 && c < nrhs) {
+        double s = 0.0;
+        for (int w = 0; w < SP_ROWS; ++w) s += s_part[w][c];
+        partial[(int64_t)blockIdx.x * nrhs + c] = s;
+    }
+}
+
+// sum over blocks of partial[blk * nrhs + c] -> out[c] (fixed order)
+__global__ void reduce_cols_kernel(const double* __restrict__ partial, int64_t nblk, int nrhs,
+                                   double* __restrict__ out)
+{
+    __shared__ double s[256];
+    const int c = blockIdx.x;
+    double acc = 0.0;
+    for (int64_t b = threadIdx.x; b < nblk; b += 256) acc += partial[b * nrhs + c];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = s[0];
+}
+
+// per-column |v|^2 of a row-major block, partial sums per block
+__global__ __launch_bounds__(256) void colnorm_kernel(const cpx<double>* __restrict__ v, int64_t n, int nrhs,
+                                                      double* __restrict__ partial)
+{
+    __shared__ double s[256];
+    const int c = blockIdx.y;
+    double acc = 0.0;
+    const int64_t rows_per_blk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
+    const int64_t r1 = r0 + rows_per_blk < n ? r0 + rows_per_blk : n;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+        cpx<double> a = v[i * nrhs + c];
+        acc += a.re * a.re + a.im * a.im;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nrhs + c] = s[0];
+}
+
+template <typename R>
+__global__ void copy_kernel(const cpx<R>* __restrict__ a, cpx<R>* __restrict__ b, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
+__global__ void convert_c128_c64_kernel(const cpx<double>* __restrict__ a, cpx<float>* __restrict__ b, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = cpx<float>((float)a[i].re, (float)a[i].im);
+}
+
+__global__ void widen_c64_kernel(const cpx<float>* __restrict__ a, cpx<double>* __restrict__ b, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = cpx<double>((double)a[i].re, (double)a[i].im);
+}
+
+}  // namespace
+
+static unsigned grid1(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+template <typename Rs, typename R>
+int wt_to_elim(const cpx<Rs>* src, const int64_t* iperm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s)
+{
+    dim3 grid((unsigned)((n + TP - 1) / TP), (nrhs + TP - 1) / TP);
+    hipLaunchKernelGGL((to_elim_kernel<Rs, R>), grid, dim3(256), 0, s, src, iperm, y, n, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R, typename Rd>
+int wt_from_elim(const cpx<R>* y, const int64_t* iperm, cpx<Rd>* dst, int64_t n, int nrhs, int accumulate,
+                 hipStream_t s)
+{
+    dim3 grid((unsigned)((n + TP - 1) / TP), (nrhs + TP - 1) / TP);
+    hipLaunchKernelGGL((from_elim_kernel<R, Rd>), grid, dim3(256), 0, s, y, iperm, dst, n, nrhs, accumulate);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int64_t wt_residual_blocks(int64_t n) { return (n + SP_ROWS - 1) / SP_ROWS; }
+
+template <typename Ra>
+int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* vals, const cpx<double>* x,
+                const cpx<double>* b, cpx<double>* r, double* partial, double* rnorm2, int64_t n, int nrhs,
+                hipStream_t s)
+{
+    if (nrhs > 64) return WT_ERR_ARG;  // one wave per row covers up to 64 columns
+    int64_t nb = wt_residual_blocks(n);
+    hipLaunchKernelGGL(residual_kernel<Ra>, dim3((unsigned)nb), dim3(256), 0, s, rowptr, colidx, vals, x, b, r,
+                       partial, n, nrhs);
+    WT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_colnorm2(const cpx<double>* v, int64_t n, int nrhs, double* partial, int64_t nblk, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(colnorm_kernel, dim3((unsigned)nblk, nrhs), dim3(256), 0, s, v, n, nrhs, partial);
+    WT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nblk, nrhs, out);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_copy(const cpx<R>* a, cpx<R>* b, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL(copy_kernel<R>, dim3(grid1(n)), dim3(256), 0, s, a, b, n);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_convert_c128_c64(const cpx<double>* a, cpx<float>* b, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL(convert_c128_c64_kernel, dim3(grid1(n)), dim3(256), 0, s, a, b, n);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_widen_c64(const cpx<float>* a, cpx<double>* b, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL(widen_c64_kernel, dim3(grid1(n)), dim3(256), 0, s, a, b, n);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template int wt_to_elim<double, float>(const cpx<double>*, const int64_t*, cpx<float>*, int64_t, int, hipStream_t);
+template int wt_to_elim<double, double>(const cpx<double>*, const int64_t*, cpx<double>*, int64_t, int, hipStream_t);
+template int wt_from_elim<float, double>(const cpx<float>*, const int64_t*, cpx<double>*, int64_t, int, int,
+                                         hipStream_t);
+template int wt_from_elim<double, double>(const cpx<double>*, const int64_t*, cpx<double>*, int64_t, int, int,
+                                          hipStream_t);
+template int wt_residual<double>(const int32_t*, const int32_t*, const cpx<double>*, const cpx<double>*,
+                                 const cpx<double>*, cpx<double>*, double*, double*, int64_t, int, hipStream_t);
+template int wt_residual<float>(const int32_t*, const int32_t*, const cpx<float>*, const cpx<double>*,
+                                const cpx<double>*, cpx<double>*, double*, double*, int64_t, int, hipStream_t);
+template int wt_copy<double>(const cpx<double>*, cpx<double>*, int64_t, hipStream_t);
+template int wt_copy<float>(const cpx<float>*, cpx<float>*, int64_t, hipStream_t);

@@ -1,0 +1,101 @@
+"""Multifrontal solver handle: the host side of the reference's solver shim.
+
+The reference drives STRUMPACK through ``strumpack_serial_finter`` jobs
+1 (reorder), 2 (factor) and 3 (solve) (strumpack_finter.c:9-120); this class
+exposes the same phases on one grid, for whole blocks of right-hand sides:
+
+    s = Solver(nx, ny, nz)                  # job 1: ordering + symbolic
+    s.set_model(lam, mu, rho)               # MODEL_MODULE state
+    s.assemble(params)                      # ASMBLE at one frequency (FP64)
+    s.factor(32)                            # job 2, complex64 fronts
+    x, st = s.solve(b, prec=32, tol=1e-8)   # job 3 for a block, FP64 refinement
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib, check, WtSolveStats
+
+_vp = ctypes.c_void_p
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+class Solver:
+    def __init__(self, nx, ny, nz, leaf_nodes=64):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = 3 * nx * ny * nz
+        h = ctypes.c_void_p()
+        check(lib().wt_solver_create(nx, ny, nz, leaf_nodes, ctypes.byref(h)), "wt_solver_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().wt_solver_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        arr = (ctypes.c_int64 * 8)()
+        fl = ctypes.c_double()
+        check(lib().wt_solver_info(self._h, arr, ctypes.byref(fl)), "wt_solver_info")
+        keys = ("n", "nnz", "fronts", "levels", "max_p", "max_u", "factor_entries", "cb_entries")
+        d = dict(zip(keys, list(arr)))
+        d["flops"] = fl.value
+        return d
+
+    def bytes_needed(self, prec=32, nrhs=64):
+        return int(lib().wt_solver_bytes(self._h, prec, nrhs))
+
+    def set_model(self, lam, mu, rho):
+        lam = np.ascontiguousarray(lam, np.complex128).ravel()
+        mu = np.ascontiguousarray(mu, np.complex128).ravel()
+        rho = np.ascontiguousarray(rho, np.float64).ravel()
+        assert lam.size == self.n // 3 and mu.size == lam.size and rho.size == lam.size
+        check(lib().wt_solver_set_model(self._h, _p(lam), _p(mu), _p(rho), 0), "wt_solver_set_model")
+
+    def assemble(self, params):
+        check(lib().wt_solver_assemble(self._h, ctypes.byref(params)), "wt_solver_assemble")
+
+    def set_values(self, vals):
+        vals = np.ascontiguousarray(vals)
+        prec = 32 if vals.dtype == np.complex64 else 64
+        vals = vals.astype(np.complex64 if prec == 32 else np.complex128, copy=False)
+        check(lib().wt_solver_set_values(self._h, _p(vals), prec, 0), "wt_solver_set_values")
+
+    def values(self):
+        from .assembly import graph_nnz
+        v = np.zeros(graph_nnz(self.nx, self.ny, self.nz), np.complex128)
+        check(lib().wt_solver_get_values(self._h, _p(v)), "wt_solver_get_values")
+        return v
+
+    def factor(self, prec=32):
+        check(lib().wt_solver_factor(self._h, prec), "wt_solver_factor")
+
+    def release(self, prec):
+        check(lib().wt_solver_release(self._h, prec), "wt_solver_release")
+
+    def solve(self, b, prec=32, tol=1e-8, max_iter=20):
+        """b: (n,) or (n, nrhs) complex; returns (x, stats dict)."""
+        b = np.asarray(b)
+        one = b.ndim == 1
+        B = np.ascontiguousarray(b.reshape(self.n, -1), np.complex128)
+        X = np.zeros_like(B)
+        st = WtSolveStats()
+        check(lib().wt_solver_solve(self._h, prec, B.shape[1], _p(B), _p(X), 0, tol, max_iter,
+                                    ctypes.byref(st)), "wt_solver_solve")
+        stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
+        return (X[:, 0] if one else X), stats
