@@ -132,6 +132,18 @@ int wt_solver_release(wt_solver *h, int factor_prec);
 int wt_solver_solve(wt_solver *h, int factor_prec, int nrhs, const void *b, void *x, int on_device,
                     double tol, int max_iter, wt_solve_stats *stats);
 
+/* Per-kernel timing with HIP events on the solver's stream: enable (1) or
+ * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and
+ * contribution-block updates), 1 solve GEMMs, 2 diagonal-tile LU, 3
+ * extend-add, 4 fills / value scatter, 5 assembly, 6 FP64 residual, 7
+ * reorder / gather.  _read returns launches, summed kernel milliseconds,
+ * algorithmic flops (8 M N K per complex GEMM, 8 per residual complex
+ * multiply-add) and algorithmic bytes.  No counterpart in the reference
+ * (STRUMPACK's verbose statistics, strumpack_finter.c:32-33, are printed). */
+int wt_solver_profile(wt_solver *h, int enable);
+int wt_solver_profile_read(wt_solver *h, int cls, int64_t *launches, double *ms, double *flops,
+                           double *bytes);
+
 /* Drop-in for the reference's solver shim with the same argument list and
  * job semantics (strumpack_finter.c:9-120, interface.f90:297-307):
  * 1 analyse, 2 factor (complex64 values), 3 solve one right-hand side,
@@ -140,6 +152,15 @@ void strumpack_serial_finter(int *job, int *nrows, int *nnz, int *irptr, int *jc
                              void *sol, int *ierr);
 /* The reference's empty MPI stub (strumpack_finter.c:123-126). */
 int strumpack_parallel_finter(void);
+
+/* Batched complex GEMM of the front kernels on one problem, host arrays
+ * (column-major): C = A B (mode 1), C -= A B (0), C += A B (2); optional
+ * row permutation of B (bperm[k] = source row).  Runs `reps` times (C is
+ * re-uploaded each time) and reports the last run's kernel time in *ms.
+ * No counterpart in the reference: a test and measurement entry for the
+ * kernel that replaces STRUMPACK's dense front updates. */
+int wt_gemm_host(int prec, int M, int N, int K, const void *A, int64_t lda, const void *B, int64_t ldb,
+                 void *C, int64_t ldc, int mode, const int32_t *bperm, int reps, double *ms);
 
 /* ---------------- device memory helpers ---------------- */
 int wt_dev_alloc(size_t bytes, void **d_ptr);

@@ -1,0 +1,51 @@
+"""The batched complex GEMM (3M on MFMA) against numpy complex128 products."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import wtomo3d_amd as W
+
+pytestmark = pytest.mark.gpu
+P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0):
+    rng = np.random.default_rng(seed)
+    ct = np.complex64 if prec == 32 else np.complex128
+    lda, ldb, ldc = lda or M, ldb or K, ldc or M
+    A = (rng.standard_normal((K, lda)) + 1j * rng.standard_normal((K, lda))).astype(ct)   # col-major M x K
+    B = (rng.standard_normal((N, ldb)) + 1j * rng.standard_normal((N, ldb))).astype(ct)
+    C = (rng.standard_normal((N, ldc)) + 1j * rng.standard_normal((N, ldc))).astype(ct)
+    bp = rng.permutation(K).astype(np.int32) if perm else None
+    Am = A[:, :M].T.astype(np.complex128)
+    Bm = B[:, :K].T.astype(np.complex128)
+    if perm:
+        Bm = Bm[bp, :]
+    Cm = C[:, :M].T.astype(np.complex128)
+    ref = {0: Cm - Am @ Bm, 1: Am @ Bm, 2: Cm + Am @ Bm}[mode]
+    Cout = C.copy()
+    ms = ctypes.c_double()
+    rc = W.lib().wt_gemm_host(prec, M, N, K, P(A), lda, P(B), ldb, P(Cout), ldc, mode,
+                              P(bp) if perm else None, 1, ctypes.byref(ms))
+    assert rc == 0
+    got = Cout[:, :M].T.astype(np.complex128)
+    scale = np.abs(Am) @ np.abs(Bm) + np.abs(Cm)
+    err = np.max(np.abs(got - ref) / scale)
+    assert np.array_equal(Cout[:, M:], C[:, M:])  # padding rows untouched
+    return err
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (33, 500, 17)])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_gemm_modes_shapes(prec, shape, mode):
+    M, N, K = shape
+    err = _run(prec, M, N, K, mode, lda=M + 3, ldb=K + 1, ldc=M + 5)
+    assert err <= (2e-6 if prec == 32 else 1e-14)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_gemm_row_permuted_b(prec):
+    err = _run(prec, 64, 200, 64, 1, perm=True, seed=3)
+    assert err <= (2e-6 if prec == 32 else 1e-14)

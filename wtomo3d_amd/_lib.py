@@ -68,6 +68,14 @@ def _declare(L):
                  "wt_solver_assemble", "wt_solver_set_values", "wt_solver_get_values", "wt_solver_factor",
                  "wt_solver_release", "wt_solver_solve"):
         getattr(L, name).restype = ctypes.c_int
+    L.wt_gemm_host.argtypes = [i32] * 4 + [_vp, i64, _vp, i64, _vp, i64, i32, _vp, i32,
+                                           ctypes.POINTER(ctypes.c_double)]
+    L.wt_gemm_host.restype = ctypes.c_int
+    L.wt_solver_profile.argtypes = [_vp, i32]
+    L.wt_solver_profile.restype = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.wt_solver_profile_read.argtypes = [_vp, i32, ctypes.POINTER(ctypes.c_int64), dp, dp, dp]
+    L.wt_solver_profile_read.restype = ctypes.c_int
     L.strumpack_serial_finter.argtypes = [i32p, i32p, i32p, _vp, _vp, _vp, _vp, _vp, i32p]
     L.strumpack_serial_finter.restype = None
     return L

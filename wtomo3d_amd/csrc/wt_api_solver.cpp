@@ -69,14 +69,23 @@ int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b
     int it = 0;
     double rel = INFINITY, first = INFINITY;
     while (it < max_iter) {
+        wt_prof& P = s.prof;
+        const double cs = sizeof(cpx<R>), blk = (double)n * nrhs;
         if ((rc = wt_solver_prepare_solve<R>(s, num, nrhs))) return rc;
-        if ((rc = wt_to_elim<double, R>(r, s.iperm, num.y, n, nrhs, q))) return rc;
+        if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (16 + cs) + n * 8.0,
+                        [&] { return wt_to_elim<double, R>(r, s.iperm, num.y, n, nrhs, q); })))
+            return rc;
         if ((rc = wt_solver_solve_block<R>(s, num, nrhs, nullptr))) return rc;
-        if ((rc = wt_from_elim<R, double>(num.y, s.iperm, x, n, nrhs, 1, q))) return rc;
-        if ((rc = wt_residual<double>(s.rowptr, s.colidx, s.vals, x, b, s.rb, s.partial, s.norms, n, nrhs, q)))
+        if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (32 + cs) + n * 8.0,
+                        [&] { return wt_from_elim<R, double>(num.y, s.iperm, x, n, nrhs, 1, q); })))
+            return rc;
+        if ((rc = P.run(q, WT_PROF_RESIDUAL, 8.0 * s.nnz * nrhs, s.nnz * 20.0 + (n + 1) * 4.0 + blk * 48.0, [&] {
+                 return wt_residual<double>(s.rowptr, s.colidx, s.vals, x, b, s.rb, s.partial, s.norms, n, nrhs, q);
+             })))
             return rc;
         WT_HIP(hipMemcpyAsync(h.data(), s.norms, 2 * nrhs * sizeof(double), hipMemcpyDeviceToHost, q));
         WT_HIP(hipStreamSynchronize(q));
+        P.flush();
         ++it;
         rel = 0.0;
         double absmax = 0.0;
@@ -185,10 +194,15 @@ int wt_solver_assemble(wt_solver* h, const wt_grid_params* p)
     int rc;
     if (!s.vals && (rc = dalloc(&s.vals, s.nnz))) return rc;
     WT_HIP(hipMemsetAsync(s.d_err, 0, sizeof(int), s.stream));
-    if ((rc = wt_launch_values<double>(*p, s.lam, s.mu, s.rho, s.vals, s.d_err, s.stream))) return rc;
+    const double nodes = (double)(s.n / 3);
+    if ((rc = s.prof.run(s.stream, WT_PROF_ASSEMBLE, 0, s.nnz * 16.0 + nodes * 40.0, [&] {
+             return wt_launch_values<double>(*p, s.lam, s.mu, s.rho, s.vals, s.d_err, s.stream);
+         })))
+        return rc;
     int he = 0;
     WT_HIP(hipMemcpyAsync(&he, s.d_err, sizeof(int), hipMemcpyDeviceToHost, s.stream));
     WT_HIP(hipStreamSynchronize(s.stream));
+    s.prof.flush();
     if (he) return WT_ERR_ARG;
     s.have_vals = true;
     return WT_OK;
@@ -232,6 +246,27 @@ int wt_solver_factor(wt_solver* h, int factor_prec)
     if (factor_prec == WT_PREC_F32) return wt_solver_factor_run<float>(h->s, h->s.f32);
     if (factor_prec == WT_PREC_F64) return wt_solver_factor_run<double>(h->s, h->s.f64);
     return WT_ERR_ARG;
+}
+
+int wt_solver_profile(wt_solver* h, int enable)
+{
+    if (!h) return WT_ERR_ARG;
+    WT_HIP(hipStreamSynchronize(h->s.stream));
+    h->s.prof.flush();
+    h->s.prof.reset();
+    h->s.prof.on = enable != 0;
+    return WT_OK;
+}
+
+int wt_solver_profile_read(wt_solver* h, int cls, int64_t* launches, double* ms, double* flops, double* bytes)
+{
+    if (!h || cls < 0 || cls >= WT_PROF_NCLASS) return WT_ERR_ARG;
+    const wt_prof& P = h->s.prof;
+    if (launches) *launches = P.launches[cls];
+    if (ms) *ms = P.ms[cls];
+    if (flops) *flops = P.flops[cls];
+    if (bytes) *bytes = P.bytes[cls];
+    return WT_OK;
 }
 
 int wt_solver_release(wt_solver* h, int factor_prec)

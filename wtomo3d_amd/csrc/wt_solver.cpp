@@ -164,6 +164,7 @@ int wt_solver_init(wt_solver_impl& s, int nx, int ny, int nz, int leaf_nodes)
 
 void wt_solver_free(wt_solver_impl& s)
 {
+    s.prof.release();
     s.f32.release();
     s.f64.release();
     dfree(s.rowptr); dfree(s.colidx); dfree(s.dst); dfree(s.iperm); dfree(s.upd); dfree(s.relmap);
@@ -280,21 +281,36 @@ int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
     if (rc) return rc;
     hipStream_t st = s.stream;
     const wt_symbolic& S = s.S;
+    wt_prof& P = s.prof;
+    const double cs = sizeof(cpx<R>);
     WT_HIP(hipMemsetAsync(num.status, 0, sizeof(int), st));
-    if ((rc = wt_fill_zero<R>(num.factor, S.factor_len, st))) return rc;
-    if ((rc = wt_scatter<double, R>(s.vals, s.dst, s.nnz, num.factor, st))) return rc;
+    if ((rc = P.run(st, WT_PROF_FILL, 0, S.factor_len * cs,
+                    [&] { return wt_fill_zero<R>(num.factor, S.factor_len, st); })))
+        return rc;
+    if ((rc = P.run(st, WT_PROF_FILL, 0, s.nnz * (16.0 + 8.0 + cs),
+                    [&] { return wt_scatter<double, R>(s.vals, s.dst, s.nnz, num.factor, st); })))
+        return rc;
     for (const wt_step& t : num.fsteps) {
         switch (t.kind) {
-        case WT_STEP_ZERO: rc = wt_fill_zero<R>(num.cb[t.buf], t.a, st); break;
+        case WT_STEP_ZERO:
+            rc = P.run(st, WT_PROF_FILL, 0, t.a * cs, [&] { return wt_fill_zero<R>(num.cb[t.buf], t.a, st); });
+            break;
         case WT_STEP_EXTADD:
-            rc = wt_extend_add<R>(s.fronts, s.lists + t.list_off, t.count, t.a, s.relmap, num.cb[t.buf ^ 1],
-                                  num.factor, num.cb[t.buf], st);
+            rc = P.run(st, WT_PROF_EXTADD, 0, 0, [&] {
+                return wt_extend_add<R>(s.fronts, s.lists + t.list_off, t.count, t.a, s.relmap, num.cb[t.buf ^ 1],
+                                        num.factor, num.cb[t.buf], st);
+            });
             break;
         case WT_STEP_DIAG:
-            rc = wt_diag_tiles<R>(s.fronts, s.lists + t.list_off, t.count, t.a, num.factor, num.tiles, num.perm,
-                                  num.status, st);
+            rc = P.run(st, WT_PROF_DIAG, 0, 0, [&] {
+                return wt_diag_tiles<R>(s.fronts, s.lists + t.list_off, t.count, t.a, num.factor, num.tiles,
+                                        num.perm, num.status, st);
+            });
             break;
-        case WT_STEP_GEMM: rc = num.fplan.run(t.gemm, st); break;
+        case WT_STEP_GEMM:
+            rc = P.run(st, WT_PROF_GEMM_FACTOR, num.fplan.launches[t.gemm].flops, 0,
+                       [&] { return num.fplan.run(t.gemm, st); });
+            break;
         default: rc = WT_ERR_STATE;
         }
         if (rc) return rc;
@@ -302,6 +318,7 @@ int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
     int hs = 0;
     WT_HIP(hipMemcpyAsync(&hs, num.status, sizeof(int), hipMemcpyDeviceToHost, st));
     WT_HIP(hipStreamSynchronize(st));
+    P.flush();
     s.factored_prec = sizeof(R) == 4 ? 32 : 64;
     return hs ? WT_ERR_PIVOT : WT_OK;
 }
@@ -426,19 +443,30 @@ int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R
     int rc;
     if (num.snrhs != nrhs && (rc = build_solve_plan(s, num, nrhs))) return rc;
     hipStream_t st = s.stream;
+    wt_prof& P = s.prof;
     for (int pass = 0; pass < 2; ++pass) {
         for (const wt_step& t : pass ? num.bwd : num.fwd) {
             switch (t.kind) {
-            case WT_STEP_ZERO: rc = wt_fill_zero<R>(num.w[t.buf], t.a, st); break;
+            case WT_STEP_ZERO:
+                rc = P.run(st, WT_PROF_FILL, 0, t.a * (double)sizeof(cpx<R>),
+                           [&] { return wt_fill_zero<R>(num.w[t.buf], t.a, st); });
+                break;
             case WT_STEP_SGATHER:
-                rc = wt_solve_gather<R>(s.fronts, s.lists + t.list_off, t.count, s.relmap, num.w[t.buf ^ 1],
-                                        num.w[t.buf], num.y, s.S.n, nrhs, st);
+                rc = P.run(st, WT_PROF_PERMUTE, 0, 0, [&] {
+                    return wt_solve_gather<R>(s.fronts, s.lists + t.list_off, t.count, s.relmap, num.w[t.buf ^ 1],
+                                              num.w[t.buf], num.y, s.S.n, nrhs, st);
+                });
                 break;
             case WT_STEP_SGATHER_UPD:
-                rc = wt_solve_gather_upd<R>(s.fronts, s.lists + t.list_off, t.count, s.upd, num.y, num.w[t.buf],
-                                            s.S.n, nrhs, st);
+                rc = P.run(st, WT_PROF_PERMUTE, 0, 0, [&] {
+                    return wt_solve_gather_upd<R>(s.fronts, s.lists + t.list_off, t.count, s.upd, num.y,
+                                                  num.w[t.buf], s.S.n, nrhs, st);
+                });
                 break;
-            case WT_STEP_GEMM: rc = num.splan.run(t.gemm, st); break;
+            case WT_STEP_GEMM:
+                rc = P.run(st, WT_PROF_GEMM_SOLVE, num.splan.launches[t.gemm].flops, 0,
+                           [&] { return num.splan.run(t.gemm, st); });
+                break;
             default: rc = WT_ERR_STATE;
             }
             if (rc) return rc;

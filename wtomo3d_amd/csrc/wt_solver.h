@@ -13,6 +13,7 @@
 struct wt_gemm_launch_rec {
     int64_t prob_off, tile_off, ntiles;
     int32_t nprob;
+    double flops;  // 8 M N K summed over the launch's problems
 };
 
 template <typename R>
@@ -29,6 +30,7 @@ struct wt_gemm_plan {
         cur.tile_off = (int64_t)tiles.size();
         cur.ntiles = 0;
         cur.nprob = 0;
+        cur.flops = 0;
         tiles.push_back(0);
     }
     void add(const cpx<R>* A, int64_t lda, const cpx<R>* B, int64_t ldb, cpx<R>* C, int64_t ldc, int64_t M,
@@ -40,6 +42,7 @@ struct wt_gemm_plan {
         p.M = (int32_t)M; p.N = (int32_t)N; p.K = (int32_t)K; p.mode = mode; p.bperm = bperm; p.pad = 0;
         probs.push_back(p);
         cur.ntiles += wt_gemm_tiles<R>(M, N);
+        cur.flops += 8.0 * (double)M * (double)N * (double)K;
         tiles.push_back(cur.ntiles);
         ++cur.nprob;
     }
@@ -92,7 +95,68 @@ struct wt_numeric {
     void release();
 };
 
+// Per-kernel-class timing with HIP events on the solver stream (enabled by
+// wt_solver_profile): launches, summed kernel time and algorithmic work.
+enum wt_prof_class {
+    WT_PROF_GEMM_FACTOR, WT_PROF_GEMM_SOLVE, WT_PROF_DIAG, WT_PROF_EXTADD, WT_PROF_FILL,
+    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_NCLASS
+};
+struct wt_prof {
+    bool on = false;
+    struct pend { int cls; double flops, bytes; hipEvent_t a, b; };
+    std::vector<pend> pending;
+    std::vector<hipEvent_t> pool;
+    int64_t launches[WT_PROF_NCLASS] = {};
+    double ms[WT_PROF_NCLASS] = {}, flops[WT_PROF_NCLASS] = {}, bytes[WT_PROF_NCLASS] = {};
+    hipEvent_t take()
+    {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    // records an event pair around fn() when enabled
+    template <typename F>
+    int run(hipStream_t s, int cls, double fl, double by, F&& fn)
+    {
+        if (!on) return fn();
+        hipEvent_t a = take(), b = take();
+        (void)hipEventRecord(a, s);
+        int rc = fn();
+        (void)hipEventRecord(b, s);
+        pending.push_back({cls, fl, by, a, b});
+        return rc;
+    }
+    // after the stream has been synchronized
+    void flush()
+    {
+        for (auto& p : pending) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) ms[p.cls] += t;
+            launches[p.cls]++;
+            flops[p.cls] += p.flops;
+            bytes[p.cls] += p.bytes;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset()
+    {
+        pending.clear();
+        for (int i = 0; i < WT_PROF_NCLASS; ++i) { launches[i] = 0; ms[i] = flops[i] = bytes[i] = 0; }
+    }
+    void release()
+    {
+        for (auto& p : pending) { pool.push_back(p.a); pool.push_back(p.b); }
+        pending.clear();
+        for (auto e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+    }
+};
+
 struct wt_solver_impl {
+    wt_prof prof;
     wt_symbolic S;
     hipStream_t stream = nullptr;
     int64_t n = 0, nnz = 0;

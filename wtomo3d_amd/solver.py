@@ -88,6 +88,23 @@ class Solver:
     def release(self, prec):
         check(lib().wt_solver_release(self._h, prec), "wt_solver_release")
 
+    PROFILE_CLASSES = ("gemm_factor", "gemm_solve", "diag_tile", "extend_add", "fill_scatter",
+                       "assemble", "residual", "reorder")
+
+    def profile(self, enable=True):
+        """Enable (and reset) per-kernel HIP-event timing on the solver stream."""
+        check(lib().wt_solver_profile(self._h, int(enable)), "wt_solver_profile")
+
+    def profile_read(self):
+        out = {}
+        for i, name in enumerate(self.PROFILE_CLASSES):
+            n = ctypes.c_int64()
+            ms, fl, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            check(lib().wt_solver_profile_read(self._h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl),
+                                               ctypes.byref(by)), "wt_solver_profile_read")
+            out[name] = dict(launches=n.value, ms=ms.value, flops=fl.value, bytes=by.value)
+        return out
+
     def solve(self, b, prec=32, tol=1e-8, max_iter=20):
         """b: (n,) or (n, nrhs) complex; returns (x, stats dict)."""
         b = np.asarray(b)
