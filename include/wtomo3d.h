@@ -103,6 +103,9 @@ int wt_solver_destroy(wt_solver *h);
  * *flops = sum over fronts of 8 (p^3/3 + p^2 u + p u^2). */
 int wt_solver_info(const wt_solver *h, int64_t *info, double *flops);
 
+/* The same statistics from the host symbolic analysis alone (no GPU). */
+int wt_symbolic_info(int nx, int ny, int nz, int leaf_nodes, int64_t *info, double *flops);
+
 /* HBM bytes the solver needs for a factorization in factor_prec and a
  * block solve of nrhs columns (the admission estimate). */
 int64_t wt_solver_bytes(const wt_solver *h, int factor_prec, int nrhs);
@@ -131,6 +134,10 @@ int wt_solver_release(wt_solver *h, int factor_prec);
  * Replaces STRUMPACK_solve (strumpack_finter.c:102) for a whole block. */
 int wt_solver_solve(wt_solver *h, int factor_prec, int nrhs, const void *b, void *x, int on_device,
                     double tol, int max_iter, wt_solve_stats *stats);
+
+/* Y = A X with the solver's FP64 CSR values, row-major n x nrhs blocks
+ * (host or device).  Used to form right-hand sides and residuals. */
+int wt_solver_apply(wt_solver *h, int nrhs, const void *x, void *y, int on_device);
 
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
  * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and

@@ -13,6 +13,7 @@ import pytest
 import wtomo3d_amd as W
 from wtomo3d_amd.solver import Solver
 from solve_cases import problem, rhs_block
+from helpers import rel_err_entries
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import solve_oracle as SO  # noqa: E402
@@ -21,6 +22,20 @@ pytestmark = pytest.mark.gpu
 
 # (nx, ny, nz, leaf nodes): single-level, multi-level, multi-tile separators
 CASES = [(5, 4, 6, 64), (10, 9, 11, 64), (16, 15, 14, 64), (12, 12, 12, 8), (9, 22, 7, 27)]
+
+
+def _uniform_all_pml(n=14):
+    """Uniform half-space where the 10-node PML covers the whole grid (the
+    24^3 plain-method setting, shrunk): stresses pivoting."""
+    import oracle_lib as O
+    vp = np.full((n, n, n), 6000.0)
+    lam, mu = O.lame(vp, vp / np.sqrt(3.0), np.full_like(vp, 2700.0), 64)
+    ipml, pmlf = O.pml_params(500.0, 5000.0, 1e-3, 64)
+    rc, vals = O.assemble(n, n, n, 500.0, 2 * np.pi * 0.2, lam, mu, np.full(n ** 3, 2700.0), 5000.0, pmlf, ipml,
+                          prec=64)
+    assert rc == 0
+    ir, jc = O.graph(n, n, n)
+    return lam, mu, ipml, pmlf, ir, jc, vals
 
 
 def _setup(nx, ny, nz, leaf, seed=1):
@@ -37,15 +52,20 @@ def _setup(nx, ny, nz, leaf, seed=1):
 def test_fp64_factor_solve_matches_superlu(case):
     nx, ny, nz, leaf = case
     s, A, vals = _setup(nx, ny, nz, leaf)
-    assert np.array_equal(s.values(), vals)  # FP64 assembly on the GPU == oracle
+    v = s.values()  # FP64 assembly on the GPU == oracle (to rounding)
+    assert np.max(np.abs(v - vals)) <= 1e-15 * np.max(np.abs(vals))
     B, Xt = rhs_block(A, 3, 11)
     Xo = SO.solve_superlu(A, B)
     s.factor(64)
     X, st = s.solve(B, prec=64, tol=0.0, max_iter=1)
     assert st["iterations"] == 1
+    # backward error at FP64 level; forward error as small as the
+    # independent direct solver's (the PML operator has cond ~1e8-1e9)
     assert st["rel_residual"] <= 1e-11
     assert SO.rel_residual(A, X, B) <= 1e-11
-    assert np.max(np.abs(X - Xo)) <= 1e-10 * np.max(np.abs(Xo))
+    err_oracle = np.max(np.abs(Xo - Xt)) / np.max(np.abs(Xt))
+    err_gpu = np.max(np.abs(X - Xt)) / np.max(np.abs(Xt))
+    assert err_gpu <= 10 * err_oracle + 1e-13
     s.close()
 
 
@@ -59,7 +79,9 @@ def test_fp32_factor_with_fp64_refinement(case):
     X, st = s.solve(B, prec=32, tol=1e-10, max_iter=30)
     assert st["rel_residual"] <= 1e-10
     assert SO.rel_residual(A, X, B) <= 1e-10
-    assert np.max(np.abs(X - Xo)) <= 1e-8 * np.max(np.abs(Xo))
+    err_oracle = np.max(np.abs(Xo - Xt)) / np.max(np.abs(Xt))
+    err_gpu = np.max(np.abs(X - Xt)) / np.max(np.abs(Xt))
+    assert err_gpu <= 10 * err_oracle + 1e-12
     assert st["rel_residual_first"] < 1e-2  # the complex64 factor alone is a good preconditioner
     s.close()
 
@@ -69,9 +91,9 @@ def test_block_solve_equals_column_solves():
     s, A, vals = _setup(10, 9, 11, 64)
     B, _ = rhs_block(A, 5, 13)
     s.factor(32)
-    Xb, _ = s.solve(B, prec=32, tol=1e-10, max_iter=30)
+    Xb, _ = s.solve(B, prec=32, tol=0.0, max_iter=3)
     for j in range(5):
-        xj, _ = s.solve(B[:, j], prec=32, tol=1e-10, max_iter=30)
+        xj, _ = s.solve(B[:, j], prec=32, tol=0.0, max_iter=3)
         assert np.array_equal(xj, Xb[:, j])
     s.close()
 
@@ -130,3 +152,21 @@ def test_strumpack_serial_finter_dropin():
     bad[5], bad[6] = bad[6], bad[5]
     L.strumpack_serial_finter(I(1), I(n), I(jc.size), P(ir), P(bad), P(vals), P(b), P(x), ctypes.byref(ierr))
     assert ierr.value == 1
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_all_pml_uniform_model(prec):
+    n = 14
+    lam, mu, ipml, pmlf, ir, jc, vals = _uniform_all_pml(n)
+    A = SO.csr(ir, jc, vals)
+    B, Xt = rhs_block(A, 2, 21)
+    Xo = SO.solve_superlu(A, B)
+    s = Solver(n, n, n)
+    s.set_model(lam, mu, np.full(n ** 3, 2700.0))
+    s.assemble(W._lib.WtGridParams(n, n, n, ipml, 1, 1, 500.0, 5000.0, pmlf, 2 * np.pi * 0.2, 0.0))
+    s.factor(prec)
+    X, st = s.solve(B, prec=prec, tol=1e-10, max_iter=30)
+    assert st["rel_residual"] <= 1e-10
+    err_oracle = np.max(np.abs(Xo - Xt)) / np.max(np.abs(Xt))
+    assert np.max(np.abs(X - Xt)) / np.max(np.abs(Xt)) <= 10 * err_oracle + 1e-12
+    s.close()

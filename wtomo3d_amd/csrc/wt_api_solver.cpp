@@ -73,7 +73,7 @@ int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b
         const double cs = sizeof(cpx<R>), blk = (double)n * nrhs;
         if ((rc = wt_solver_prepare_solve<R>(s, num, nrhs))) return rc;
         if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (16 + cs) + n * 8.0,
-                        [&] { return wt_to_elim<double, R>(r, s.iperm, num.y, n, nrhs, q); })))
+                        [&] { return wt_to_elim<double, R>(r, s.iperm, s.rscale, num.y, n, nrhs, q); })))
             return rc;
         if ((rc = wt_solver_solve_block<R>(s, num, nrhs, nullptr))) return rc;
         if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (32 + cs) + n * 8.0,
@@ -127,6 +127,24 @@ int wt_solver_create(int nx, int ny, int nz, int leaf_nodes, wt_solver** out)
         return rc;
     }
     *out = h;
+    return WT_OK;
+}
+
+int wt_symbolic_info(int nx, int ny, int nz, int leaf_nodes, int64_t* info, double* flops)
+{
+    if (!info) return WT_ERR_ARG;
+    wt_symbolic S;
+    int rc = wt_symbolic_build(nx, ny, nz, leaf_nodes > 0 ? leaf_nodes : 64, WT_NB, nullptr, nullptr, S);
+    if (rc) return rc;
+    info[0] = S.n;
+    info[1] = wt_nnz(nx, ny, nz);
+    info[2] = (int64_t)S.fronts.size();
+    info[3] = (int64_t)S.levels.size();
+    info[4] = S.max_p;
+    info[5] = S.max_u;
+    info[6] = S.factor_len;
+    info[7] = S.cb_len[0] + S.cb_len[1];
+    if (flops) *flops = S.flops;
     return WT_OK;
 }
 
@@ -248,6 +266,32 @@ int wt_solver_factor(wt_solver* h, int factor_prec)
     return WT_ERR_ARG;
 }
 
+int wt_solver_apply(wt_solver* h, int nrhs, const void* x, void* y, int on_device)
+{
+    if (!h || !x || !y || nrhs < 1) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    if (!s.have_vals) return WT_ERR_STATE;
+    const int chunk = nrhs < 64 ? nrhs : 64;
+    int rc;
+    if ((rc = ensure_ir(s, chunk))) return rc;
+    hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    hipMemcpyKind kout = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    const cpx<double>* xs = (const cpx<double>*)x;
+    cpx<double>* yd = (cpx<double>*)y;
+    for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+        const int nc = nrhs - c0 < chunk ? nrhs - c0 : chunk;
+        if (nc == nrhs) WT_HIP(hipMemcpyAsync(s.xb, xs, s.n * nc * 16, kin, s.stream));
+        else WT_HIP(hipMemcpy2DAsync(s.xb, nc * 16, xs + c0, (size_t)nrhs * 16, nc * 16, s.n, kin, s.stream));
+        if ((rc = wt_residual<double>(s.rowptr, s.colidx, s.vals, s.xb, nullptr, s.rb, s.partial, s.norms, s.n, nc,
+                                      s.stream)))
+            return rc;
+        if (nc == nrhs) WT_HIP(hipMemcpyAsync(yd, s.rb, s.n * nc * 16, kout, s.stream));
+        else WT_HIP(hipMemcpy2DAsync(yd + c0, (size_t)nrhs * 16, s.rb, nc * 16, nc * 16, s.n, kout, s.stream));
+        WT_HIP(hipStreamSynchronize(s.stream));
+    }
+    return WT_OK;
+}
+
 int wt_solver_profile(wt_solver* h, int enable)
 {
     if (!h) return WT_ERR_ARG;
@@ -305,8 +349,9 @@ int wt_solver_solve(wt_solver* h, int factor_prec, int nrhs, const void* b, void
             bblk = src;
             xblk = dstx;
         } else {
-            WT_HIP(hipMemcpy2DAsync(s.bb, nc * 16, src + c0, (size_t)nrhs * 16, nc * 16, s.n,
-                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.stream));
+            hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            if (nc == nrhs) WT_HIP(hipMemcpyAsync(s.bb, src, s.n * nc * 16, k, s.stream));
+            else WT_HIP(hipMemcpy2DAsync(s.bb, nc * 16, src + c0, (size_t)nrhs * 16, nc * 16, s.n, k, s.stream));
             bblk = s.bb;
             xblk = s.xb;
         }
@@ -314,9 +359,11 @@ int wt_solver_solve(wt_solver* h, int factor_prec, int nrhs, const void* b, void
         rc = f32 ? refine<float>(s, s.f32, nc, bblk, xblk, tol, max_iter, &cs)
                  : refine<double>(s, s.f64, nc, bblk, xblk, tol, max_iter, &cs);
         if (rc) return rc;
-        if (xblk == s.xb)
-            WT_HIP(hipMemcpy2DAsync(dstx + c0, (size_t)nrhs * 16, s.xb, nc * 16, nc * 16, s.n,
-                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s.stream));
+        if (xblk == s.xb) {
+            hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+            if (nc == nrhs) WT_HIP(hipMemcpyAsync(dstx, s.xb, s.n * nc * 16, k, s.stream));
+            else WT_HIP(hipMemcpy2DAsync(dstx + c0, (size_t)nrhs * 16, s.xb, nc * 16, nc * 16, s.n, k, s.stream));
+        }
         WT_HIP(hipStreamSynchronize(s.stream));
         agg.iterations = cs.iterations > agg.iterations ? cs.iterations : agg.iterations;
         agg.rel_residual = cs.rel_residual > agg.rel_residual ? cs.rel_residual : agg.rel_residual;

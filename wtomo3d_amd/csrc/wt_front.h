@@ -19,6 +19,10 @@ template <typename R>
 int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s);
 template <typename Ri, typename R>
 int wt_scatter(const cpx<Ri>* vals, const int64_t* dst, int64_t nnz, cpx<R>* arena, hipStream_t s);
+int wt_row_scale(const int32_t* rowptr, const cpx<double>* vals, int64_t n, double* scale, hipStream_t s);
+template <typename R>
+int wt_scatter_rows(const int32_t* rowptr, const cpx<double>* vals, const int64_t* dst, const double* scale,
+                    int64_t n, cpx<R>* arena, hipStream_t s);
 template <typename R>
 int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
                   const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, hipStream_t s);

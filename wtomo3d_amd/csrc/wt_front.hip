@@ -31,6 +31,48 @@ __global__ void scatter_kernel(const cpx<Ri>* __restrict__ vals, const int64_t* 
     }
 }
 
+// Row equilibration: scale[i] = 2^-e with 2^e <= max_j max(|re a_ij|,
+// |im a_ij|) < 2^(e+1), an exact power of two, so the scaled system D A x =
+// D b carries no extra rounding.  One wave per row.
+template <typename Ri>
+__global__ __launch_bounds__(256) void row_scale_kernel(const int32_t* __restrict__ rowptr,
+                                                        const cpx<Ri>* __restrict__ vals, int64_t n,
+                                                        double* __restrict__ scale)
+{
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double m = 0.0;
+    for (int32_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) {
+        double a = fabs((double)vals[e].re), b = fabs((double)vals[e].im);
+        m = fmax(m, fmax(a, b));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (lane == 0) {
+        int ex = 0;
+        (void)frexp(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+        scale[row] = m > 0.0 ? ldexp(1.0, 1 - ex) : 1.0;
+    }
+}
+
+// arena[dst[e]] = scale[row] * vals[e] for every entry of the CSR rows.
+template <typename Ri, typename R>
+__global__ __launch_bounds__(256) void scatter_rows_kernel(const int32_t* __restrict__ rowptr,
+                                                           const cpx<Ri>* __restrict__ vals,
+                                                           const int64_t* __restrict__ dst,
+                                                           const double* __restrict__ scale, int64_t n,
+                                                           cpx<R>* __restrict__ arena)
+{
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double sc = scale[row];
+    for (int32_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) {
+        cpx<Ri> v = vals[e];
+        arena[dst[e]] = cpx<R>((R)(sc * (double)v.re), (R)(sc * (double)v.im));
+    }
+}
+
 constexpr int EA_COLS = 8;  // CB columns per extend-add block
 
 // Adds the contribution blocks of the listed children into their parents.
@@ -257,6 +299,26 @@ int wt_scatter(const cpx<Ri>* vals, const int64_t* dst, int64_t nnz, cpx<R>* are
     return WT_OK;
 }
 
+int wt_row_scale(const int32_t* rowptr, const cpx<double>* vals, int64_t n, double* scale, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL(row_scale_kernel<double>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, rowptr, vals, n,
+                       scale);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+template <typename R>
+int wt_scatter_rows(const int32_t* rowptr, const cpx<double>* vals, const int64_t* dst, const double* scale,
+                    int64_t n, cpx<R>* arena, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL((scatter_rows_kernel<double, R>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, rowptr,
+                       vals, dst, scale, n, arena);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
 template <typename R>
 int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
                   const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, hipStream_t s)
@@ -300,6 +362,11 @@ int wt_solve_gather_upd(const wt_dfront* fr, const int32_t* list, int nlist, con
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
+
+template int wt_scatter_rows<float>(const int32_t*, const cpx<double>*, const int64_t*, const double*, int64_t,
+                                    cpx<float>*, hipStream_t);
+template int wt_scatter_rows<double>(const int32_t*, const cpx<double>*, const int64_t*, const double*, int64_t,
+                                     cpx<double>*, hipStream_t);
 
 #define WT_INST(R)                                                                                              \
     template int wt_fill_zero<R>(cpx<R>*, int64_t, hipStream_t);                                                \

@@ -168,7 +168,7 @@ void wt_solver_free(wt_solver_impl& s)
     s.f32.release();
     s.f64.release();
     dfree(s.rowptr); dfree(s.colidx); dfree(s.dst); dfree(s.iperm); dfree(s.upd); dfree(s.relmap);
-    dfree(s.fronts); dfree(s.lists); dfree(s.lam); dfree(s.mu); dfree(s.rho); dfree(s.vals); dfree(s.d_err);
+    dfree(s.fronts); dfree(s.lists); dfree(s.rscale); dfree(s.lam); dfree(s.mu); dfree(s.rho); dfree(s.vals); dfree(s.d_err);
     dfree(s.xb); dfree(s.rb); dfree(s.bb); dfree(s.partial); dfree(s.norms);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
@@ -287,8 +287,12 @@ int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
     if ((rc = P.run(st, WT_PROF_FILL, 0, S.factor_len * cs,
                     [&] { return wt_fill_zero<R>(num.factor, S.factor_len, st); })))
         return rc;
+    if (!s.rscale && (rc = dalloc(&s.rscale, s.n))) return rc;
+    if ((rc = P.run(st, WT_PROF_FILL, 0, s.nnz * 16.0 + s.n * 12.0,
+                    [&] { return wt_row_scale(s.rowptr, s.vals, s.n, s.rscale, st); })))
+        return rc;
     if ((rc = P.run(st, WT_PROF_FILL, 0, s.nnz * (16.0 + 8.0 + cs),
-                    [&] { return wt_scatter<double, R>(s.vals, s.dst, s.nnz, num.factor, st); })))
+                    [&] { return wt_scatter_rows<R>(s.rowptr, s.vals, s.dst, s.rscale, s.n, num.factor, st); })))
         return rc;
     for (const wt_step& t : num.fsteps) {
         switch (t.kind) {

@@ -183,6 +183,7 @@ struct wt_solver_impl {
     cpx<double>* mu = nullptr;
     double* rho = nullptr;
     cpx<double>* vals = nullptr;  // CSR values, complex128
+    double* rscale = nullptr;     // row equilibration of the factored matrix
     bool have_vals = false;
     int* d_err = nullptr;
     // refinement buffers (row-major n x 64)

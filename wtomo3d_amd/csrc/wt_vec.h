@@ -6,7 +6,8 @@
 #include "wt_common.h"
 
 template <typename Rs, typename R>
-int wt_to_elim(const cpx<Rs>* src, const int64_t* iperm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s);
+int wt_to_elim(const cpx<Rs>* src, const int64_t* iperm, const double* scale, cpx<R>* y, int64_t n, int nrhs,
+               hipStream_t s);
 template <typename R, typename Rd>
 int wt_from_elim(const cpx<R>* y, const int64_t* iperm, cpx<Rd>* dst, int64_t n, int nrhs, int accumulate,
                  hipStream_t s);

@@ -12,10 +12,11 @@ namespace {
 
 constexpr int TP = 32;  // transpose tile
 
-// y[i' + c n] = (R) src[iperm[i'] * nrhs + c]
+// y[i' + c n] = (R) (scale[o] src[o * nrhs + c]),  o = iperm[i']
 template <typename Rs, typename R>
 __global__ __launch_bounds__(256) void to_elim_kernel(const cpx<Rs>* __restrict__ src, const int64_t* __restrict__ iperm,
-                                                      cpx<R>* __restrict__ y, int64_t n, int nrhs)
+                                                      const double* __restrict__ scale, cpx<R>* __restrict__ y,
+                                                      int64_t n, int nrhs)
 {
     __shared__ cpx<R> t[TP][TP + 1];
     const int64_t i0 = (int64_t)blockIdx.x * TP;
@@ -25,8 +26,10 @@ __global__ __launch_bounds__(256) void to_elim_kernel(const cpx<Rs>* __restrict_
         int64_t i = i0 + k;
         int c = c0 + tx;
         if (i < n && c < nrhs) {
-            cpx<Rs> v = src[iperm[i] * nrhs + c];
-            t[k][tx] = cpx<R>((R)v.re, (R)v.im);
+            const int64_t o = iperm[i];
+            const double sc = scale ? scale[o] : 1.0;
+            cpx<Rs> v = src[o * nrhs + c];
+            t[k][tx] = cpx<R>((R)(sc * (double)v.re), (R)(sc * (double)v.im));
         }
     }
     __syncthreads();
@@ -83,13 +86,15 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
     const int64_t row = (int64_t)blockIdx.x * SP_ROWS + wave;
     double nrm = 0.0;
     if (row < n && c < nrhs) {
-        cpx<double> acc = b[row * nrhs + c];
+        // b == null: r = A x instead of b - A x
+        cpx<double> acc = b ? b[row * nrhs + c] : cpx<double>(0.0, 0.0);
+        const double sg = b ? -1.0 : 1.0;
         const int32_t e1 = rowptr[row + 1];
         for (int32_t e = rowptr[row]; e < e1; ++e) {
             const cpx<Ra> a = vals[e];
             const cpx<double> xv = x[(int64_t)colidx[e] * nrhs + c];
-            acc.re -= (double)a.re * xv.re - (double)a.im * xv.im;
-            acc.im -= (double)a.re * xv.im + (double)a.im * xv.re;
+            acc.re += sg * ((double)a.re * xv.re - (double)a.im * xv.im);
+            acc.im += sg * ((double)a.re * xv.im + (double)a.im * xv.re);
         }
         r[row * nrhs + c] = acc;
         nrm = acc.re * acc.re + acc.im * acc.im;
@@ -174,10 +179,11 @@ static unsigned grid1(int64_t n)
 }
 
 template <typename Rs, typename R>
-int wt_to_elim(const cpx<Rs>* src, const int64_t* iperm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s)
+int wt_to_elim(const cpx<Rs>* src, const int64_t* iperm, const double* scale, cpx<R>* y, int64_t n, int nrhs,
+               hipStream_t s)
 {
     dim3 grid((unsigned)((n + TP - 1) / TP), (nrhs + TP - 1) / TP);
-    hipLaunchKernelGGL((to_elim_kernel<Rs, R>), grid, dim3(256), 0, s, src, iperm, y, n, nrhs);
+    hipLaunchKernelGGL((to_elim_kernel<Rs, R>), grid, dim3(256), 0, s, src, iperm, scale, y, n, nrhs);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
@@ -243,8 +249,10 @@ int wt_widen_c64(const cpx<float>* a, cpx<double>* b, int64_t n, hipStream_t s)
     return WT_OK;
 }
 
-template int wt_to_elim<double, float>(const cpx<double>*, const int64_t*, cpx<float>*, int64_t, int, hipStream_t);
-template int wt_to_elim<double, double>(const cpx<double>*, const int64_t*, cpx<double>*, int64_t, int, hipStream_t);
+template int wt_to_elim<double, float>(const cpx<double>*, const int64_t*, const double*, cpx<float>*, int64_t, int,
+                                       hipStream_t);
+template int wt_to_elim<double, double>(const cpx<double>*, const int64_t*, const double*, cpx<double>*, int64_t,
+                                        int, hipStream_t);
 template int wt_from_elim<float, double>(const cpx<float>*, const int64_t*, cpx<double>*, int64_t, int, int,
                                          hipStream_t);
 template int wt_from_elim<double, double>(const cpx<double>*, const int64_t*, cpx<double>*, int64_t, int, int,

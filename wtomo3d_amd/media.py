@@ -1,0 +1,73 @@
+"""Synthetic elastic media and sources for the benchmark configurations.
+
+Velocity/density layouts follow the reference's model arrays: (nz, nx, ny)
+Fortran order, i.e. numpy arrays of shape (ny, nx, nz) in C order (z
+fastest), as MODELIO_READ_MODEL stores them (modelio.f90:98-113).
+"""
+import numpy as np
+
+
+def layered(nx, ny, nz, dx, seed, nlayers=6):
+    """1-D layered model: seeded random layer thicknesses, Vp uniform in
+    4-8 km/s increasing with depth, Vs = Vp/1.73, Gardner density
+    rho = 310 Vp^0.25 (kg/m^3, Vp in m/s).  Returns (vp, vs, rho) 1-D in z."""
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz) * dx
+    tops = np.sort(rng.uniform(0.0, nz * dx, nlayers - 1))
+    vps = np.sort(rng.uniform(4000.0, 8000.0, nlayers))
+    vp = vps[np.searchsorted(tops, z, side="right")]
+    return vp, vp / 1.73, 310.0 * vp ** 0.25
+
+
+def gaussian_field(shape, dx, corr_len, std, seed):
+    """Zero-mean Gaussian random field with Gaussian covariance of the given
+    correlation length, scaled to the requested standard deviation
+    (spectral filtering of seeded white noise)."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n, d=dx) * 2 * np.pi for n in shape]
+    K2 = k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2
+    amp = np.exp(-K2 * corr_len ** 2 / 8.0)  # sqrt of the Gaussian spectrum
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * amp))
+    f -= f.mean()
+    return f * (std / f.std())
+
+
+def heterogeneous(nx, ny, nz, dx, seed, std=0.05, corr_len=2000.0):
+    """Layered background plus a Gaussian random field on Vp and Vs (std,
+    correlation length) with the density perturbed by 0.3 times the Vp
+    perturbation.  Returns (vp, vs, rho) of shape (ny, nx, nz)."""
+    vp1, vs1, rho1 = layered(nx, ny, nz, dx, seed)
+    d = gaussian_field((ny, nx, nz), dx, corr_len, std, seed + 7)
+    vp = vp1[None, None, :] * (1.0 + d)
+    vs = vs1[None, None, :] * (1.0 + d)
+    rho = rho1[None, None, :] * (1.0 + 0.3 * d)
+    return vp, vs, rho
+
+
+def lame(vp, vs, rho):
+    """(lambda, mu) complex128 and rho float64, flattened z-fastest, formed
+    as MODELIO_READ_MODEL does (modelio.f90:78-92)."""
+    vp = np.asarray(vp, np.float64).ravel()
+    vs = np.asarray(vs, np.float64).ravel()
+    rho = np.asarray(rho, np.float64).ravel()
+    mu = (vs * vs) * rho
+    lam = (vp * vp) * rho - 2.0 * mu
+    return lam.astype(np.complex128), mu.astype(np.complex128), rho
+
+
+def plane_wave_fields(nx, ny, nz, dx, omega, vp, nrhs, seed, inc=(10.0, 35.0)):
+    """Displacement fields of homogeneous P plane waves arriving from below
+    (incidence uniform in `inc` degrees, azimuth uniform in 0-360), one per
+    column: u(r) = d exp(-i omega d.r / vp), d the propagation direction
+    (z positive down).  Row-major (n, nrhs) complex128 in the reference's
+    unknown order (u, v, w per node, z fastest, then x, then y)."""
+    rng = np.random.default_rng(seed)
+    th = np.deg2rad(rng.uniform(inc[0], inc[1], nrhs))
+    ph = np.deg2rad(rng.uniform(0.0, 360.0, nrhs))
+    d = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)], axis=1)  # (nrhs, 3)
+    iy, ix, iz = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    r = np.stack([ix.ravel() * dx, iy.ravel() * dx, iz.ravel() * dx], axis=1)  # nodes, z fastest
+    phase = np.exp(-1j * (omega / vp) * (r @ d.T))  # (nodes, nrhs)
+    U = phase[:, None, :] * d.T[None, :, :]        # (nodes, 3, nrhs)
+    return np.ascontiguousarray(U.reshape(-1, nrhs))

@@ -82,6 +82,15 @@ class Solver:
         check(lib().wt_solver_get_values(self._h, _p(v)), "wt_solver_get_values")
         return v
 
+    def apply(self, x):
+        """A x with the FP64 CSR values (x: (n,) or (n, nrhs))."""
+        x = np.asarray(x)
+        one = x.ndim == 1
+        X = np.ascontiguousarray(x.reshape(self.n, -1), np.complex128)
+        Y = np.zeros_like(X)
+        check(lib().wt_solver_apply(self._h, X.shape[1], _p(X), _p(Y), 0), "wt_solver_apply")
+        return Y[:, 0] if one else Y
+
     def factor(self, prec=32):
         check(lib().wt_solver_factor(self._h, prec), "wt_solver_factor")
 
