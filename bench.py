@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain complex128 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel 5 times on BYTES (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -91,6 +93,15 @@ def main():
               "of frequencies / models / right-hand-side blocks is not built", file=sys.stderr)
         sys.exit(2)
     import wtomo3d_amd as W
+    if a.copy_probe:
+        L = W.lib()
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        W._lib.check(L.wt_dev_alloc(a.copy_probe, ctypes.byref(src)), "alloc")
+        W._lib.check(L.wt_dev_alloc(a.copy_probe, ctypes.byref(dst)), "alloc")
+        for _ in range(5):
+            W._lib.check(L.wt_dev_copy(dst, src, a.copy_probe), "copy")
+        print(json.dumps({"copy_probe_bytes": a.copy_probe, "launches": 5}))
+        return
     from wtomo3d_amd import media
     from wtomo3d_amd.solver import Solver
 

@@ -175,6 +175,9 @@ int wt_dev_free(void *d_ptr);
 int wt_h2d(void *d_dst, const void *src, size_t bytes);
 int wt_d2h(void *dst, const void *d_src, size_t bytes);
 int wt_dev_sync(void);
+/* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0): the
+ * reference point for HBM traffic counters. */
+int wt_dev_copy(void *d_dst, const void *d_src, size_t bytes);
 /* Number of visible GPUs (0 without one). */
 int wt_device_count(void);
 

@@ -73,6 +73,8 @@ def _declare(L):
     L.wt_gemm_host.restype = ctypes.c_int
     L.wt_symbolic_info.argtypes = [i32] * 4 + [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.wt_symbolic_info.restype = ctypes.c_int
+    L.wt_dev_copy.argtypes = [_vp, _vp, ctypes.c_size_t]
+    L.wt_dev_copy.restype = ctypes.c_int
     L.wt_solver_apply.argtypes = [_vp, i32, _vp, _vp, i32]
     L.wt_solver_apply.restype = ctypes.c_int
     L.wt_solver_profile.argtypes = [_vp, i32]

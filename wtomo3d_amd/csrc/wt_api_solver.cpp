@@ -292,6 +292,15 @@ int wt_solver_apply(wt_solver* h, int nrhs, const void* x, void* y, int on_devic
     return WT_OK;
 }
 
+int wt_dev_copy(void* d_dst, const void* d_src, size_t bytes)
+{
+    if (!d_dst || !d_src || bytes % 16) return WT_ERR_ARG;
+    int rc = wt_copy<double>((const cpx<double>*)d_src, (cpx<double>*)d_dst, (int64_t)(bytes / 16), nullptr);
+    if (rc) return rc;
+    WT_HIP(hipDeviceSynchronize());
+    return WT_OK;
+}
+
 int wt_solver_profile(wt_solver* h, int enable)
 {
     if (!h) return WT_ERR_ARG;
