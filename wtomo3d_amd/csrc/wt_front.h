@@ -7,6 +7,10 @@
 
 // Diagonal tile width: pivoting is restricted to rows of one tile.
 #define WT_NB 64
+// Super-panel width of the two-level blocked factorization of each pivot
+// block: trailing updates beyond the super-panel are deferred to one
+// rank-WT_NB2 GEMM (a multiple of WT_NB).
+#define WT_NB2 512
 
 // Device copy of the per-front offsets of wt_front (wt_symbolic.h).
 struct wt_dfront {

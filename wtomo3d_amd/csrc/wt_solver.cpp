@@ -188,6 +188,7 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
     const wt_symbolic& S = s.S;
     const int nl = (int)S.levels.size();
     const int64_t nb = S.nb;
+    const int64_t NB2 = WT_NB2;
     num.fplan.release();
     num.fsteps.clear();
     for (int l = nl - 1; l >= 0; --l) {
@@ -224,16 +225,37 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
             }
             int g = num.fplan.end();
             if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            // updates by this tile: inside the current super-panel of NB2
+            // pivots (all rows), and beyond it for the super-panel's own rows
+            // (remaining pivot columns and U12)
             num.fplan.begin();
             for (int i = 0; i < cnt; ++i) {
                 const wt_front& F = S.fronts[lst[i]];
                 const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
+                const int64_t K1 = std::min((kb / NB2 + 1) * NB2, p);
                 cpx<R>* L = num.factor + F.Loff;
                 cpx<R>* U = num.factor + F.Uoff;
-                // trailing update of the pivot columns (all rows) and of U12
                 num.fplan.add(L + kb * ld + kb + w, ld, L + (kb + w) * ld + kb, ld, L + (kb + w) * ld + kb + w, ld,
-                              p + u - kb - w, p - kb - w, w, WT_GEMM_SUB);
-                num.fplan.add(L + kb * ld + kb + w, ld, U + kb, p, U + kb + w, p, p - kb - w, u, w, WT_GEMM_SUB);
+                              p + u - kb - w, K1 - kb - w, w, WT_GEMM_SUB);
+                num.fplan.add(L + kb * ld + kb + w, ld, L + K1 * ld + kb, ld, L + K1 * ld + kb + w, ld,
+                              K1 - kb - w, p - K1, w, WT_GEMM_SUB);
+                num.fplan.add(L + kb * ld + kb + w, ld, U + kb, p, U + kb + w, p, K1 - kb - w, u, w, WT_GEMM_SUB);
+            }
+            g = num.fplan.end();
+            if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            // super-panel complete: one rank-(K1-K0) update of everything to
+            // the right of and below it
+            num.fplan.begin();
+            for (int i = 0; i < cnt; ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
+                const int64_t K0 = (kb / NB2) * NB2, K1 = std::min(K0 + NB2, p);
+                if (kb + w != K1 || K1 >= p) continue;
+                cpx<R>* L = num.factor + F.Loff;
+                cpx<R>* U = num.factor + F.Uoff;
+                num.fplan.add(L + K0 * ld + K1, ld, L + K1 * ld + K0, ld, L + K1 * ld + K1, ld, p + u - K1, p - K1,
+                              K1 - K0, WT_GEMM_SUB);
+                num.fplan.add(L + K0 * ld + K1, ld, U + K0, p, U + K1, p, p - K1, u, K1 - K0, WT_GEMM_SUB);
             }
             g = num.fplan.end();
             if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
@@ -332,7 +354,7 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
 {
     const wt_symbolic& S = s.S;
     const int nl = (int)S.levels.size();
-    const int64_t nb = S.nb, n = S.n;
+    const int64_t nb = S.nb, n = S.n, NB2 = WT_NB2;
     dfree(num.y);
     dfree(num.w[0]);
     dfree(num.w[1]);
@@ -368,14 +390,30 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
             }
             int g = num.splan.end();
             if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            // rows of the same super-panel below the tile
+            num.splan.begin();
+            for (int i = 0; i < cnt; ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t p = F.p, ld = p + F.u, w = std::min(nb, p - kb);
+                const int64_t K1 = std::min((kb / NB2 + 1) * NB2, p);
+                const cpx<R>* L = num.factor + F.Loff;
+                cpx<R>* yt = y + F.pstart + kb;
+                num.splan.add(L + kb * ld + kb + w, ld, yt, n, yt + w, n, K1 - kb - w, nrhs, w, WT_GEMM_SUB);
+            }
+            g = num.splan.end();
+            if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            // super-panel done: remaining pivot rows and the update rows W
             num.splan.begin();
             for (int i = 0; i < cnt; ++i) {
                 const wt_front& F = S.fronts[lst[i]];
                 const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
+                const int64_t K0 = (kb / NB2) * NB2, K1 = std::min(K0 + NB2, p);
+                if (kb + w != K1) continue;
                 const cpx<R>* L = num.factor + F.Loff;
-                cpx<R>* yt = y + F.pstart + kb;
-                num.splan.add(L + kb * ld + kb + w, ld, yt, n, yt + w, n, p - kb - w, nrhs, w, WT_GEMM_SUB);
-                num.splan.add(L + kb * ld + p, ld, yt, n, num.w[par] + F.Woff * nrhs, u, u, nrhs, w, WT_GEMM_SUB);
+                cpx<R>* ys = y + F.pstart + K0;
+                num.splan.add(L + K0 * ld + K1, ld, ys, n, y + F.pstart + K1, n, p - K1, nrhs, K1 - K0, WT_GEMM_SUB);
+                num.splan.add(L + K0 * ld + p, ld, ys, n, num.w[par] + F.Woff * nrhs, u, u, nrhs, K1 - K0,
+                              WT_GEMM_SUB);
             }
             g = num.splan.end();
             if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
@@ -412,14 +450,30 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
             }
             int g = num.splan.end();
             if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            // rows of the same super-panel above the tile
             num.splan.begin();
             for (int i = 0; i < (int)ps.size(); ++i) {
                 const wt_front& F = S.fronts[lst[i]];
                 const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
-                if (kb <= 0) continue;
-                const int64_t p = F.p, ld = p + F.u, w = std::min(nb, p - kb);
+                if (kb < 0) continue;
+                const int64_t p = F.p, ld = p + F.u, w = std::min(nb, p - kb), K0 = (kb / NB2) * NB2;
                 const cpx<R>* L = num.factor + F.Loff;
-                num.splan.add(L + kb * ld, ld, y + F.pstart + kb, n, y + F.pstart, n, kb, nrhs, w, WT_GEMM_SUB);
+                num.splan.add(L + kb * ld + K0, ld, y + F.pstart + kb, n, y + F.pstart + K0, n, kb - K0, nrhs, w,
+                              WT_GEMM_SUB);
+            }
+            g = num.splan.end();
+            if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+            // super-panel done: all pivot rows above it
+            num.splan.begin();
+            for (int i = 0; i < (int)ps.size(); ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
+                const int64_t K0 = kb >= 0 ? (kb / NB2) * NB2 : -1;
+                if (kb < 0 || kb != K0 || K0 == 0) continue;
+                const int64_t p = F.p, ld = p + F.u, K1 = std::min(K0 + NB2, p);
+                const cpx<R>* L = num.factor + F.Loff;
+                num.splan.add(L + K0 * ld, ld, y + F.pstart + K0, n, y + F.pstart, n, K0, nrhs, K1 - K0,
+                              WT_GEMM_SUB);
             }
             g = num.splan.end();
             if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
