@@ -143,7 +143,7 @@ int wt_solver_apply(wt_solver *h, int nrhs, const void *x, void *y, int on_devic
  * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and
  * contribution-block updates), 1 solve GEMMs, 2 diagonal-tile LU, 3
  * extend-add, 4 fills / value scatter, 5 assembly, 6 FP64 residual, 7
- * reorder / gather.  _read returns launches, summed kernel milliseconds,
+ * reorder / gather, 8 super-panel triangular solves.  _read returns launches, summed kernel milliseconds,
  * algorithmic flops (8 M N K per complex GEMM, 8 per residual complex
  * multiply-add) and algorithmic bytes.  No counterpart in the reference
  * (STRUMPACK's verbose statistics, strumpack_finter.c:32-33, are printed). */

@@ -37,7 +37,8 @@ def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
-@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (33, 500, 17)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (33, 500, 17),
+                                   (600, 64, 200), (513, 37, 70)])
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_gemm_modes_shapes(prec, shape, mode):
     M, N, K = shape

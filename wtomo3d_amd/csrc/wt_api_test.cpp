@@ -26,7 +26,8 @@ int run_gemm(int M, int N, int K, const void* A, int64_t lda, const void* B, int
         wt_gemm_prob<R> p;
         p.A = dA; p.B = dB; p.C = dC; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
         p.M = M; p.N = N; p.K = K; p.mode = mode; p.bperm = dp; p.pad = 0;
-        int64_t ts[2] = {0, wt_gemm_tiles<R>(M, N)};
+        const int var = N <= 64 ? 1 : 0;
+        int64_t ts[2] = {0, wt_gemm_tiles<R>(M, N, var)};
         if (hipMemcpy(dA, A, na * cs, hipMemcpyHostToDevice) || hipMemcpy(dB, B, nbv * cs, hipMemcpyHostToDevice) ||
             (bperm && hipMemcpy(dp, bperm, K * 4, hipMemcpyHostToDevice)) ||
             hipMemcpy(dprob, &p, sizeof(p), hipMemcpyHostToDevice) ||
@@ -36,7 +37,7 @@ int run_gemm(int M, int N, int K, const void* A, int64_t lda, const void* B, int
             for (int r = 0; r < reps && rc == WT_OK; ++r) {
                 if (hipMemcpy(dC, C, nc * cs, hipMemcpyHostToDevice)) { rc = WT_ERR_HIP; break; }
                 (void)hipEventRecord(e0, 0);
-                rc = wt_gemm_launch<R>(dprob, dts, 1, ts[1], 0);
+                rc = wt_gemm_launch<R>(dprob, dts, 1, ts[1], var, 0);
                 (void)hipEventRecord(e1, 0);
             }
             float t = 0;

@@ -39,3 +39,6 @@ int wt_solve_gather(const wt_dfront* fr, const int32_t* kids, int nkids, const i
 template <typename R>
 int wt_solve_gather_upd(const wt_dfront* fr, const int32_t* list, int nlist, const int64_t* upd_idx,
                         const cpx<R>* x, cpx<R>* work, int64_t n, int nrhs, hipStream_t s);
+template <typename R>
+int wt_sp_solve(int fwd, const wt_dfront* fr, const int32_t* ent, int nent, const cpx<R>* factor,
+                const cpx<R>* tiles, const int32_t* perm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s);

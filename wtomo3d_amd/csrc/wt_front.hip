@@ -226,6 +226,209 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restr
     }
 }
 
+// Same contract as diag_tile_kernel, done by a single wave: lane = row
+// during the elimination (pivot search by cross-lane reduction, row swaps
+// by lane = column), lane = column for the triangular inverses.  With one
+// wave per tile the barriers only order LDS accesses.
+template <typename R, int NB>
+__global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __restrict__ fr,
+                                                           const int32_t* __restrict__ list, int64_t kb,
+                                                           cpx<R>* __restrict__ factor, cpx<R>* __restrict__ tiles,
+                                                           int32_t* __restrict__ perm, int* __restrict__ status)
+{
+    static_assert(NB == 64, "one lane per tile row");
+    __shared__ cpx<R> T[NB][NB + 1];  // T[col][row]
+    __shared__ cpx<R> X[NB][NB + 1];
+    const wt_dfront F = fr[list[blockIdx.x]];
+    if (kb >= F.p) return;
+    const int w = (int)(F.p - kb < NB ? F.p - kb : NB);
+    const int64_t ld = F.p + F.u;
+    cpx<R>* base = factor + F.Loff + kb * ld + kb;
+    const int lane = threadIdx.x;
+    for (int c = 0; c < w; ++c)
+        if (lane < w) T[c][lane] = base[(int64_t)c * ld + lane];
+    int myperm = lane;  // tile row that ends at position `lane`
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        R a = R(-1);
+        if (lane >= j && lane < w) a = fabs(T[j][lane].re) + fabs(T[j][lane].im);
+        int bi = lane;
+        for (int off = 32; off > 0; off >>= 1) {
+            R oa = __shfl_xor(a, off);
+            int oi = __shfl_xor(bi, off);
+            if (oa > a || (oa == a && oi < bi)) { a = oa; bi = oi; }
+        }
+        const int pv = bi;  // uniform across the wave
+        if (pv != j) {
+            if (lane < w) {
+                cpx<R> t0 = T[lane][j];
+                T[lane][j] = T[lane][pv];
+                T[lane][pv] = t0;
+            }
+            const int pj = __shfl(myperm, j), pp = __shfl(myperm, pv);
+            if (lane == j) myperm = pp;
+            if (lane == pv) myperm = pj;
+        }
+        __syncthreads();
+        cpx<R> d = T[j][j];
+        if (d.re == R(0) && d.im == R(0)) {
+            if (lane == 0) atomicOr(status, 1);
+            d = cpx<R>(R(1e-30), R(0));
+        }
+        const cpx<R> dinv = cdiv(cpx<R>(R(1), R(0)), d);
+        __syncthreads();
+        if (lane == j) T[j][j] = d;
+        if (lane > j && lane < w) {
+            const cpx<R> l = T[j][lane] * dinv;
+            T[j][lane] = l;
+            for (int c = j + 1; c < w; ++c) T[c][lane] = T[c][lane] - l * T[c][j];
+        }
+        __syncthreads();
+    }
+    for (int c = 0; c < w; ++c)
+        if (lane < w) base[(int64_t)c * ld + lane] = T[c][lane];
+    if (lane < w) perm[F.pivoff + kb + lane] = myperm;
+    cpx<R>* Li = tiles + F.tileoff + (kb / NB) * 2 * NB * NB;
+    cpx<R>* Ui = Li + NB * NB;
+    // unit-lower inverse, lane = column
+    if (lane < w) {
+        const int c = lane;
+        for (int i = 0; i < c; ++i) X[c][i] = cpx<R>(R(0), R(0));
+        X[c][c] = cpx<R>(R(1), R(0));
+        for (int i = c + 1; i < w; ++i) {
+            cpx<R> s(R(0), R(0));
+            for (int k = c; k < i; ++k) s = s + T[k][i] * X[c][k];
+            X[c][i] = -s;
+        }
+    }
+    __syncthreads();
+    for (int c = 0; c < w; ++c)
+        if (lane < w) Li[(int64_t)c * NB + lane] = X[c][lane];
+    __syncthreads();
+    // upper inverse, lane = column
+    if (lane < w) {
+        const int c = lane;
+        for (int i = c + 1; i < w; ++i) X[c][i] = cpx<R>(R(0), R(0));
+        X[c][c] = cdiv(cpx<R>(R(1), R(0)), T[c][c]);
+        for (int i = c - 1; i >= 0; --i) {
+            cpx<R> s(R(0), R(0));
+            for (int k = i + 1; k <= c; ++k) s = s + T[k][i] * X[c][k];
+            X[c][i] = -cdiv(s, T[i][i]);
+        }
+    }
+    __syncthreads();
+    for (int c = 0; c < w; ++c)
+        if (lane < w) Ui[(int64_t)c * NB + lane] = X[c][lane];
+}
+
+// ---- super-panel triangular solves of the block solve ----
+//
+// One launch covers one super-panel (<= WT_NB2 pivots) of every listed front
+// (entries {front, K0, K1}); blocks split the right-hand sides into groups
+// of SP_CG columns, which are independent, so a block walks the panel's
+// 64-pivot tiles sequentially with no inter-block synchronization.
+constexpr int SP_CG = 8;
+
+// forward: y_t = L_t^{-1} P_t y_t, then y[t+w..K1) -= L[t+w..K1, t] y_t
+template <typename R>
+__global__ __launch_bounds__(256) void sp_fwd_kernel(const wt_dfront* __restrict__ fr, const int32_t* __restrict__ ent,
+                                                     const cpx<R>* __restrict__ factor,
+                                                     const cpx<R>* __restrict__ tiles,
+                                                     const int32_t* __restrict__ perm, cpx<R>* __restrict__ y,
+                                                     int64_t n, int nrhs)
+{
+    constexpr int NB = WT_NB;
+    __shared__ cpx<R> yt[NB][SP_CG];
+    __shared__ cpx<R> xt[NB][SP_CG];
+    const int32_t* e = ent + 3 * blockIdx.x;
+    const wt_dfront F = fr[e[0]];
+    const int64_t K0 = e[1], K1 = e[2];
+    const int c0 = blockIdx.y * SP_CG;
+    if (c0 >= nrhs) return;
+    const int nc = nrhs - c0 < SP_CG ? nrhs - c0 : SP_CG;
+    const int64_t ld = F.p + F.u;
+    const cpx<R>* L = factor + F.Loff;
+    cpx<R>* yb = y + F.pstart + (int64_t)c0 * n;
+    const int tid = threadIdx.x;
+    for (int64_t t = K0; t < K1; t += NB) {
+        const int w = (int)(K1 - t < NB ? K1 - t : NB);
+        const int32_t* pr = perm + F.pivoff + t;
+        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
+            const int i = idx % w, c = idx / w;
+            yt[i][c] = c < nc ? yb[(int64_t)c * n + t + pr[i]] : cpx<R>(R(0), R(0));
+        }
+        __syncthreads();
+        const cpx<R>* Li = tiles + F.tileoff + (t / NB) * 2 * NB * NB;
+        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
+            const int i = idx % w, c = idx / w;
+            cpx<R> acc(R(0), R(0));
+            for (int k = 0; k <= i; ++k) acc = acc + Li[(int64_t)k * NB + i] * yt[k][c];
+            xt[i][c] = acc;
+            if (c < nc) yb[(int64_t)c * n + t + i] = acc;
+        }
+        __syncthreads();
+        const int rows = (int)(K1 - t - w);
+        for (int idx = tid; idx < rows * SP_CG; idx += blockDim.x) {
+            const int r = idx % rows, c = idx / rows;
+            if (c >= nc) continue;
+            const int64_t row = t + w + r;
+            cpx<R> acc = yb[(int64_t)c * n + row];
+            for (int k = 0; k < w; ++k) acc = acc - L[(t + k) * ld + row] * xt[k][c];
+            yb[(int64_t)c * n + row] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// backward: x_t = U_t^{-1} y_t, then y[K0..t) -= U[K0..t, t] x_t, tiles descending
+template <typename R>
+__global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict__ fr, const int32_t* __restrict__ ent,
+                                                     const cpx<R>* __restrict__ factor,
+                                                     const cpx<R>* __restrict__ tiles, cpx<R>* __restrict__ y,
+                                                     int64_t n, int nrhs)
+{
+    constexpr int NB = WT_NB;
+    __shared__ cpx<R> yt[NB][SP_CG];
+    __shared__ cpx<R> xt[NB][SP_CG];
+    const int32_t* e = ent + 3 * blockIdx.x;
+    const wt_dfront F = fr[e[0]];
+    const int64_t K0 = e[1], K1 = e[2];
+    const int c0 = blockIdx.y * SP_CG;
+    if (c0 >= nrhs) return;
+    const int nc = nrhs - c0 < SP_CG ? nrhs - c0 : SP_CG;
+    const int64_t ld = F.p + F.u;
+    const cpx<R>* L = factor + F.Loff;
+    cpx<R>* yb = y + F.pstart + (int64_t)c0 * n;
+    const int tid = threadIdx.x;
+    for (int64_t t = K0 + ((K1 - 1 - K0) / NB) * NB; t >= K0; t -= NB) {
+        const int w = (int)(K1 - t < NB ? K1 - t : NB);
+        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
+            const int i = idx % w, c = idx / w;
+            yt[i][c] = c < nc ? yb[(int64_t)c * n + t + i] : cpx<R>(R(0), R(0));
+        }
+        __syncthreads();
+        const cpx<R>* Ui = tiles + F.tileoff + (t / NB) * 2 * NB * NB + NB * NB;
+        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
+            const int i = idx % w, c = idx / w;
+            cpx<R> acc(R(0), R(0));
+            for (int k = i; k < w; ++k) acc = acc + Ui[(int64_t)k * NB + i] * yt[k][c];
+            xt[i][c] = acc;
+            if (c < nc) yb[(int64_t)c * n + t + i] = acc;
+        }
+        __syncthreads();
+        const int rows = (int)(t - K0);
+        for (int idx = tid; idx < rows * SP_CG; idx += blockDim.x) {
+            const int r = idx % rows, c = idx / rows;
+            if (c >= nc) continue;
+            const int64_t row = K0 + r;
+            cpx<R> acc = yb[(int64_t)c * n + row];
+            for (int k = 0; k < w; ++k) acc = acc - L[(t + k) * ld + row] * xt[k][c];
+            yb[(int64_t)c * n + row] = acc;
+        }
+        __syncthreads();
+    }
+}
+
 // ---- solve-side gathers / scatters (block of nrhs columns) ----
 
 // y (elimination order, ld = n) += children's update rows that land on the
@@ -335,8 +538,8 @@ int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t k
                   int32_t* perm, int* status, hipStream_t s)
 {
     if (nlist <= 0) return WT_OK;
-    hipLaunchKernelGGL((diag_tile_kernel<R, WT_NB>), dim3(nlist), dim3(256), 0, s, fr, list, kb, factor, tiles,
-                       perm, status);
+    hipLaunchKernelGGL((diag_tile_wave_kernel<R, WT_NB>), dim3(nlist), dim3(64), 0, s, fr, list, kb, factor,
+                       tiles, perm, status);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
@@ -367,6 +570,24 @@ template int wt_scatter_rows<float>(const int32_t*, const cpx<double>*, const in
                                     cpx<float>*, hipStream_t);
 template int wt_scatter_rows<double>(const int32_t*, const cpx<double>*, const int64_t*, const double*, int64_t,
                                      cpx<double>*, hipStream_t);
+
+template <typename R>
+int wt_sp_solve(int fwd, const wt_dfront* fr, const int32_t* ent, int nent, const cpx<R>* factor,
+                const cpx<R>* tiles, const int32_t* perm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s)
+{
+    if (nent <= 0) return WT_OK;
+    dim3 grid(nent, (nrhs + SP_CG - 1) / SP_CG);
+    if (fwd)
+        hipLaunchKernelGGL(sp_fwd_kernel<R>, grid, dim3(256), 0, s, fr, ent, factor, tiles, perm, y, n, nrhs);
+    else
+        hipLaunchKernelGGL(sp_bwd_kernel<R>, grid, dim3(256), 0, s, fr, ent, factor, tiles, y, n, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+template int wt_sp_solve<float>(int, const wt_dfront*, const int32_t*, int, const cpx<float>*, const cpx<float>*,
+                                const int32_t*, cpx<float>*, int64_t, int, hipStream_t);
+template int wt_sp_solve<double>(int, const wt_dfront*, const int32_t*, int, const cpx<double>*,
+                                 const cpx<double>*, const int32_t*, cpx<double>*, int64_t, int, hipStream_t);
 
 #define WT_INST(R)                                                                                              \
     template int wt_fill_zero<R>(cpx<R>*, int64_t, hipStream_t);                                                \

@@ -26,23 +26,30 @@ struct wt_gemm_prob {
     int64_t pad;
 };
 
-// Tile sizes of the kernel for precision R (rows x cols of C per block).
-template <typename R>
+// Tile shapes of the kernel for precision R: variant 0 is the square-ish
+// default, variant 1 serves problems with N <= 64 (the block solves with
+// 64 right-hand sides) without wasting half of each tile.
+template <typename R, int V>
 struct wt_gemm_tile;
 template <>
-struct wt_gemm_tile<float> { static constexpr int BM = 128, BN = 128; };
+struct wt_gemm_tile<float, 0> { static constexpr int BM = 128, BN = 128, WGM = 2, WGN = 2; };
 template <>
-struct wt_gemm_tile<double> { static constexpr int BM = 128, BN = 64; };
+struct wt_gemm_tile<float, 1> { static constexpr int BM = 256, BN = 64, WGM = 4, WGN = 1; };
+template <>
+struct wt_gemm_tile<double, 0> { static constexpr int BM = 128, BN = 64, WGM = 2, WGN = 2; };
+template <>
+struct wt_gemm_tile<double, 1> { static constexpr int BM = 128, BN = 64, WGM = 2, WGN = 2; };
 
 template <typename R>
-inline int64_t wt_gemm_tiles(int64_t M, int64_t N)
+inline int64_t wt_gemm_tiles(int64_t M, int64_t N, int var = 0)
 {
-    return ((M + wt_gemm_tile<R>::BM - 1) / wt_gemm_tile<R>::BM) *
-           ((N + wt_gemm_tile<R>::BN - 1) / wt_gemm_tile<R>::BN);
+    const int BM = var ? wt_gemm_tile<R, 1>::BM : wt_gemm_tile<R, 0>::BM;
+    const int BN = var ? wt_gemm_tile<R, 1>::BN : wt_gemm_tile<R, 0>::BN;
+    return ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
 }
 
 // Launches nprob problems (device arrays probs[nprob], tile_start[nprob+1],
-// tile_start[0] = 0) as one grid of tile_start[nprob] blocks.
+// tile_start[0] = 0) as one grid of tile_start[nprob] blocks of variant var.
 template <typename R>
 int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, int nprob,
-                   int64_t ntiles, hipStream_t stream);
+                   int64_t ntiles, int var, hipStream_t stream);

@@ -55,14 +55,15 @@ struct MF<double> {
 constexpr int NT = 256;  // threads per block
 constexpr int BK = 8;    // complex k per slice
 
-template <typename R>
+template <typename R, int V>
 __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kernel(const wt_gemm_prob<R>* __restrict__ probs,
                                                             const int64_t* __restrict__ tile_start,
                                                             int nprob, int64_t ntiles)
 {
     typedef MF<R> M_;
-    constexpr int BM = wt_gemm_tile<R>::BM, BN = wt_gemm_tile<R>::BN;
-    constexpr int WM = BM / 2, WN = BN / 2;
+    typedef wt_gemm_tile<R, V> TL;
+    constexpr int BM = TL::BM, BN = TL::BN;
+    constexpr int WM = BM / TL::WGM, WN = BN / TL::WGN;
     constexpr int SM = WM / M_::MT, SN = WN / M_::MT;
     constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
     constexpr int TPA = BM / EA, TPB = BK / EB;  // threads per column of A / B
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
     const int m0 = tm * BM, n0 = tn * BN;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+    const int wm0 = (wave % TL::WGM) * WM, wn0 = (wave / TL::WGM) * WN;
 
     typename M_::acc_t acc[3][SM][SN];
 #pragma unroll
@@ -220,15 +221,19 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
 }  // namespace
 
 template <typename R>
-int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, int nprob, int64_t ntiles,
+int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, int nprob, int64_t ntiles, int var,
                    hipStream_t stream)
 {
     if (nprob <= 0 || ntiles <= 0) return WT_OK;
-    hipLaunchKernelGGL(gemm_batched_kernel<R>, dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
-                       d_tile_start, nprob, ntiles);
+    if (var)
+        hipLaunchKernelGGL((gemm_batched_kernel<R, 1>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                           d_tile_start, nprob, ntiles);
+    else
+        hipLaunchKernelGGL((gemm_batched_kernel<R, 0>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                           d_tile_start, nprob, ntiles);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
 
-template int wt_gemm_launch<float>(const wt_gemm_prob<float>*, const int64_t*, int, int64_t, hipStream_t);
-template int wt_gemm_launch<double>(const wt_gemm_prob<double>*, const int64_t*, int, int64_t, hipStream_t);
+template int wt_gemm_launch<float>(const wt_gemm_prob<float>*, const int64_t*, int, int64_t, int, hipStream_t);
+template int wt_gemm_launch<double>(const wt_gemm_prob<double>*, const int64_t*, int, int64_t, int, hipStream_t);

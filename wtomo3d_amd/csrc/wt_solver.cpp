@@ -71,6 +71,8 @@ void wt_numeric<R>::release()
     dfree(y);
     dfree(w[0]);
     dfree(w[1]);
+    dfree(d_sp);
+    hsp.clear();
     fplan.release();
     splan.release();
     fsteps.clear();
@@ -354,10 +356,12 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
 {
     const wt_symbolic& S = s.S;
     const int nl = (int)S.levels.size();
-    const int64_t nb = S.nb, n = S.n, NB2 = WT_NB2;
+    const int64_t n = S.n, NB2 = WT_NB2;
     dfree(num.y);
     dfree(num.w[0]);
     dfree(num.w[1]);
+    dfree(num.d_sp);
+    num.hsp.clear();
     num.splan.release();
     num.fwd.clear();
     num.bwd.clear();
@@ -366,7 +370,9 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
         (rc = dalloc(&num.w[1], S.w_len[1] * nrhs)))
         return rc;
     cpx<R>* y = num.y;
-    // forward substitution, bottom-up
+    auto nsp = [&](const wt_front& F) { return (F.p + NB2 - 1) / NB2; };
+    // forward substitution, bottom-up; per super-panel: the panel's tiles in
+    // one launch, then the rows below it and the update rows W by GEMM
     for (int l = nl - 1; l >= 0; --l) {
         const int par = l & 1;
         int64_t wl = 0;
@@ -378,48 +384,31 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
                     num.fwd.push_back({WT_STEP_SGATHER, s.lev_kids_cnt[k][l + 1], 0, s.lev_kids_off[k][l + 1], par, -1});
         const std::vector<int64_t>& ps = s.lev_sorted_p[l];
         const int32_t* lst = s.hlists.data() + s.lev_sorted_off[l];
-        for (int64_t kb = 0; kb < ps[0]; kb += nb) {
-            const int cnt = count_gt(ps, kb);
-            num.splan.begin();
+        for (int64_t K0 = 0; K0 < ps[0]; K0 += NB2) {
+            const int cnt = count_gt(ps, K0);
+            const int64_t off = (int64_t)num.hsp.size();
             for (int i = 0; i < cnt; ++i) {
-                const wt_front& F = S.fronts[lst[i]];
-                const int64_t w = std::min(nb, F.p - kb);
-                const cpx<R>* Li = num.tiles + F.tileoff + (kb / nb) * 2 * nb * nb;
-                cpx<R>* yt = y + F.pstart + kb;
-                num.splan.add(Li, nb, yt, n, yt, n, w, nrhs, w, WT_GEMM_SET, num.perm + F.pivoff + kb);
+                const int64_t K1 = std::min(K0 + NB2, S.fronts[lst[i]].p);
+                num.hsp.push_back(lst[i]);
+                num.hsp.push_back((int32_t)K0);
+                num.hsp.push_back((int32_t)K1);
             }
-            int g = num.splan.end();
-            if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
-            // rows of the same super-panel below the tile
+            num.fwd.push_back({WT_STEP_SPFWD, cnt, 0, off, par, -1});
             num.splan.begin();
             for (int i = 0; i < cnt; ++i) {
                 const wt_front& F = S.fronts[lst[i]];
-                const int64_t p = F.p, ld = p + F.u, w = std::min(nb, p - kb);
-                const int64_t K1 = std::min((kb / NB2 + 1) * NB2, p);
-                const cpx<R>* L = num.factor + F.Loff;
-                cpx<R>* yt = y + F.pstart + kb;
-                num.splan.add(L + kb * ld + kb + w, ld, yt, n, yt + w, n, K1 - kb - w, nrhs, w, WT_GEMM_SUB);
-            }
-            g = num.splan.end();
-            if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
-            // super-panel done: remaining pivot rows and the update rows W
-            num.splan.begin();
-            for (int i = 0; i < cnt; ++i) {
-                const wt_front& F = S.fronts[lst[i]];
-                const int64_t p = F.p, u = F.u, ld = p + u, w = std::min(nb, p - kb);
-                const int64_t K0 = (kb / NB2) * NB2, K1 = std::min(K0 + NB2, p);
-                if (kb + w != K1) continue;
+                const int64_t p = F.p, u = F.u, ld = p + u, K1 = std::min(K0 + NB2, p);
                 const cpx<R>* L = num.factor + F.Loff;
                 cpx<R>* ys = y + F.pstart + K0;
                 num.splan.add(L + K0 * ld + K1, ld, ys, n, y + F.pstart + K1, n, p - K1, nrhs, K1 - K0, WT_GEMM_SUB);
                 num.splan.add(L + K0 * ld + p, ld, ys, n, num.w[par] + F.Woff * nrhs, u, u, nrhs, K1 - K0,
                               WT_GEMM_SUB);
             }
-            g = num.splan.end();
+            int g = num.splan.end();
             if (g >= 0) num.fwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
         }
     }
-    // back substitution, top-down
+    // back substitution, top-down; super-panels from the last one back
     for (int l = 0; l < nl; ++l) {
         const int par = l & 1;
         if (s.lev_upd_cnt[l] > 0) {
@@ -436,49 +425,36 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
         }
         const std::vector<int64_t>& ps = s.lev_sorted_p[l];
         const int32_t* lst = s.hlists.data() + s.lev_sorted_off[l];
-        const int64_t nsteps = (ps[0] + nb - 1) / nb;
+        const int64_t nsteps = (ps[0] + NB2 - 1) / NB2;
         for (int64_t st = 0; st < nsteps; ++st) {
+            const int64_t off = (int64_t)num.hsp.size();
+            int cnt = 0;
+            for (int i = 0; i < (int)ps.size(); ++i) {
+                const wt_front& F = S.fronts[lst[i]];
+                const int64_t sp = nsp(F) - 1 - st;
+                if (sp < 0) continue;
+                num.hsp.push_back(lst[i]);
+                num.hsp.push_back((int32_t)(sp * NB2));
+                num.hsp.push_back((int32_t)std::min((sp + 1) * NB2, F.p));
+                ++cnt;
+            }
+            if (cnt > 0) num.bwd.push_back({WT_STEP_SPBWD, cnt, 0, off, par, -1});
             num.splan.begin();
             for (int i = 0; i < (int)ps.size(); ++i) {
                 const wt_front& F = S.fronts[lst[i]];
-                const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
-                if (kb < 0) continue;
-                const int64_t w = std::min(nb, F.p - kb);
-                const cpx<R>* Ui = num.tiles + F.tileoff + (kb / nb) * 2 * nb * nb + nb * nb;
-                cpx<R>* yt = y + F.pstart + kb;
-                num.splan.add(Ui, nb, yt, n, yt, n, w, nrhs, w, WT_GEMM_SET);
+                const int64_t sp = nsp(F) - 1 - st;
+                if (sp <= 0) continue;
+                const int64_t p = F.p, ld = p + F.u, K0 = sp * NB2, K1 = std::min(K0 + NB2, p);
+                const cpx<R>* L = num.factor + F.Loff;
+                num.splan.add(L + K0 * ld, ld, y + F.pstart + K0, n, y + F.pstart, n, K0, nrhs, K1 - K0, WT_GEMM_SUB);
             }
             int g = num.splan.end();
             if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
-            // rows of the same super-panel above the tile
-            num.splan.begin();
-            for (int i = 0; i < (int)ps.size(); ++i) {
-                const wt_front& F = S.fronts[lst[i]];
-                const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
-                if (kb < 0) continue;
-                const int64_t p = F.p, ld = p + F.u, w = std::min(nb, p - kb), K0 = (kb / NB2) * NB2;
-                const cpx<R>* L = num.factor + F.Loff;
-                num.splan.add(L + kb * ld + K0, ld, y + F.pstart + kb, n, y + F.pstart + K0, n, kb - K0, nrhs, w,
-                              WT_GEMM_SUB);
-            }
-            g = num.splan.end();
-            if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
-            // super-panel done: all pivot rows above it
-            num.splan.begin();
-            for (int i = 0; i < (int)ps.size(); ++i) {
-                const wt_front& F = S.fronts[lst[i]];
-                const int64_t last = ((F.p - 1) / nb) * nb, kb = last - st * nb;
-                const int64_t K0 = kb >= 0 ? (kb / NB2) * NB2 : -1;
-                if (kb < 0 || kb != K0 || K0 == 0) continue;
-                const int64_t p = F.p, ld = p + F.u, K1 = std::min(K0 + NB2, p);
-                const cpx<R>* L = num.factor + F.Loff;
-                num.splan.add(L + K0 * ld, ld, y + F.pstart + K0, n, y + F.pstart, n, K0, nrhs, K1 - K0,
-                              WT_GEMM_SUB);
-            }
-            g = num.splan.end();
-            if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
         }
     }
+    if ((rc = dalloc(&num.d_sp, (int64_t)num.hsp.size()))) return rc;
+    if (!num.hsp.empty())
+        WT_HIP(hipMemcpy(num.d_sp, num.hsp.data(), num.hsp.size() * 4, hipMemcpyHostToDevice));
     if ((rc = num.splan.upload())) return rc;
     num.snrhs = nrhs;
     return WT_OK;
@@ -524,6 +500,13 @@ int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R
             case WT_STEP_GEMM:
                 rc = P.run(st, WT_PROF_GEMM_SOLVE, num.splan.launches[t.gemm].flops, 0,
                            [&] { return num.splan.run(t.gemm, st); });
+                break;
+            case WT_STEP_SPFWD:
+            case WT_STEP_SPBWD:
+                rc = P.run(st, WT_PROF_TRSM, 0, 0, [&] {
+                    return wt_sp_solve<R>(t.kind == WT_STEP_SPFWD, s.fronts, num.d_sp + t.list_off, t.count, num.factor,
+                                          num.tiles, num.perm, num.y, s.S.n, nrhs, st);
+                });
                 break;
             default: rc = WT_ERR_STATE;
             }

@@ -13,6 +13,7 @@
 struct wt_gemm_launch_rec {
     int64_t prob_off, tile_off, ntiles;
     int32_t nprob;
+    int32_t var;   // tile variant (1 when every problem has N <= 64)
     double flops;  // 8 M N K summed over the launch's problems
 };
 
@@ -41,7 +42,7 @@ struct wt_gemm_plan {
         p.A = A; p.B = B; p.C = C; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
         p.M = (int32_t)M; p.N = (int32_t)N; p.K = (int32_t)K; p.mode = mode; p.bperm = bperm; p.pad = 0;
         probs.push_back(p);
-        cur.ntiles += wt_gemm_tiles<R>(M, N);
+        cur.ntiles += wt_gemm_tiles<R>(M, N, 0);
         cur.flops += 8.0 * (double)M * (double)N * (double)K;
         tiles.push_back(cur.ntiles);
         ++cur.nprob;
@@ -53,6 +54,17 @@ struct wt_gemm_plan {
             tiles.pop_back();
             return -1;
         }
+        cur.var = 1;
+        for (int64_t i = cur.prob_off; i < (int64_t)probs.size(); ++i)
+            if (probs[i].N > 64) cur.var = 0;
+        if (cur.var) {  // recount tiles for the narrow variant
+            int64_t t = 0;
+            for (int64_t i = cur.prob_off, j = 1; i < (int64_t)probs.size(); ++i, ++j) {
+                t += wt_gemm_tiles<R>(probs[i].M, probs[i].N, 1);
+                tiles[cur.tile_off + j] = t;
+            }
+            cur.ntiles = t;
+        }
         launches.push_back(cur);
         return (int)launches.size() - 1;
     }
@@ -61,11 +73,14 @@ struct wt_gemm_plan {
     int run(int i, hipStream_t s) const
     {
         const wt_gemm_launch_rec& L = launches[i];
-        return wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.tile_off, L.nprob, L.ntiles, s);
+        return wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.tile_off, L.nprob, L.ntiles, L.var, s);
     }
 };
 
-enum wt_step_kind { WT_STEP_ZERO, WT_STEP_EXTADD, WT_STEP_DIAG, WT_STEP_GEMM, WT_STEP_SGATHER, WT_STEP_SGATHER_UPD };
+enum wt_step_kind {
+    WT_STEP_ZERO, WT_STEP_EXTADD, WT_STEP_DIAG, WT_STEP_GEMM, WT_STEP_SGATHER, WT_STEP_SGATHER_UPD,
+    WT_STEP_SPFWD, WT_STEP_SPBWD
+};
 
 struct wt_step {
     int kind;
@@ -92,6 +107,8 @@ struct wt_numeric {
     cpx<R>* w[2] = {nullptr, nullptr};
     wt_gemm_plan<R> splan;
     std::vector<wt_step> fwd, bwd;
+    std::vector<int32_t> hsp;   // super-panel entries {front, K0, K1}
+    int32_t* d_sp = nullptr;
     void release();
 };
 
@@ -99,7 +116,7 @@ struct wt_numeric {
 // wt_solver_profile): launches, summed kernel time and algorithmic work.
 enum wt_prof_class {
     WT_PROF_GEMM_FACTOR, WT_PROF_GEMM_SOLVE, WT_PROF_DIAG, WT_PROF_EXTADD, WT_PROF_FILL,
-    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_NCLASS
+    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_TRSM, WT_PROF_NCLASS
 };
 struct wt_prof {
     bool on = false;
