@@ -305,15 +305,18 @@ __global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __r
     for (int c = 0; c < w; ++c)
         if (lane < w) Li[(int64_t)c * NB + lane] = X[c][lane];
     __syncthreads();
-    // upper inverse, lane = column
+    // upper inverse, lane = column; reciprocals of the diagonal first
+    __shared__ cpx<R> rd[NB];
+    if (lane < w) rd[lane] = cdiv(cpx<R>(R(1), R(0)), T[lane][lane]);
+    __syncthreads();
     if (lane < w) {
         const int c = lane;
         for (int i = c + 1; i < w; ++i) X[c][i] = cpx<R>(R(0), R(0));
-        X[c][c] = cdiv(cpx<R>(R(1), R(0)), T[c][c]);
+        X[c][c] = rd[c];
         for (int i = c - 1; i >= 0; --i) {
             cpx<R> s(R(0), R(0));
             for (int k = i + 1; k <= c; ++k) s = s + T[k][i] * X[c][k];
-            X[c][i] = -cdiv(s, T[i][i]);
+            X[c][i] = -(s * rd[i]);
         }
     }
     __syncthreads();
@@ -326,8 +329,10 @@ __global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __r
 // One launch covers one super-panel (<= WT_NB2 pivots) of every listed front
 // (entries {front, K0, K1}); blocks split the right-hand sides into groups
 // of SP_CG columns, which are independent, so a block walks the panel's
-// 64-pivot tiles sequentially with no inter-block synchronization.
-constexpr int SP_CG = 8;
+// 64-pivot tiles sequentially with no inter-block synchronization.  Each
+// thread owns matrix rows and updates all SP_CG columns of a row, so every
+// factor entry it loads feeds SP_CG complex multiply-adds.
+constexpr int SP_CG = 16;
 
 // forward: y_t = L_t^{-1} P_t y_t, then y[t+w..K1) -= L[t+w..K1, t] y_t
 template <typename R>
@@ -359,22 +364,38 @@ __global__ __launch_bounds__(256) void sp_fwd_kernel(const wt_dfront* __restrict
         }
         __syncthreads();
         const cpx<R>* Li = tiles + F.tileoff + (t / NB) * 2 * NB * NB;
-        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
-            const int i = idx % w, c = idx / w;
-            cpx<R> acc(R(0), R(0));
-            for (int k = 0; k <= i; ++k) acc = acc + Li[(int64_t)k * NB + i] * yt[k][c];
-            xt[i][c] = acc;
-            if (c < nc) yb[(int64_t)c * n + t + i] = acc;
+        if (tid < w) {
+            const int i = tid;
+            cpx<R> acc[SP_CG];
+#pragma unroll
+            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
+            for (int k = 0; k <= i; ++k) {
+                const cpx<R> a = Li[(int64_t)k * NB + i];
+#pragma unroll
+                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * yt[k][c];
+            }
+#pragma unroll
+            for (int c = 0; c < SP_CG; ++c) {
+                xt[i][c] = acc[c];
+                if (c < nc) yb[(int64_t)c * n + t + i] = acc[c];
+            }
         }
         __syncthreads();
         const int rows = (int)(K1 - t - w);
-        for (int idx = tid; idx < rows * SP_CG; idx += blockDim.x) {
-            const int r = idx % rows, c = idx / rows;
-            if (c >= nc) continue;
+        for (int r = tid; r < rows; r += blockDim.x) {
             const int64_t row = t + w + r;
-            cpx<R> acc = yb[(int64_t)c * n + row];
-            for (int k = 0; k < w; ++k) acc = acc - L[(t + k) * ld + row] * xt[k][c];
-            yb[(int64_t)c * n + row] = acc;
+            cpx<R> acc[SP_CG];
+#pragma unroll
+            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
+            for (int k = 0; k < w; ++k) {
+                const cpx<R> a = L[(t + k) * ld + row];
+#pragma unroll
+                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * xt[k][c];
+            }
+            for (int c = 0; c < nc; ++c) {
+                cpx<R>* q = yb + (int64_t)c * n + row;
+                *q = *q - acc[c];
+            }
         }
         __syncthreads();
     }
@@ -408,24 +429,41 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
         }
         __syncthreads();
         const cpx<R>* Ui = tiles + F.tileoff + (t / NB) * 2 * NB * NB + NB * NB;
-        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
-            const int i = idx % w, c = idx / w;
-            cpx<R> acc(R(0), R(0));
-            for (int k = i; k < w; ++k) acc = acc + Ui[(int64_t)k * NB + i] * yt[k][c];
-            xt[i][c] = acc;
-            if (c < nc) yb[(int64_t)c * n + t + i] = acc;
+        if (tid < w) {
+            const int i = tid;
+            cpx<R> acc[SP_CG];
+#pragma unroll
+            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
+            for (int k = i; k < w; ++k) {
+                const cpx<R> a = Ui[(int64_t)k * NB + i];
+#pragma unroll
+                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * yt[k][c];
+            }
+#pragma unroll
+            for (int c = 0; c < SP_CG; ++c) {
+                xt[i][c] = acc[c];
+                if (c < nc) yb[(int64_t)c * n + t + i] = acc[c];
+            }
         }
         __syncthreads();
         const int rows = (int)(t - K0);
-        for (int idx = tid; idx < rows * SP_CG; idx += blockDim.x) {
-            const int r = idx % rows, c = idx / rows;
-            if (c >= nc) continue;
+        for (int r = tid; r < rows; r += blockDim.x) {
             const int64_t row = K0 + r;
-            cpx<R> acc = yb[(int64_t)c * n + row];
-            for (int k = 0; k < w; ++k) acc = acc - L[(t + k) * ld + row] * xt[k][c];
-            yb[(int64_t)c * n + row] = acc;
+            cpx<R> acc[SP_CG];
+#pragma unroll
+            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
+            for (int k = 0; k < w; ++k) {
+                const cpx<R> a = L[(t + k) * ld + row];
+#pragma unroll
+                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * xt[k][c];
+            }
+            for (int c = 0; c < nc; ++c) {
+                cpx<R>* q = yb + (int64_t)c * n + row;
+                *q = *q - acc[c];
+            }
         }
         __syncthreads();
+        if (t == K0) break;
     }
 }
 

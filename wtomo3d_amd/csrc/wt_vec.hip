@@ -83,7 +83,14 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
 {
     __shared__ double s_part[SP_ROWS][64];
     const int wave = threadIdx.x >> 6, c = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * SP_ROWS + wave;
+    // each of the 8 XCDs gets a contiguous range of rows, so the x rows of
+    // 27-point neighbours are shared through its L2
+    int64_t blk = blockIdx.x;
+    {
+        const int64_t nb = gridDim.x, q = nb / 8, r = nb % 8, x = blk % 8, i = blk / 8;
+        blk = x * q + (x < r ? x : r) + i;
+    }
+    const int64_t row = blk * SP_ROWS + wave;
     double nrm = 0.0;
     if (row < n && c < nrhs) {
         // b == null: r = A x instead of b - A x
@@ -104,7 +111,7 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
     if (wave == 0 && c < nrhs) {
         double s = 0.0;
         for (int w = 0; w < SP_ROWS; ++w) s += s_part[w][c];
-        partial[(int64_t)blockIdx.x * nrhs + c] = s;
+        partial[blk * nrhs + c] = s;
     }
 }
 
