@@ -281,7 +281,16 @@ __global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __r
         if (lane > j && lane < w) {
             const cpx<R> l = T[j][lane] * dinv;
             T[j][lane] = l;
-            for (int c = j + 1; c < w; ++c) T[c][lane] = T[c][lane] - l * T[c][j];
+            // rank-1 update of this row, 8 columns per batch of LDS loads
+            for (int c0 = j + 1; c0 < w; c0 += 8) {
+                cpx<R> pr[8], q[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (c0 + k < w) { pr[k] = T[c0 + k][j]; q[k] = T[c0 + k][lane]; }
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (c0 + k < w) T[c0 + k][lane] = q[k] - l * pr[k];
+            }
         }
         __syncthreads();
     }
@@ -296,9 +305,19 @@ __global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __r
         for (int i = 0; i < c; ++i) X[c][i] = cpx<R>(R(0), R(0));
         X[c][c] = cpx<R>(R(1), R(0));
         for (int i = c + 1; i < w; ++i) {
-            cpx<R> s(R(0), R(0));
-            for (int k = c; k < i; ++k) s = s + T[k][i] * X[c][k];
-            X[c][i] = -s;
+            cpx<R> s0(R(0), R(0)), s1(R(0), R(0));
+            for (int k0 = c; k0 < i; k0 += 8) {
+                cpx<R> a[8], x[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k0 + k < i) { a[k] = T[k0 + k][i]; x[k] = X[c][k0 + k]; }
+#pragma unroll
+                for (int k = 0; k < 8; k += 2) {
+                    if (k0 + k < i) s0 = s0 + a[k] * x[k];
+                    if (k0 + k + 1 < i) s1 = s1 + a[k + 1] * x[k + 1];
+                }
+            }
+            X[c][i] = -(s0 + s1);
         }
     }
     __syncthreads();
@@ -314,9 +333,19 @@ __global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __r
         for (int i = c + 1; i < w; ++i) X[c][i] = cpx<R>(R(0), R(0));
         X[c][c] = rd[c];
         for (int i = c - 1; i >= 0; --i) {
-            cpx<R> s(R(0), R(0));
-            for (int k = i + 1; k <= c; ++k) s = s + T[k][i] * X[c][k];
-            X[c][i] = -(s * rd[i]);
+            cpx<R> s0(R(0), R(0)), s1(R(0), R(0));
+            for (int k0 = i + 1; k0 <= c; k0 += 8) {
+                cpx<R> a[8], x[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k0 + k <= c) { a[k] = T[k0 + k][i]; x[k] = X[c][k0 + k]; }
+#pragma unroll
+                for (int k = 0; k < 8; k += 2) {
+                    if (k0 + k <= c) s0 = s0 + a[k] * x[k];
+                    if (k0 + k + 1 <= c) s1 = s1 + a[k + 1] * x[k + 1];
+                }
+            }
+            X[c][i] = -((s0 + s1) * rd[i]);
         }
     }
     __syncthreads();
@@ -333,6 +362,7 @@ __global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __r
 // thread owns matrix rows and updates all SP_CG columns of a row, so every
 // factor entry it loads feeds SP_CG complex multiply-adds.
 constexpr int SP_CG = 16;
+constexpr int SP_CH = 16;  // factor entries loaded per batch (independent loads in flight)
 
 // forward: y_t = L_t^{-1} P_t y_t, then y[t+w..K1) -= L[t+w..K1, t] y_t
 template <typename R>
@@ -358,9 +388,10 @@ __global__ __launch_bounds__(256) void sp_fwd_kernel(const wt_dfront* __restrict
     for (int64_t t = K0; t < K1; t += NB) {
         const int w = (int)(K1 - t < NB ? K1 - t : NB);
         const int32_t* pr = perm + F.pivoff + t;
-        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
-            const int i = idx % w, c = idx / w;
-            yt[i][c] = c < nc ? yb[(int64_t)c * n + t + pr[i]] : cpx<R>(R(0), R(0));
+        for (int idx = tid; idx < NB * SP_CG; idx += blockDim.x) {
+            const int i = idx % NB, c = idx / NB;
+            yt[i][c] = (c < nc && i < w) ? yb[(int64_t)c * n + t + pr[i]] : cpx<R>(R(0), R(0));
+            xt[i][c] = cpx<R>(R(0), R(0));
         }
         __syncthreads();
         const cpx<R>* Li = tiles + F.tileoff + (t / NB) * 2 * NB * NB;
@@ -369,10 +400,17 @@ __global__ __launch_bounds__(256) void sp_fwd_kernel(const wt_dfront* __restrict
             cpx<R> acc[SP_CG];
 #pragma unroll
             for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k = 0; k <= i; ++k) {
-                const cpx<R> a = Li[(int64_t)k * NB + i];
+            // the inverse holds explicit zeros above its diagonal inside the
+            // tile; entries past the tile width are never read
+            for (int k0 = 0; k0 < w; k0 += SP_CH) {
+                cpx<R> a[SP_CH];
 #pragma unroll
-                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * yt[k][c];
+                for (int j = 0; j < SP_CH; ++j)
+                    a[j] = k0 + j < w ? Li[(int64_t)(k0 + j) * NB + i] : cpx<R>(R(0), R(0));
+#pragma unroll
+                for (int j = 0; j < SP_CH; ++j)
+#pragma unroll
+                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * yt[k0 + j][c];
             }
 #pragma unroll
             for (int c = 0; c < SP_CG; ++c) {
@@ -387,10 +425,15 @@ __global__ __launch_bounds__(256) void sp_fwd_kernel(const wt_dfront* __restrict
             cpx<R> acc[SP_CG];
 #pragma unroll
             for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k = 0; k < w; ++k) {
-                const cpx<R> a = L[(t + k) * ld + row];
+            for (int k0 = 0; k0 < w; k0 += SP_CH) {
+                cpx<R> a[SP_CH];
 #pragma unroll
-                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * xt[k][c];
+                for (int j = 0; j < SP_CH; ++j)
+                    a[j] = k0 + j < w ? L[(t + k0 + j) * ld + row] : cpx<R>(R(0), R(0));
+#pragma unroll
+                for (int j = 0; j < SP_CH; ++j)
+#pragma unroll
+                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * xt[k0 + j][c];
             }
             for (int c = 0; c < nc; ++c) {
                 cpx<R>* q = yb + (int64_t)c * n + row;
@@ -423,9 +466,10 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
     const int tid = threadIdx.x;
     for (int64_t t = K0 + ((K1 - 1 - K0) / NB) * NB; t >= K0; t -= NB) {
         const int w = (int)(K1 - t < NB ? K1 - t : NB);
-        for (int idx = tid; idx < w * SP_CG; idx += blockDim.x) {
-            const int i = idx % w, c = idx / w;
-            yt[i][c] = c < nc ? yb[(int64_t)c * n + t + i] : cpx<R>(R(0), R(0));
+        for (int idx = tid; idx < NB * SP_CG; idx += blockDim.x) {
+            const int i = idx % NB, c = idx / NB;
+            yt[i][c] = (c < nc && i < w) ? yb[(int64_t)c * n + t + i] : cpx<R>(R(0), R(0));
+            xt[i][c] = cpx<R>(R(0), R(0));
         }
         __syncthreads();
         const cpx<R>* Ui = tiles + F.tileoff + (t / NB) * 2 * NB * NB + NB * NB;
@@ -434,10 +478,15 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
             cpx<R> acc[SP_CG];
 #pragma unroll
             for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k = i; k < w; ++k) {
-                const cpx<R> a = Ui[(int64_t)k * NB + i];
+            for (int k0 = 0; k0 < w; k0 += SP_CH) {
+                cpx<R> a[SP_CH];
 #pragma unroll
-                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * yt[k][c];
+                for (int j = 0; j < SP_CH; ++j)
+                    a[j] = k0 + j < w ? Ui[(int64_t)(k0 + j) * NB + i] : cpx<R>(R(0), R(0));
+#pragma unroll
+                for (int j = 0; j < SP_CH; ++j)
+#pragma unroll
+                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * yt[k0 + j][c];
             }
 #pragma unroll
             for (int c = 0; c < SP_CG; ++c) {
@@ -452,10 +501,15 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
             cpx<R> acc[SP_CG];
 #pragma unroll
             for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k = 0; k < w; ++k) {
-                const cpx<R> a = L[(t + k) * ld + row];
+            for (int k0 = 0; k0 < w; k0 += SP_CH) {
+                cpx<R> a[SP_CH];
 #pragma unroll
-                for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a * xt[k][c];
+                for (int j = 0; j < SP_CH; ++j)
+                    a[j] = k0 + j < w ? L[(t + k0 + j) * ld + row] : cpx<R>(R(0), R(0));
+#pragma unroll
+                for (int j = 0; j < SP_CH; ++j)
+#pragma unroll
+                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * xt[k0 + j][c];
             }
             for (int c = 0; c < nc; ++c) {
                 cpx<R>* q = yb + (int64_t)c * n + row;
