@@ -106,6 +106,10 @@ int wt_solver_info(const wt_solver *h, int64_t *info, double *flops);
 /* The same statistics from the host symbolic analysis alone (no GPU). */
 int wt_symbolic_info(int nx, int ny, int nz, int leaf_nodes, int64_t *info, double *flops);
 
+/* HBM bytes a solver for this grid will need (host-only symbolic
+ * analysis; used to admit requests before anything is allocated). */
+int64_t wt_estimate_bytes(int nx, int ny, int nz, int leaf_nodes, int factor_prec, int nrhs);
+
 /* HBM bytes the solver needs for a factorization in factor_prec and a
  * block solve of nrhs columns (the admission estimate). */
 int64_t wt_solver_bytes(const wt_solver *h, int factor_prec, int nrhs);
@@ -138,6 +142,11 @@ int wt_solver_solve(wt_solver *h, int factor_prec, int nrhs, const void *b, void
 /* Y = A X with the solver's FP64 CSR values, row-major n x nrhs blocks
  * (host or device).  Used to form right-hand sides and residuals. */
 int wt_solver_apply(wt_solver *h, int nrhs, const void *x, void *y, int on_device);
+
+/* Device block d_x (row-major n x nrhs complex128) of homogeneous plane
+ * waves u_j(r) = d_j exp(-i kmag d_j.r), d_j = dirs[3j..3j+2] (unit,
+ * z positive down), r in metres on the solver's grid with spacing dx. */
+int wt_solver_planewaves(wt_solver *h, int nrhs, const double *dirs, double kmag, double dx, void *d_x);
 
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
  * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and

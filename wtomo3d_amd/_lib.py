@@ -71,6 +71,10 @@ def _declare(L):
     L.wt_gemm_host.argtypes = [i32] * 4 + [_vp, i64, _vp, i64, _vp, i64, i32, _vp, i32,
                                            ctypes.POINTER(ctypes.c_double)]
     L.wt_gemm_host.restype = ctypes.c_int
+    L.wt_solver_planewaves.argtypes = [_vp, i32, _vp, ctypes.c_double, ctypes.c_double, _vp]
+    L.wt_solver_planewaves.restype = ctypes.c_int
+    L.wt_estimate_bytes.argtypes = [i32] * 6
+    L.wt_estimate_bytes.restype = i64
     L.wt_symbolic_info.argtypes = [i32] * 4 + [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.wt_symbolic_info.restype = ctypes.c_int
     L.wt_dev_copy.argtypes = [_vp, _vp, ctypes.c_size_t]

@@ -148,6 +148,22 @@ int wt_symbolic_info(int nx, int ny, int nz, int leaf_nodes, int64_t* info, doub
     return WT_OK;
 }
 
+int64_t wt_estimate_bytes(int nx, int ny, int nz, int leaf_nodes, int factor_prec, int nrhs)
+{
+    wt_symbolic S;
+    if (wt_symbolic_build(nx, ny, nz, leaf_nodes > 0 ? leaf_nodes : 64, WT_NB, nullptr, nullptr, S) != WT_OK)
+        return -1;
+    const int64_t cs = factor_prec == WT_PREC_F32 ? 8 : 16, nnz = wt_nnz(nx, ny, nz);
+    const int64_t rb = nrhs < 64 ? nrhs : 64;
+    int64_t b = (S.n + 1) * 4 + nnz * (4 + 8 + 16) + S.n * 16;          // pattern, map, values, perm, scale
+    b += (int64_t)S.upd_idx.size() * 8 + (int64_t)S.relmap.size() * 4;
+    b += (S.n / 3) * 40;                                                // model
+    b += (S.factor_len + S.cb_len[0] + S.cb_len[1] + S.tile_len) * cs + S.piv_len * 4;
+    b += (S.n + S.w_len[0] + S.w_len[1]) * rb * cs;                    // solve blocks
+    b += 3 * S.n * rb * 16;                                             // refinement blocks
+    return b;
+}
+
 int wt_solver_destroy(wt_solver* h)
 {
     if (!h) return WT_OK;
@@ -290,6 +306,20 @@ int wt_solver_apply(wt_solver* h, int nrhs, const void* x, void* y, int on_devic
         WT_HIP(hipStreamSynchronize(s.stream));
     }
     return WT_OK;
+}
+
+int wt_solver_planewaves(wt_solver* h, int nrhs, const double* dirs, double kmag, double dx, void* d_x)
+{
+    if (!h || !dirs || !d_x || nrhs < 1) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    double* dd = nullptr;
+    if (hipMalloc(&dd, 3 * nrhs * sizeof(double)) != hipSuccess) return WT_ERR_NOMEM;
+    int rc = hipMemcpyAsync(dd, dirs, 3 * nrhs * sizeof(double), hipMemcpyHostToDevice, s.stream) == hipSuccess
+                 ? WT_OK : WT_ERR_HIP;
+    if (!rc) rc = wt_planewave((cpx<double>*)d_x, s.S.nx, s.S.ny, s.S.nz, dx, kmag, dd, nrhs, s.stream);
+    if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
+    (void)hipFree(dd);
+    return rc;
 }
 
 int wt_dev_copy(void* d_dst, const void* d_src, size_t bytes)

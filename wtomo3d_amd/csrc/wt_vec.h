@@ -22,3 +22,5 @@ template <typename R>
 int wt_copy(const cpx<R>* a, cpx<R>* b, int64_t n, hipStream_t s);
 int wt_convert_c128_c64(const cpx<double>* a, cpx<float>* b, int64_t n, hipStream_t s);
 int wt_widen_c64(const cpx<float>* a, cpx<double>* b, int64_t n, hipStream_t s);
+int wt_planewave(cpx<double>* x, int nx, int ny, int nz, double dx, double kmag, const double* d_dirs, int nrhs,
+                 hipStream_t s);

@@ -177,6 +177,29 @@ __global__ void widen_c64_kernel(const cpx<float>* __restrict__ a, cpx<double>* 
     for (; i < n; i += stride) b[i] = cpx<double>((double)a[i].re, (double)a[i].im);
 }
 
+// Homogeneous plane waves u_j(r) = d_j exp(-i k d_j . r), row-major
+// n x nrhs in the reference's unknown order (node z fastest, then x, then
+// y; components u, v, w), r = ((ix-1) dx, (iy-1) dx, (iz-1) dx).
+__global__ void planewave_kernel(cpx<double>* __restrict__ x, int nx, int ny, int nz, double dx, double kmag,
+                                 const double* __restrict__ dirs, int nrhs)
+{
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    const int64_t total = nodes * nrhs;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(i % nrhs);
+        const int64_t g = i / nrhs;
+        const int iz = (int)(g % nz), ix = (int)((g / nz) % nx), iy = (int)(g / ((int64_t)nz * nx));
+        const double d0 = dirs[3 * j], d1 = dirs[3 * j + 1], d2 = dirs[3 * j + 2];
+        const double ph = -kmag * dx * (d0 * ix + d1 * iy + d2 * iz);
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        cpx<double>* row = x + 3 * g * nrhs + j;
+        row[0] = cpx<double>(d0 * cs, d0 * sn);
+        row[nrhs] = cpx<double>(d1 * cs, d1 * sn);
+        row[2 * nrhs] = cpx<double>(d2 * cs, d2 * sn);
+    }
+}
+
 }  // namespace
 
 static unsigned grid1(int64_t n)
@@ -244,6 +267,15 @@ int wt_convert_c128_c64(const cpx<double>* a, cpx<float>* b, int64_t n, hipStrea
 {
     if (n <= 0) return WT_OK;
     hipLaunchKernelGGL(convert_c128_c64_kernel, dim3(grid1(n)), dim3(256), 0, s, a, b, n);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_planewave(cpx<double>* x, int nx, int ny, int nz, double dx, double kmag, const double* d_dirs, int nrhs,
+                 hipStream_t s)
+{
+    const int64_t total = (int64_t)nx * ny * nz * nrhs;
+    hipLaunchKernelGGL(planewave_kernel, dim3(grid1(total)), dim3(256), 0, s, x, nx, ny, nz, dx, kmag, d_dirs, nrhs);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }

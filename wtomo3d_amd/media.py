@@ -71,3 +71,13 @@ def plane_wave_fields(nx, ny, nz, dx, omega, vp, nrhs, seed, inc=(10.0, 35.0)):
     phase = np.exp(-1j * (omega / vp) * (r @ d.T))  # (nodes, nrhs)
     U = phase[:, None, :] * d.T[None, :, :]        # (nodes, 3, nrhs)
     return np.ascontiguousarray(U.reshape(-1, nrhs))
+
+
+def plane_wave_dirs(nrhs, seed, inc=(10.0, 35.0)):
+    """Unit propagation directions (z down, arriving from below) of nrhs
+    plane waves with incidence uniform in `inc` degrees and azimuth uniform
+    in 0-360 degrees: array (nrhs, 3)."""
+    rng = np.random.default_rng(seed)
+    th = np.deg2rad(rng.uniform(inc[0], inc[1], nrhs))
+    ph = np.deg2rad(rng.uniform(0.0, 360.0, nrhs))
+    return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)], axis=1)
