@@ -111,410 +111,301 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __rest
     }
 }
 
-// LU with partial pivoting of the nb x nb diagonal tile starting at pivot kb
+// LU with partial pivoting of the 64 x 64 diagonal tile starting at pivot kb
 // of every listed front, plus the inverses of its unit-lower and upper
 // factors.  perm[i] = tile row that ends at position i.
-template <typename R, int NB>
+//
+// One 256-thread block per tile.  The four waves load and store the tile
+// together (every load of a thread in flight at once); the elimination runs
+// with lane = row, wave 0 choosing the pivot (largest |re|+|im| in the
+// column, first on ties) and swapping whole tile rows, and the columns of
+// each rank-1 update split over the waves.  The inverses are swept row by
+// row with lane = column, so the factor entry of every step is a
+// wave-uniform LDS broadcast: wave 0 builds L^{-1} while wave 1 builds
+// U^{-1} (one after the other in FP64, whose second buffer would not fit in
+// LDS) and the other waves write the factored tile out.
+template <typename R>
 __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restrict__ fr,
                                                         const int32_t* __restrict__ list, int64_t kb,
                                                         cpx<R>* __restrict__ factor, cpx<R>* __restrict__ tiles,
                                                         int32_t* __restrict__ perm, int* __restrict__ status)
 {
-    __shared__ cpx<R> T[NB][NB + 1];  // T[col][row]
-    __shared__ cpx<R> X[NB][NB + 1];
-    __shared__ int s_piv;
-    __shared__ int s_perm[NB];
+    constexpr int NB = WT_NB, CPT = NB / 4;
+    constexpr int NX = sizeof(R) == 4 ? 2 : 1;  // inverse buffers that fit beside T
+    __shared__ cpx<R> T[NB][NB + 1];          // T[col][row]
+    __shared__ cpx<R> X[NX][NB][NB + 1];      // X[.][row][col] of an inverse
     const wt_dfront F = fr[list[blockIdx.x]];
     if (kb >= F.p) return;
     const int w = (int)(F.p - kb < NB ? F.p - kb : NB);
     const int64_t ld = F.p + F.u;
     cpx<R>* base = factor + F.Loff + kb * ld + kb;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < w * w; e += blockDim.x) {
-        int c = e / w, r = e - c * w;
-        T[c][r] = base[(int64_t)c * ld + r];
+    const int tid = threadIdx.x, lane = tid & 63, wq = tid >> 6;
+    {
+        cpx<R> v[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = wq + 4 * k;
+            v[k] = (c < w && lane < w) ? base[(int64_t)c * ld + lane] : cpx<R>(R(0), R(0));
+        }
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) T[wq + 4 * k][lane] = v[k];
     }
-    if (tid < NB) s_perm[tid] = tid;
+    int myperm = lane;  // wave 0: tile row that ends at position `lane`
     __syncthreads();
     for (int j = 0; j < w; ++j) {
-        if (tid < 64) {
-            // pivot: largest |re| + |im| in column j, rows j..w-1 (first on ties)
-            R best = R(-1);
-            int bi = j;
-            for (int i = j + tid; i < w; i += 64) {
-                R a = fabs(T[j][i].re) + fabs(T[j][i].im);
-                if (a > best) { best = a; bi = i; }
-            }
+        if (wq == 0) {
+            R a = R(-1);
+            if (lane >= j && lane < w) a = fabs(T[j][lane].re) + fabs(T[j][lane].im);
+            int bi = lane;
             for (int off = 32; off > 0; off >>= 1) {
-                R ob = __shfl_xor(best, off);
-                int oi = __shfl_xor(bi, off);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+                const R oa = __shfl_xor(a, off);
+                const int oi = __shfl_xor(bi, off);
+                if (oa > a || (oa == a && oi < bi)) { a = oa; bi = oi; }
             }
-            if (tid == 0) {
-                s_piv = bi;
-                int t0 = s_perm[j];
-                s_perm[j] = s_perm[bi];
-                s_perm[bi] = t0;
+            const int pv = bi;  // uniform across the wave
+            if (pv != j) {
+                if (lane < w) {
+                    const cpx<R> t0 = T[lane][j];
+                    T[lane][j] = T[lane][pv];
+                    T[lane][pv] = t0;
+                }
+                const int pj = __shfl(myperm, j), pp = __shfl(myperm, pv);
+                if (lane == j) myperm = pp;
+                if (lane == pv) myperm = pj;
             }
         }
-        __syncthreads();
-        const int pv = s_piv;
-        if (pv != j)
-            for (int c = tid; c < w; c += blockDim.x) {
-                cpx<R> t0 = T[c][j];
-                T[c][j] = T[c][pv];
-                T[c][pv] = t0;
-            }
         __syncthreads();
         cpx<R> d = T[j][j];
         if (d.re == R(0) && d.im == R(0)) {
             if (tid == 0) atomicOr(status, 1);
             d = cpx<R>(R(1e-30), R(0));
-            if (tid == 0) T[j][j] = d;
         }
         const cpx<R> dinv = cdiv(cpx<R>(R(1), R(0)), d);
-        __syncthreads();
-        for (int i = j + 1 + tid; i < w; i += blockDim.x) T[j][i] = T[j][i] * dinv;
-        __syncthreads();
-        const int m = w - j - 1;
-        for (int e = tid; e < m * m; e += blockDim.x) {
-            int c = j + 1 + e / m, i = j + 1 + e % m;
-            T[c][i] = T[c][i] - T[j][i] * T[c][j];
-        }
-        __syncthreads();
-    }
-    // write the factors back and the row permutation
-    for (int e = tid; e < w * w; e += blockDim.x) {
-        int c = e / w, r = e - c * w;
-        base[(int64_t)c * ld + r] = T[c][r];
-    }
-    if (tid < w) perm[F.pivoff + kb + tid] = s_perm[tid];
-    // inverse of the unit-lower factor, one column per thread
-    cpx<R>* Li = tiles + F.tileoff + (kb / NB) * 2 * NB * NB;
-    cpx<R>* Ui = Li + NB * NB;
-    if (tid < w) {
-        const int c = tid;
-        for (int i = 0; i < c; ++i) X[c][i] = cpx<R>(R(0), R(0));
-        X[c][c] = cpx<R>(R(1), R(0));
-        for (int i = c + 1; i < w; ++i) {
-            cpx<R> s(R(0), R(0));
-            for (int k = c; k < i; ++k) s = s + T[k][i] * X[c][k];
-            X[c][i] = -s;
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < w * w; e += blockDim.x) {
-        int c = e / w, r = e - c * w;
-        Li[(int64_t)c * NB + r] = X[c][r];
-    }
-    __syncthreads();
-    // inverse of the upper factor
-    if (tid < w) {
-        const int c = tid;
-        for (int i = c + 1; i < w; ++i) X[c][i] = cpx<R>(R(0), R(0));
-        X[c][c] = cdiv(cpx<R>(R(1), R(0)), T[c][c]);
-        for (int i = c - 1; i >= 0; --i) {
-            cpx<R> s(R(0), R(0));
-            for (int k = i + 1; k <= c; ++k) s = s + T[k][i] * X[c][k];
-            X[c][i] = -cdiv(s, T[i][i]);
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < w * w; e += blockDim.x) {
-        int c = e / w, r = e - c * w;
-        Ui[(int64_t)c * NB + r] = X[c][r];
-    }
-}
-
-// Same contract as diag_tile_kernel, done by a single wave: lane = row
-// during the elimination (pivot search by cross-lane reduction, row swaps
-// by lane = column), lane = column for the triangular inverses.  With one
-// wave per tile the barriers only order LDS accesses.
-template <typename R, int NB>
-__global__ __launch_bounds__(64) void diag_tile_wave_kernel(const wt_dfront* __restrict__ fr,
-                                                           const int32_t* __restrict__ list, int64_t kb,
-                                                           cpx<R>* __restrict__ factor, cpx<R>* __restrict__ tiles,
-                                                           int32_t* __restrict__ perm, int* __restrict__ status)
-{
-    static_assert(NB == 64, "one lane per tile row");
-    __shared__ cpx<R> T[NB][NB + 1];  // T[col][row]
-    __shared__ cpx<R> X[NB][NB + 1];
-    const wt_dfront F = fr[list[blockIdx.x]];
-    if (kb >= F.p) return;
-    const int w = (int)(F.p - kb < NB ? F.p - kb : NB);
-    const int64_t ld = F.p + F.u;
-    cpx<R>* base = factor + F.Loff + kb * ld + kb;
-    const int lane = threadIdx.x;
-    for (int c = 0; c < w; ++c)
-        if (lane < w) T[c][lane] = base[(int64_t)c * ld + lane];
-    int myperm = lane;  // tile row that ends at position `lane`
-    __syncthreads();
-    for (int j = 0; j < w; ++j) {
-        R a = R(-1);
-        if (lane >= j && lane < w) a = fabs(T[j][lane].re) + fabs(T[j][lane].im);
-        int bi = lane;
-        for (int off = 32; off > 0; off >>= 1) {
-            R oa = __shfl_xor(a, off);
-            int oi = __shfl_xor(bi, off);
-            if (oa > a || (oa == a && oi < bi)) { a = oa; bi = oi; }
-        }
-        const int pv = bi;  // uniform across the wave
-        if (pv != j) {
-            if (lane < w) {
-                cpx<R> t0 = T[lane][j];
-                T[lane][j] = T[lane][pv];
-                T[lane][pv] = t0;
-            }
-            const int pj = __shfl(myperm, j), pp = __shfl(myperm, pv);
-            if (lane == j) myperm = pp;
-            if (lane == pv) myperm = pj;
-        }
-        __syncthreads();
-        cpx<R> d = T[j][j];
-        if (d.re == R(0) && d.im == R(0)) {
-            if (lane == 0) atomicOr(status, 1);
-            d = cpx<R>(R(1e-30), R(0));
-        }
-        const cpx<R> dinv = cdiv(cpx<R>(R(1), R(0)), d);
-        __syncthreads();
-        if (lane == j) T[j][j] = d;
+        cpx<R> l(R(0), R(0));
         if (lane > j && lane < w) {
-            const cpx<R> l = T[j][lane] * dinv;
-            T[j][lane] = l;
-            // rank-1 update of this row, 8 columns per batch of LDS loads
-            for (int c0 = j + 1; c0 < w; c0 += 8) {
-                cpx<R> pr[8], q[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (c0 + k < w) { pr[k] = T[c0 + k][j]; q[k] = T[c0 + k][lane]; }
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (c0 + k < w) T[c0 + k][lane] = q[k] - l * pr[k];
-            }
+            l = T[j][lane] * dinv;
+            for (int c = j + 1 + wq; c < w; c += 4) T[c][lane] = T[c][lane] - l * T[c][j];
         }
         __syncthreads();
+        // column j is read again only by wave 0 (next swap), after these writes
+        if (wq == 0) {
+            if (lane == j) T[j][j] = d;
+            if (lane > j && lane < w) T[j][lane] = l;
+        }
     }
-    for (int c = 0; c < w; ++c)
-        if (lane < w) base[(int64_t)c * ld + lane] = T[c][lane];
-    if (lane < w) perm[F.pivoff + kb + lane] = myperm;
+    __syncthreads();
+    if (wq == 0 && lane < w) perm[F.pivoff + kb + lane] = myperm;
     cpx<R>* Li = tiles + F.tileoff + (kb / NB) * 2 * NB * NB;
     cpx<R>* Ui = Li + NB * NB;
-    // unit-lower inverse, lane = column
-    if (lane < w) {
-        const int c = lane;
-        for (int i = 0; i < c; ++i) X[c][i] = cpx<R>(R(0), R(0));
-        X[c][c] = cpx<R>(R(1), R(0));
-        for (int i = c + 1; i < w; ++i) {
+    const int c = lane;
+    // L^{-1}[i][c] = delta_ic - sum_{k<i} L[i][k] L^{-1}[k][c]   (zero above the diagonal)
+    auto lower_inv = [&](cpx<R> (*XL)[NB + 1]) {
+        for (int i = 0; i < w; ++i) {
             cpx<R> s0(R(0), R(0)), s1(R(0), R(0));
-            for (int k0 = c; k0 < i; k0 += 8) {
-                cpx<R> a[8], x[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (k0 + k < i) { a[k] = T[k0 + k][i]; x[k] = X[c][k0 + k]; }
-#pragma unroll
-                for (int k = 0; k < 8; k += 2) {
-                    if (k0 + k < i) s0 = s0 + a[k] * x[k];
-                    if (k0 + k + 1 < i) s1 = s1 + a[k + 1] * x[k + 1];
-                }
+            int k = 0;
+            for (; k + 1 < i; k += 2) {
+                s0 = s0 + T[k][i] * XL[k][c];
+                s1 = s1 + T[k + 1][i] * XL[k + 1][c];
             }
-            X[c][i] = -(s0 + s1);
+            if (k < i) s0 = s0 + T[k][i] * XL[k][c];
+            XL[i][c] = i == c ? cpx<R>(R(1), R(0)) : (i < c ? cpx<R>(R(0), R(0)) : -(s0 + s1));
         }
+    };
+    // U^{-1}[i][c] = (delta_ic - sum_{k>i} U[i][k] U^{-1}[k][c]) / U[i][i]   (zero below)
+    auto upper_inv = [&](cpx<R> (*XU)[NB + 1]) {
+        for (int i = w - 1; i >= 0; --i) {
+            cpx<R> s0(R(0), R(0)), s1(R(0), R(0));
+            int k = i + 1;
+            for (; k + 1 < w; k += 2) {
+                s0 = s0 + T[k][i] * XU[k][c];
+                s1 = s1 + T[k + 1][i] * XU[k + 1][c];
+            }
+            if (k < w) s0 = s0 + T[k][i] * XU[k][c];
+            const cpx<R> rdi = cdiv(cpx<R>(R(1), R(0)), T[i][i]);
+            XU[i][c] = i > c ? cpx<R>(R(0), R(0)) : (i == c ? rdi : -((s0 + s1) * rdi));
+        }
+    };
+    auto store_inv = [&](cpx<R> (*Xs)[NB + 1], cpx<R>* dst) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int cc = wq + 4 * k;
+            if (cc < w && lane < w) dst[(int64_t)cc * NB + lane] = Xs[lane][cc];
+        }
+    };
+    if (NX == 2) {
+        if (wq == 0) lower_inv(X[0]);
+        else if (wq == 1) upper_inv(X[NX - 1]);
+    } else if (wq == 0) {
+        lower_inv(X[0]);
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int cc = wq + 4 * k;
+        if (cc < w && lane < w) base[(int64_t)cc * ld + lane] = T[cc][lane];
     }
     __syncthreads();
-    for (int c = 0; c < w; ++c)
-        if (lane < w) Li[(int64_t)c * NB + lane] = X[c][lane];
-    __syncthreads();
-    // upper inverse, lane = column; reciprocals of the diagonal first
-    __shared__ cpx<R> rd[NB];
-    if (lane < w) rd[lane] = cdiv(cpx<R>(R(1), R(0)), T[lane][lane]);
-    __syncthreads();
-    if (lane < w) {
-        const int c = lane;
-        for (int i = c + 1; i < w; ++i) X[c][i] = cpx<R>(R(0), R(0));
-        X[c][c] = rd[c];
-        for (int i = c - 1; i >= 0; --i) {
-            cpx<R> s0(R(0), R(0)), s1(R(0), R(0));
-            for (int k0 = i + 1; k0 <= c; k0 += 8) {
-                cpx<R> a[8], x[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (k0 + k <= c) { a[k] = T[k0 + k][i]; x[k] = X[c][k0 + k]; }
-#pragma unroll
-                for (int k = 0; k < 8; k += 2) {
-                    if (k0 + k <= c) s0 = s0 + a[k] * x[k];
-                    if (k0 + k + 1 <= c) s1 = s1 + a[k + 1] * x[k + 1];
-                }
-            }
-            X[c][i] = -((s0 + s1) * rd[i]);
-        }
+    store_inv(X[0], Li);
+    if (NX == 1) {
+        __syncthreads();
+        if (wq == 1) upper_inv(X[0]);
+        __syncthreads();
     }
-    __syncthreads();
-    for (int c = 0; c < w; ++c)
-        if (lane < w) Ui[(int64_t)c * NB + lane] = X[c][lane];
+    store_inv(X[NX - 1], Ui);
 }
 
 // ---- super-panel triangular solves of the block solve ----
 //
 // One launch covers one super-panel (<= WT_NB2 pivots) of every listed front
 // (entries {front, K0, K1}); blocks split the right-hand sides into groups
-// of SP_CG columns, which are independent, so a block walks the panel's
-// 64-pivot tiles sequentially with no inter-block synchronization.  Each
-// thread owns matrix rows and updates all SP_CG columns of a row, so every
-// factor entry it loads feeds SP_CG complex multiply-adds.
-constexpr int SP_CG = 16;
-constexpr int SP_CH = 16;  // factor entries loaded per batch (independent loads in flight)
+// of CG columns, which are independent, so a block walks the panel's
+// 64-pivot tiles sequentially with no inter-block synchronization.  Wave q
+// of the block's four owns columns [q CG/4, (q+1) CG/4) of the group and
+// lane l owns rows l, l+64, ...: every factor entry a lane loads feeds CG/4
+// complex multiply-adds, and the right-hand-side values come from LDS as
+// wave-uniform broadcasts.
+// acc[c] += sum_k A[k*lda + row] * V[k][cb + c] over k < w (V zero past w).
+// Factor entries are loaded in batches of KB, all in flight together: the
+// launches of the top of the tree have few blocks and are latency bound, so
+// their 4-column groups take the whole 64-entry row at once; the 16-column
+// groups of the wide launches keep a smaller batch for occupancy.
+template <typename R, int CG, int CPW>
+__device__ __forceinline__ void sp_dot(const cpx<R>* __restrict__ A, int64_t lda, int w,
+                                       const cpx<R> (*V)[CG], int cb, cpx<R>* acc)
+{
+    constexpr int KB = CG == 4 ? (sizeof(R) == 4 ? 64 : 32) : 16;
+    for (int k0 = 0; k0 < w; k0 += KB) {
+        cpx<R> a[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+            a[j] = k0 + j < w ? A[(int64_t)(k0 + j) * lda] : cpx<R>(R(0), R(0));
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) acc[c] = acc[c] + a[j] * V[k0 + j][cb + c];
+    }
+}
 
 // forward: y_t = L_t^{-1} P_t y_t, then y[t+w..K1) -= L[t+w..K1, t] y_t
-template <typename R>
+template <typename R, int CG>
 __global__ __launch_bounds__(256) void sp_fwd_kernel(const wt_dfront* __restrict__ fr, const int32_t* __restrict__ ent,
                                                      const cpx<R>* __restrict__ factor,
                                                      const cpx<R>* __restrict__ tiles,
                                                      const int32_t* __restrict__ perm, cpx<R>* __restrict__ y,
                                                      int64_t n, int nrhs)
 {
-    constexpr int NB = WT_NB;
-    __shared__ cpx<R> yt[NB][SP_CG];
-    __shared__ cpx<R> xt[NB][SP_CG];
+    constexpr int NB = WT_NB, CPW = CG / 4;
+    __shared__ cpx<R> yt[NB][CG];
+    __shared__ cpx<R> xt[NB][CG];
     const int32_t* e = ent + 3 * blockIdx.x;
     const wt_dfront F = fr[e[0]];
     const int64_t K0 = e[1], K1 = e[2];
-    const int c0 = blockIdx.y * SP_CG;
+    const int c0 = blockIdx.y * CG;
     if (c0 >= nrhs) return;
-    const int nc = nrhs - c0 < SP_CG ? nrhs - c0 : SP_CG;
+    const int nc = nrhs - c0 < CG ? nrhs - c0 : CG;
     const int64_t ld = F.p + F.u;
     const cpx<R>* L = factor + F.Loff;
     cpx<R>* yb = y + F.pstart + (int64_t)c0 * n;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, cb = (tid >> 6) * CPW;
     for (int64_t t = K0; t < K1; t += NB) {
         const int w = (int)(K1 - t < NB ? K1 - t : NB);
         const int32_t* pr = perm + F.pivoff + t;
-        for (int idx = tid; idx < NB * SP_CG; idx += blockDim.x) {
+        for (int idx = tid; idx < NB * CG; idx += blockDim.x) {
             const int i = idx % NB, c = idx / NB;
             yt[i][c] = (c < nc && i < w) ? yb[(int64_t)c * n + t + pr[i]] : cpx<R>(R(0), R(0));
-            xt[i][c] = cpx<R>(R(0), R(0));
         }
         __syncthreads();
-        const cpx<R>* Li = tiles + F.tileoff + (t / NB) * 2 * NB * NB;
-        if (tid < w) {
-            const int i = tid;
-            cpx<R> acc[SP_CG];
+        // the inverse holds explicit zeros above its diagonal inside the
+        // tile; entries past the tile width are never read
+        if (lane < w) {
+            const cpx<R>* Li = tiles + F.tileoff + (t / NB) * 2 * NB * NB + lane;
+            cpx<R> acc[CPW];
 #pragma unroll
-            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            // the inverse holds explicit zeros above its diagonal inside the
-            // tile; entries past the tile width are never read
-            for (int k0 = 0; k0 < w; k0 += SP_CH) {
-                cpx<R> a[SP_CH];
+            for (int c = 0; c < CPW; ++c) acc[c] = cpx<R>(R(0), R(0));
+            sp_dot<R, CG, CPW>(Li, NB, w, yt, cb, acc);
 #pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-                    a[j] = k0 + j < w ? Li[(int64_t)(k0 + j) * NB + i] : cpx<R>(R(0), R(0));
-#pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-#pragma unroll
-                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * yt[k0 + j][c];
+            for (int c = 0; c < CPW; ++c) {
+                xt[lane][cb + c] = acc[c];
+                if (cb + c < nc) yb[(int64_t)(cb + c) * n + t + lane] = acc[c];
             }
+        } else {
 #pragma unroll
-            for (int c = 0; c < SP_CG; ++c) {
-                xt[i][c] = acc[c];
-                if (c < nc) yb[(int64_t)c * n + t + i] = acc[c];
-            }
+            for (int c = 0; c < CPW; ++c) xt[lane][cb + c] = cpx<R>(R(0), R(0));
         }
         __syncthreads();
         const int rows = (int)(K1 - t - w);
-        for (int r = tid; r < rows; r += blockDim.x) {
+        for (int r = lane; r < rows; r += 64) {
             const int64_t row = t + w + r;
-            cpx<R> acc[SP_CG];
+            cpx<R> acc[CPW];
 #pragma unroll
-            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k0 = 0; k0 < w; k0 += SP_CH) {
-                cpx<R> a[SP_CH];
+            for (int c = 0; c < CPW; ++c) acc[c] = cpx<R>(R(0), R(0));
+            sp_dot<R, CG, CPW>(L + t * ld + row, ld, w, xt, cb, acc);
 #pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-                    a[j] = k0 + j < w ? L[(t + k0 + j) * ld + row] : cpx<R>(R(0), R(0));
-#pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-#pragma unroll
-                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * xt[k0 + j][c];
-            }
-            for (int c = 0; c < nc; ++c) {
-                cpx<R>* q = yb + (int64_t)c * n + row;
-                *q = *q - acc[c];
-            }
+            for (int c = 0; c < CPW; ++c)
+                if (cb + c < nc) {
+                    cpx<R>* q = yb + (int64_t)(cb + c) * n + row;
+                    *q = *q - acc[c];
+                }
         }
         __syncthreads();
     }
 }
 
 // backward: x_t = U_t^{-1} y_t, then y[K0..t) -= U[K0..t, t] x_t, tiles descending
-template <typename R>
+template <typename R, int CG>
 __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict__ fr, const int32_t* __restrict__ ent,
                                                      const cpx<R>* __restrict__ factor,
                                                      const cpx<R>* __restrict__ tiles, cpx<R>* __restrict__ y,
                                                      int64_t n, int nrhs)
 {
-    constexpr int NB = WT_NB;
-    __shared__ cpx<R> yt[NB][SP_CG];
-    __shared__ cpx<R> xt[NB][SP_CG];
+    constexpr int NB = WT_NB, CPW = CG / 4;
+    __shared__ cpx<R> yt[NB][CG];
+    __shared__ cpx<R> xt[NB][CG];
     const int32_t* e = ent + 3 * blockIdx.x;
     const wt_dfront F = fr[e[0]];
     const int64_t K0 = e[1], K1 = e[2];
-    const int c0 = blockIdx.y * SP_CG;
+    const int c0 = blockIdx.y * CG;
     if (c0 >= nrhs) return;
-    const int nc = nrhs - c0 < SP_CG ? nrhs - c0 : SP_CG;
+    const int nc = nrhs - c0 < CG ? nrhs - c0 : CG;
     const int64_t ld = F.p + F.u;
     const cpx<R>* L = factor + F.Loff;
     cpx<R>* yb = y + F.pstart + (int64_t)c0 * n;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, cb = (tid >> 6) * CPW;
     for (int64_t t = K0 + ((K1 - 1 - K0) / NB) * NB; t >= K0; t -= NB) {
         const int w = (int)(K1 - t < NB ? K1 - t : NB);
-        for (int idx = tid; idx < NB * SP_CG; idx += blockDim.x) {
+        for (int idx = tid; idx < NB * CG; idx += blockDim.x) {
             const int i = idx % NB, c = idx / NB;
             yt[i][c] = (c < nc && i < w) ? yb[(int64_t)c * n + t + i] : cpx<R>(R(0), R(0));
-            xt[i][c] = cpx<R>(R(0), R(0));
         }
         __syncthreads();
-        const cpx<R>* Ui = tiles + F.tileoff + (t / NB) * 2 * NB * NB + NB * NB;
-        if (tid < w) {
-            const int i = tid;
-            cpx<R> acc[SP_CG];
+        if (lane < w) {
+            const cpx<R>* Ui = tiles + F.tileoff + (t / NB) * 2 * NB * NB + NB * NB + lane;
+            cpx<R> acc[CPW];
 #pragma unroll
-            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k0 = 0; k0 < w; k0 += SP_CH) {
-                cpx<R> a[SP_CH];
+            for (int c = 0; c < CPW; ++c) acc[c] = cpx<R>(R(0), R(0));
+            sp_dot<R, CG, CPW>(Ui, NB, w, yt, cb, acc);
 #pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-                    a[j] = k0 + j < w ? Ui[(int64_t)(k0 + j) * NB + i] : cpx<R>(R(0), R(0));
-#pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-#pragma unroll
-                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * yt[k0 + j][c];
+            for (int c = 0; c < CPW; ++c) {
+                xt[lane][cb + c] = acc[c];
+                if (cb + c < nc) yb[(int64_t)(cb + c) * n + t + lane] = acc[c];
             }
+        } else {
 #pragma unroll
-            for (int c = 0; c < SP_CG; ++c) {
-                xt[i][c] = acc[c];
-                if (c < nc) yb[(int64_t)c * n + t + i] = acc[c];
-            }
+            for (int c = 0; c < CPW; ++c) xt[lane][cb + c] = cpx<R>(R(0), R(0));
         }
         __syncthreads();
         const int rows = (int)(t - K0);
-        for (int r = tid; r < rows; r += blockDim.x) {
+        for (int r = lane; r < rows; r += 64) {
             const int64_t row = K0 + r;
-            cpx<R> acc[SP_CG];
+            cpx<R> acc[CPW];
 #pragma unroll
-            for (int c = 0; c < SP_CG; ++c) acc[c] = cpx<R>(R(0), R(0));
-            for (int k0 = 0; k0 < w; k0 += SP_CH) {
-                cpx<R> a[SP_CH];
+            for (int c = 0; c < CPW; ++c) acc[c] = cpx<R>(R(0), R(0));
+            sp_dot<R, CG, CPW>(L + t * ld + row, ld, w, xt, cb, acc);
 #pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-                    a[j] = k0 + j < w ? L[(t + k0 + j) * ld + row] : cpx<R>(R(0), R(0));
-#pragma unroll
-                for (int j = 0; j < SP_CH; ++j)
-#pragma unroll
-                    for (int c = 0; c < SP_CG; ++c) acc[c] = acc[c] + a[j] * xt[k0 + j][c];
-            }
-            for (int c = 0; c < nc; ++c) {
-                cpx<R>* q = yb + (int64_t)c * n + row;
-                *q = *q - acc[c];
-            }
+            for (int c = 0; c < CPW; ++c)
+                if (cb + c < nc) {
+                    cpx<R>* q = yb + (int64_t)(cb + c) * n + row;
+                    *q = *q - acc[c];
+                }
         }
         __syncthreads();
         if (t == K0) break;
@@ -630,8 +521,8 @@ int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t k
                   int32_t* perm, int* status, hipStream_t s)
 {
     if (nlist <= 0) return WT_OK;
-    hipLaunchKernelGGL((diag_tile_wave_kernel<R, WT_NB>), dim3(nlist), dim3(64), 0, s, fr, list, kb, factor,
-                       tiles, perm, status);
+    hipLaunchKernelGGL(diag_tile_kernel<R>, dim3(nlist), dim3(256), 0, s, fr, list, kb, factor, tiles, perm,
+                       status);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
@@ -668,11 +559,23 @@ int wt_sp_solve(int fwd, const wt_dfront* fr, const int32_t* ent, int nent, cons
                 const cpx<R>* tiles, const int32_t* perm, cpx<R>* y, int64_t n, int nrhs, hipStream_t s)
 {
     if (nent <= 0) return WT_OK;
-    dim3 grid(nent, (nrhs + SP_CG - 1) / SP_CG);
-    if (fwd)
-        hipLaunchKernelGGL(sp_fwd_kernel<R>, grid, dim3(256), 0, s, fr, ent, factor, tiles, perm, y, n, nrhs);
-    else
-        hipLaunchKernelGGL(sp_bwd_kernel<R>, grid, dim3(256), 0, s, fr, ent, factor, tiles, y, n, nrhs);
+    // 16-column groups load each factor entry for the most columns; launches
+    // with few fronts (the top of the tree) use 4-column groups so that more
+    // blocks share the sequential tile walk
+    const bool wide = (int64_t)nent * ((nrhs + 15) / 16) >= 1024;
+    const int cg = wide ? 16 : 4;
+    dim3 grid(nent, (nrhs + cg - 1) / cg);
+    if (fwd) {
+        if (wide)
+            hipLaunchKernelGGL((sp_fwd_kernel<R, 16>), grid, dim3(256), 0, s, fr, ent, factor, tiles, perm, y, n, nrhs);
+        else
+            hipLaunchKernelGGL((sp_fwd_kernel<R, 4>), grid, dim3(256), 0, s, fr, ent, factor, tiles, perm, y, n, nrhs);
+    } else {
+        if (wide)
+            hipLaunchKernelGGL((sp_bwd_kernel<R, 16>), grid, dim3(256), 0, s, fr, ent, factor, tiles, y, n, nrhs);
+        else
+            hipLaunchKernelGGL((sp_bwd_kernel<R, 4>), grid, dim3(256), 0, s, fr, ent, factor, tiles, y, n, nrhs);
+    }
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
