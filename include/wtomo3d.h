@@ -148,6 +148,58 @@ int wt_solver_apply(wt_solver *h, int nrhs, const void *x, void *y, int on_devic
  * z positive down), r in metres on the solver's grid with spacing dx. */
 int wt_solver_planewaves(wt_solver *h, int nrhs, const double *dirs, double kmag, double dx, void *d_x);
 
+/* ---- plane-wave right-hand sides of a layered half-space ---- */
+
+/* Thomson-Haskell depth profiles of plane waves incident from below on a
+ * layered half-space; replaces HASKGRN (haskgrn.f:31-672), whose inputs are
+ * module globals (HASKELL_MODULE vp1d, vs1d, rh1d, h1d, vp1disc, vs1disc,
+ * nl; SRCPRM_MODULE ain, srctyp; MODEL_MODULE dz, nz, zorig), one source
+ * per call there and ns sources per call here.
+ *   nl layers (<= 64), the last one the half-space: vp, vs continuous speeds
+ *   (they set the horizontal speed cv = v(nl)/sin(ain)), vpd, vsd the
+ *   discrete speeds of the layer terms, rho densities, top = depth of each
+ *   layer's top (top[0] <= zorig);
+ *   ain[ns] incidence angles (rad), is_s[ns] = 1 for an incident S wave
+ *   (srctyp 'ps'), 0 for P; omega angular frequency (the reference's df);
+ *   depths dz (iz-1) + zorig, iz = 1..nz.
+ * prec WT_PREC_F32: float arrays and scalars rounded to float, complex64
+ * profiles, with the reference's single-precision subexpressions (as it is
+ * compiled); WT_PREC_F64: double / complex128 throughout (the reference
+ * with 8-byte REAL).
+ * Out: device d_u, d_w [ns][nz] horizontal (along the propagation azimuth)
+ * and vertical (positive down) profiles; device d_pic[ns] = omega / cv
+ * (double); host status[ns]: 0, or 1 when a depth lies above top[0] (the
+ * reference's ierr = 1; the profile is zero from that depth on).  nl < 2
+ * with nz > 1 is refused (the reference would read h1d(0)). */
+int wt_haskgrn(int prec, int nl, const void *vp, const void *vs, const void *rho, const void *top,
+               const void *vpd, const void *vsd, int ns, const void *ain, const int32_t *is_s, double omega,
+               int nz, double dz, double zorig, void *d_u, void *d_w, double *d_pic, int32_t *status);
+
+/* 3-D background fields of plane waves from their depth profiles (the
+ * expansion elwave's missing planewvexp.f90 would do; defined by this
+ * library):
+ *   u_j = (U_j(z) cos phi_j, U_j(z) sin phi_j, W_j(z))
+ *         * exp(-i kh_j ((ix-1) dx cos phi_j + (iy-1) dx sin phi_j)),
+ * phi_j the azimuth of horizontal propagation from +x towards +y (host,
+ * radians), kh_j = d_kh[j] (device; wt_haskgrn's d_pic), profiles d_u, d_w
+ * [nrhs][nz] (device, complex64 for prec WT_PREC_F32, complex128 for
+ * WT_PREC_F64).  Out: device d_x, row-major n x nrhs complex128 in the
+ * unknown order (node z fastest, then x, then y; components u, v, w). */
+int wt_planewave_layered(int nx, int ny, int nz, double dx, int prec, int nrhs, const void *d_u,
+                         const void *d_w, const double *d_kh, const double *phi, void *d_x);
+
+/* Scattered-field source; replaces HETFLDSRC (hetfldsrc.f90:17-1039):
+ *   sfld_j = srcfn_j (A_bg - A) u_bg_j,
+ * A the handle's FP64 operator from its last wt_solver_assemble, A_bg the
+ * operator of the background model (lam_bg, mu_bg complex128, rho_bg
+ * double; (nz,nx,ny) z fastest; host, or device with on_device) assembled
+ * with the same parameters.  d_ubg, d_sfld: device, row-major n x nrhs
+ * complex128; srcfn: host, nrhs complex128 (NULL: all 1).  Node numbering
+ * is the unknown order of the CSR; the reference's own (hetfldsrc.f90:1020)
+ * agrees with it only when ny = nz.  WT_ERR_STATE without an assembly. */
+int wt_solver_scattered_source(wt_solver *h, const void *lam_bg, const void *mu_bg, const double *rho_bg,
+                               int on_device, int nrhs, const void *d_ubg, const double *srcfn, void *d_sfld);
+
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
  * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and
  * contribution-block updates), 1 solve GEMMs, 2 diagonal-tile LU, 3

@@ -142,3 +142,62 @@ def ref_assemble(nx, ny, nz, dx, freq, vp, vs, rho, pmld, pmlr, fst=True, usemin
                      F(pmld), F(pmlr), _ptr(vp), _ptr(vs), _ptr(rho), I(jc.size),
                      _ptr(ir), _ptr(jc), _ptr(a), ctypes.byref(ierr))
     return ierr.value, ir, jc, a
+
+
+def _hk_arrays(layers, ft):
+    """layers: dict of per-layer sequences vp, vs, rho, top, vpd, vsd."""
+    return [np.ascontiguousarray(layers[k], ft) for k in ("vp", "vs", "rho", "top", "vpd", "vsd")]
+
+
+def haskgrn(layers, ain, is_s, omega, nz, dz, zorig, prec=32):
+    """Oracle HASKGRN (oracle/hask_oracle.c) -> (rc, ugrn, wgrn, pic)."""
+    R = ctypes.c_float if prec == 32 else ctypes.c_double
+    ft = np.float32 if prec == 32 else np.float64
+    ct = np.complex64 if prec == 32 else np.complex128
+    f = getattr(oracle(), "ora_haskgrn_f%d" % prec)
+    f.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6 + [R, ctypes.c_int, R, ctypes.c_int, R, R] + \
+        [ctypes.c_void_p] * 2 + [ctypes.POINTER(ctypes.c_double)]
+    f.restype = ctypes.c_int
+    arrs = _hk_arrays(layers, ft)
+    u = np.zeros(nz, ct)
+    w = np.zeros(nz, ct)
+    pic = ctypes.c_double()
+    rc = f(len(arrs[0]), *[_ptr(a) for a in arrs], ain, int(is_s), omega, nz, dz, zorig, _ptr(u), _ptr(w),
+           ctypes.byref(pic))
+    return rc, u, w, pic.value
+
+
+def ref_haskgrn(layers, ain, is_s, omega, nz, dz, zorig, prec=32):
+    """Reference HASKGRN (haskgrn.f:31) through oracle/_ref -> (ierr, ugrn, wgrn)."""
+    L = ref(prec)
+    R = ctypes.c_float if prec == 32 else ctypes.c_double
+    ft = np.float32 if prec == 32 else np.float64
+    ct = np.complex64 if prec == 32 else np.complex128
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    F = lambda v: ctypes.byref(R(v))
+    arrs = _hk_arrays(layers, ft)
+    u = np.zeros(nz, ct)
+    w = np.zeros(nz, ct)
+    ierr = ctypes.c_int(-1)
+    L.wtref_haskgrn(I(len(arrs[0])), *[_ptr(a) for a in arrs], F(ain), I(int(is_s)), F(omega), I(nz), F(dz),
+                    F(zorig), _ptr(u), _ptr(w), ctypes.byref(ierr))
+    return ierr.value, u, w
+
+
+def ref_hetfldsrc(nx, ny, nz, dx, freq, vp, vs, rho, vpb, vsb, rhob, ub, vb, wb, srcfn, pmld, pmlr,
+                  fst=True, usemin=True, tau=999.999, prec=64):
+    """Reference HETFLDSRC (hetfldsrc.f90:17) -> (ierr, sfld[3 n]); grids (nz,nx,ny)."""
+    L = ref(prec)
+    R = ctypes.c_float if prec == 32 else ctypes.c_double
+    ft = np.float32 if prec == 32 else np.float64
+    ct = np.complex64 if prec == 32 else np.complex128
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    F = lambda v: ctypes.byref(R(v))
+    reals = [np.ascontiguousarray(a, ft) for a in (vp, vs, rho, vpb, vsb, rhob)]
+    flds = [np.ascontiguousarray(a, ct) for a in (ub, vb, wb)]
+    sf = np.zeros(3 * nx * ny * nz, ct)
+    ierr = ctypes.c_int(-1)
+    L.wtref_hetfldsrc(I(nx), I(ny), I(nz), F(dx), F(freq), F(tau), I(int(fst)), I(int(usemin)), F(pmld),
+                      F(pmlr), *[_ptr(a) for a in reals], *[_ptr(a) for a in flds], F(complex(srcfn).real),
+                      F(complex(srcfn).imag), _ptr(sf), ctypes.byref(ierr))
+    return ierr.value, sf

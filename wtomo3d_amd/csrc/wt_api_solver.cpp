@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "wt_solver.h"
+#include "wt_rhs.h"
 #include "wt_vec.h"
 
 int64_t wt_nnz(int nx, int ny, int nz);
@@ -239,6 +240,8 @@ int wt_solver_assemble(wt_solver* h, const wt_grid_params* p)
     s.prof.flush();
     if (he) return WT_ERR_ARG;
     s.have_vals = true;
+    s.last_p = *p;
+    s.have_p = true;
     return WT_OK;
 }
 
@@ -246,6 +249,7 @@ int wt_solver_set_values(wt_solver* h, const void* vals, int prec, int on_device
 {
     if (!h || !vals) return WT_ERR_ARG;
     wt_solver_impl& s = h->s;
+    s.have_p = false;
     int rc;
     if (!s.vals && (rc = dalloc(&s.vals, s.nnz))) return rc;
     hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -319,6 +323,67 @@ int wt_solver_planewaves(wt_solver* h, int nrhs, const double* dirs, double kmag
     if (!rc) rc = wt_planewave((cpx<double>*)d_x, s.S.nx, s.S.ny, s.S.nz, dx, kmag, dd, nrhs, s.stream);
     if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
     (void)hipFree(dd);
+    return rc;
+}
+
+int wt_solver_scattered_source(wt_solver* h, const void* lam_bg, const void* mu_bg, const double* rho_bg,
+                               int on_device, int nrhs, const void* d_ubg, const double* srcfn, void* d_sfld)
+{
+    if (!h || !lam_bg || !mu_bg || !rho_bg || !d_ubg || !d_sfld || nrhs < 1) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    if (!s.have_vals || !s.have_p) return WT_ERR_STATE;
+    const int64_t nodes = s.n / 3;
+    cpx<double>*lb = nullptr, *mb = nullptr, *ab = nullptr, *sf = nullptr;
+    double* rb = nullptr;
+    int rc = WT_OK;
+    auto cleanup = [&] {
+        if (lb) (void)hipFree(lb);
+        if (mb) (void)hipFree(mb);
+        if (rb) (void)hipFree(rb);
+        if (ab) (void)hipFree(ab);
+        if (sf) (void)hipFree(sf);
+    };
+    if ((rc = dalloc(&lb, nodes)) || (rc = dalloc(&mb, nodes)) || (rc = dalloc(&rb, nodes)) ||
+        (rc = dalloc(&ab, s.nnz)) || (srcfn && (rc = dalloc(&sf, nrhs)))) {
+        cleanup();
+        return rc;
+    }
+    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (hipMemcpyAsync(lb, lam_bg, nodes * 16, k, s.stream) != hipSuccess ||
+        hipMemcpyAsync(mb, mu_bg, nodes * 16, k, s.stream) != hipSuccess ||
+        hipMemcpyAsync(rb, rho_bg, nodes * 8, k, s.stream) != hipSuccess ||
+        (sf && hipMemcpyAsync(sf, srcfn, nrhs * 16, hipMemcpyHostToDevice, s.stream) != hipSuccess) ||
+        hipMemsetAsync(s.d_err, 0, sizeof(int), s.stream) != hipSuccess) {
+        cleanup();
+        return WT_ERR_HIP;
+    }
+    // the background operator at the parameters of the last assembly
+    rc = wt_launch_values<double>(s.last_p, lb, mb, rb, ab, s.d_err, s.stream);
+    const cpx<double>* u = (const cpx<double>*)d_ubg;
+    cpx<double>* out = (cpx<double>*)d_sfld;
+    for (int c0 = 0; !rc && c0 < nrhs; c0 += 64) {
+        const int nc = nrhs - c0 < 64 ? nrhs - c0 : 64;
+        if (nc == nrhs) {
+            rc = wt_launch_scatsrc(s.rowptr, s.colidx, s.vals, ab, u, sf, out, s.n, nc, s.stream);
+        } else {
+            // column groups of a wider block go through the refinement buffers
+            if ((rc = ensure_ir(s, 64))) break;
+            if (hipMemcpy2DAsync(s.xb, nc * 16, u + c0, (size_t)nrhs * 16, nc * 16, s.n, hipMemcpyDeviceToDevice,
+                                 s.stream) != hipSuccess) { rc = WT_ERR_HIP; break; }
+            rc = wt_launch_scatsrc(s.rowptr, s.colidx, s.vals, ab, s.xb, sf ? sf + c0 : nullptr, s.rb, s.n, nc,
+                                   s.stream);
+            if (!rc && hipMemcpy2DAsync(out + c0, (size_t)nrhs * 16, s.rb, nc * 16, nc * 16, s.n,
+                                        hipMemcpyDeviceToDevice, s.stream) != hipSuccess)
+                rc = WT_ERR_HIP;
+        }
+    }
+    int he = 0;
+    if (!rc && (hipMemcpyAsync(&he, s.d_err, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+                hipStreamSynchronize(s.stream) != hipSuccess))
+        rc = WT_ERR_HIP;
+    if (rc == WT_OK && he) rc = WT_ERR_ARG;
+    if (rc != WT_OK) (void)hipStreamSynchronize(s.stream);
+    cleanup();
     return rc;
 }
 

@@ -202,6 +202,8 @@ struct wt_solver_impl {
     cpx<double>* vals = nullptr;  // CSR values, complex128
     double* rscale = nullptr;     // row equilibration of the factored matrix
     bool have_vals = false;
+    wt_grid_params last_p{};  // parameters of the last assembly (have_p)
+    bool have_p = false;
     int* d_err = nullptr;
     // refinement buffers (row-major n x 64)
     int ir_cap = 0;

@@ -19,6 +19,19 @@ def layered(nx, ny, nz, dx, seed, nlayers=6):
     return vp, vp / 1.73, 310.0 * vp ** 0.25
 
 
+def layer_stack(nx, ny, nz, dx, seed, nlayers=6):
+    """The layers of ``layered`` (same seed, same draws) as a 1-D stack for
+    the Thomson-Haskell profiles: dict of per-layer arrays ``top`` (depth of
+    each layer's top, the first 0), ``vp``, ``vs``, ``rho``; the last layer
+    is the half-space.  The discrete speeds ``vpd``/``vsd`` are the same."""
+    rng = np.random.default_rng(seed)
+    tops = np.sort(rng.uniform(0.0, nz * dx, nlayers - 1))
+    vps = np.sort(rng.uniform(4000.0, 8000.0, nlayers))
+    top = np.concatenate([[0.0], tops])
+    vs = vps / 1.73
+    return dict(top=top, vp=vps, vs=vs, rho=310.0 * vps ** 0.25, vpd=vps.copy(), vsd=vs.copy())
+
+
 def gaussian_field(shape, dx, corr_len, std, seed):
     """Zero-mean Gaussian random field with Gaussian covariance of the given
     correlation length, scaled to the requested standard deviation

@@ -1,0 +1,108 @@
+"""Plane-wave right-hand sides of a layered half-space, on the device.
+
+The reference computes 1-D Thomson-Haskell depth profiles per plane-wave
+source with HASKGRN (haskgrn.f:31-672) and the scattered-field source
+(A_bg - A) u_bg with HETFLDSRC (hetfldsrc.f90:17-1039); the expansion of the
+profiles to 3-D fields (planewvexp.f90) is missing from the reference and
+is defined here as
+
+    u_j = (U_j(z) cos phi_j, U_j(z) sin phi_j, W_j(z))
+          * exp(-i kh_j (x cos phi_j + y sin phi_j)),   kh_j = omega / cv_j
+
+(phi_j the azimuth of horizontal propagation from +x towards +y, z down).
+
+    prof = profiles(layers, ain, is_s, omega, nz, dz)      # HASKGRN, ns sources
+    ubg = background_fields(nx, ny, nz, dx, prof, phi)     # device block (n, ns)
+    sfld = solver.scattered_source(lam_bg, mu_bg, rho_bg, ubg, srcfn)
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib, check
+
+_vp = ctypes.c_void_p
+WT_PREC_F32, WT_PREC_F64 = 32, 64
+
+
+class DeviceBuffer:
+    """A device allocation of the library (freed with the object)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(lib().wt_dev_alloc(max(self.nbytes, 1), ctypes.byref(self.ptr)), "wt_dev_alloc")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(lib().wt_h2d(b.ptr, a.ctypes.data_as(_vp), a.nbytes), "wt_h2d")
+        return b
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().wt_d2h(out.ctypes.data_as(_vp), self.ptr, out.nbytes), "wt_d2h")
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().wt_dev_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Profiles:
+    """Device-resident depth profiles of ns sources: U, W [ns][nz] and kh[ns]."""
+
+    def __init__(self, U, W, kh, ns, nz, prec, status):
+        self.U, self.W, self.kh = U, W, kh
+        self.ns, self.nz, self.prec = ns, nz, prec
+        self.status = status
+
+    def host(self):
+        ct = np.complex64 if self.prec == WT_PREC_F32 else np.complex128
+        return (self.U.to_host(ct, (self.ns, self.nz)), self.W.to_host(ct, (self.ns, self.nz)),
+                self.kh.to_host(np.float64, (self.ns,)))
+
+
+def profiles(layers, ain, is_s, omega, nz, dz, zorig=0.0, prec=WT_PREC_F64):
+    """HASKGRN for every source on the device.
+
+    layers: dict of per-layer sequences ``vp, vs, rho, top, vpd, vsd`` (the
+    last layer the half-space); ain: incidence angles (radians); is_s: 1 for
+    an incident S wave, 0 for P.  Raises on a reference error (a depth above
+    the first layer top)."""
+    ft = np.float32 if prec == WT_PREC_F32 else np.float64
+    ct = np.complex64 if prec == WT_PREC_F32 else np.complex128
+    arrs = [np.ascontiguousarray(layers[k], ft) for k in ("vp", "vs", "rho", "top", "vpd", "vsd")]
+    ain = np.ascontiguousarray(np.atleast_1d(ain), ft)
+    ns = ain.size
+    iss = np.ascontiguousarray(np.broadcast_to(np.asarray(is_s, np.int32), (ns,)))
+    U = DeviceBuffer(ns * nz * np.dtype(ct).itemsize)
+    W = DeviceBuffer(ns * nz * np.dtype(ct).itemsize)
+    kh = DeviceBuffer(ns * 8)
+    status = np.zeros(ns, np.int32)
+    check(lib().wt_haskgrn(prec, arrs[0].size, *[a.ctypes.data_as(_vp) for a in arrs], ns,
+                           ain.ctypes.data_as(_vp), iss.ctypes.data_as(_vp), float(omega), nz, float(dz),
+                           float(zorig), U.ptr, W.ptr, kh.ptr, status.ctypes.data_as(_vp)), "wt_haskgrn")
+    if status.any():
+        raise RuntimeError("haskgrn: depth above the first layer top for sources %s" % np.nonzero(status)[0])
+    return Profiles(U, W, kh, ns, nz, prec, status)
+
+
+def background_fields(nx, ny, nz, dx, prof, phi):
+    """3-D background fields of the profiled plane waves: a device block,
+    row-major (n, ns) complex128 in the unknown order."""
+    assert prof.nz == nz
+    phi = np.ascontiguousarray(np.broadcast_to(np.asarray(phi, np.float64), (prof.ns,)))
+    X = DeviceBuffer(3 * nx * ny * nz * prof.ns * 16)
+    check(lib().wt_planewave_layered(nx, ny, nz, float(dx), prof.prec, prof.ns, prof.U.ptr, prof.W.ptr,
+                                     prof.kh.ptr, phi.ctypes.data_as(_vp), X.ptr), "wt_planewave_layered")
+    return X

@@ -125,3 +125,31 @@ class Solver:
                                     ctypes.byref(st)), "wt_solver_solve")
         stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
         return (X[:, 0] if one else X), stats
+
+    def scattered_source(self, lam_bg, mu_bg, rho_bg, ubg, srcfn=None):
+        """HETFLDSRC (hetfldsrc.f90:17-1039): srcfn_j (A_bg - A) u_bg_j with A
+        the operator of the last ``assemble`` and A_bg the background model's at
+        the same parameters.  ubg: a device block (rhs.DeviceBuffer, row-major
+        (n, nrhs) complex128) or a host array (n, nrhs); returns the same kind."""
+        from .rhs import DeviceBuffer
+        host = not isinstance(ubg, DeviceBuffer)
+        if host:
+            ubg = np.ascontiguousarray(np.asarray(ubg).reshape(self.n, -1), np.complex128)
+            nrhs = ubg.shape[1]
+            ub = DeviceBuffer.from_host(ubg)
+        else:
+            nrhs = ubg.nbytes // (self.n * 16)
+            ub = ubg
+        lb = np.ascontiguousarray(lam_bg, np.complex128).ravel()
+        mb = np.ascontiguousarray(mu_bg, np.complex128).ravel()
+        rb = np.ascontiguousarray(rho_bg, np.float64).ravel()
+        sf = None
+        if srcfn is not None:
+            sf = np.ascontiguousarray(np.broadcast_to(np.asarray(srcfn, np.complex128), (nrhs,)))
+        out = DeviceBuffer(self.n * nrhs * 16)
+        check(lib().wt_solver_scattered_source(self._h, _p(lb), _p(mb), _p(rb), 0, nrhs, ub.ptr,
+                                               None if sf is None else _p(sf), out.ptr),
+              "wt_solver_scattered_source")
+        if host:
+            return out.to_host(np.complex128, (self.n, nrhs))
+        return out
