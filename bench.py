@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain complex128 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-latency sample")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="requests/s of the serving sample")
+    ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the serving sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 5 times on BYTES (HBM counter calibration)")
     return ap.parse_args()
@@ -84,6 +87,26 @@ def cpu_baseline(nx, ny, nz, nrhs, budget):
                       "(p,u)=%s of the 96x96x64 elimination tree, %.1f s at %.3g GFLOP/s; whole-tree "
                       "%.3g flop extrapolated to %.1f s per 64-RHS factorization (solve not counted)"
                       % (len(sample), sample[:3], t, rate / 1e9, total, t_tree)}
+
+
+def progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def serving_sample(rate, seconds):
+    """Latency of the serving loop on a bounded sample of the Poisson
+    request stream: heterogeneous models with sides uniform in 32-80 nodes,
+    0.05-0.5 Hz, 1-64 plane-wave sources, 30 % of requests on a (model,
+    frequency) whose factor is resident; realtime arrivals at `rate`."""
+    from wtomo3d_amd import serving
+    reqs = serving.poisson_stream(rate, seconds, seed=1005, side=(32, 80), freq=(0.05, 0.5), nrhs=(1, 64),
+                                  reuse=0.3, dx=200.0)
+    eng = serving.Engine(hbm_budget=200e9, slots=4, tol=1e-8, max_iter=30)
+    wall = eng.run(reqs, realtime=True)
+    eng.close()
+    rep = serving.latency_report(reqs, wall)
+    rep.update(rate_per_s=rate, window_s=seconds, sides="32-80", slots=4)
+    return rep
 
 
 def main():
@@ -133,8 +156,10 @@ def main():
         W._lib.check(L.wt_solver_solve(s._h, prec, nrhs, dB, dX, 1, tol, maxit, ctypes.byref(st)), "solve")
         return st.rel_residual, st.iterations
 
+    progress("symbolic %.2f s; warmup" % t_symbolic)
     for _ in range(a.warmup):
         step(32, a.tol, 30)
+    progress("timed steps")
     L.wt_dev_sync()
     s.profile(True)
     L.wt_dev_sync()
@@ -153,6 +178,7 @@ def main():
     s.release(32)
 
     fp64 = None
+    progress("FP32 path %.3f s/step" % t32)
     if not a.no_fp64:
         need = s.bytes_needed(64, nrhs)
         try:
@@ -175,7 +201,10 @@ def main():
     achieved = g_fac["flops"] / (g_fac["ms"] * 1e-3) / 1e12 if g_fac["ms"] > 0 else 0.0
     asm = prof["assemble"]
     asm_gbs = asm["bytes"] / (asm["ms"] * 1e-3) / 1e9 if asm["ms"] > 0 else 0.0
+    progress("CPU baseline")
     cpu = None if a.no_cpu else cpu_baseline(nx, ny, nz, nrhs, a.cpu_budget)
+    progress("serving sample")
+    serve = None if a.no_serve else serving_sample(a.serve_rate, a.serve_seconds)
     value = nrhs / t32
     out = {
         "metric": METRIC, "value": value, "unit": "RHS/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -195,7 +224,10 @@ def main():
         "fp64": fp64,
         "assembly": {"gbs": asm_gbs, "frac_hbm": asm_gbs / PEAK_HBM_GBS, "ms": asm["ms"] / max(asm["launches"], 1)},
         "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()},
-        "symbolic_s": t_symbolic, "p99_latency_ms": None,
+        "symbolic_s": t_symbolic,
+        "p50_latency_ms": serve["p50_latency_ms"] if serve else None,
+        "p99_latency_ms": serve["p99_latency_ms"] if serve else None,
+        "serving": serve,
     }
     if fp64 and "rhs_per_s" in fp64:
         out["fp32ir_over_fp64"] = value / fp64["rhs_per_s"]

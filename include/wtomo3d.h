@@ -170,10 +170,13 @@ int wt_solver_planewaves(wt_solver *h, int nrhs, const double *dirs, double kmag
  * and vertical (positive down) profiles; device d_pic[ns] = omega / cv
  * (double); host status[ns]: 0, or 1 when a depth lies above top[0] (the
  * reference's ierr = 1; the profile is zero from that depth on).  nl < 2
- * with nz > 1 is refused (the reference would read h1d(0)). */
+ * with nz > 1 is refused (the reference would read h1d(0)).  Runs on the
+ * given HIP stream (NULL: the default stream) and returns when it is done;
+ * scratch is stream-ordered, so other streams are not waited for. */
 int wt_haskgrn(int prec, int nl, const void *vp, const void *vs, const void *rho, const void *top,
                const void *vpd, const void *vsd, int ns, const void *ain, const int32_t *is_s, double omega,
-               int nz, double dz, double zorig, void *d_u, void *d_w, double *d_pic, int32_t *status);
+               int nz, double dz, double zorig, void *d_u, void *d_w, double *d_pic, int32_t *status,
+               void *stream);
 
 /* 3-D background fields of plane waves from their depth profiles (the
  * expansion elwave's missing planewvexp.f90 would do; defined by this
@@ -184,9 +187,10 @@ int wt_haskgrn(int prec, int nl, const void *vp, const void *vs, const void *rho
  * radians), kh_j = d_kh[j] (device; wt_haskgrn's d_pic), profiles d_u, d_w
  * [nrhs][nz] (device, complex64 for prec WT_PREC_F32, complex128 for
  * WT_PREC_F64).  Out: device d_x, row-major n x nrhs complex128 in the
- * unknown order (node z fastest, then x, then y; components u, v, w). */
+ * unknown order (node z fastest, then x, then y; components u, v, w).
+ * Runs on the given stream (NULL: default) and returns when it is done. */
 int wt_planewave_layered(int nx, int ny, int nz, double dx, int prec, int nrhs, const void *d_u,
-                         const void *d_w, const double *d_kh, const double *phi, void *d_x);
+                         const void *d_w, const double *d_kh, const double *phi, void *d_x, void *stream);
 
 /* Scattered-field source; replaces HETFLDSRC (hetfldsrc.f90:17-1039):
  *   sfld_j = srcfn_j (A_bg - A) u_bg_j,
@@ -199,6 +203,15 @@ int wt_planewave_layered(int nx, int ny, int nz, double dx, int prec, int nrhs, 
  * agrees with it only when ny = nz.  WT_ERR_STATE without an assembly. */
 int wt_solver_scattered_source(wt_solver *h, const void *lam_bg, const void *mu_bg, const double *rho_bg,
                                int on_device, int nrhs, const void *d_ubg, const double *srcfn, void *d_sfld);
+
+/* Total field from the scattered field: d_uh += d_ubg over count complex128
+ * values (device); replaces the totfld branch of MAKEUH (makeuh.f:41-59).
+ * Runs on the given stream (NULL: default) and returns when it is done. */
+int wt_total_field(void *d_uh, const void *d_ubg, int64_t count, void *stream);
+
+/* The HIP stream a solver handle runs its work on (for callers that put
+ * their own device work in order with it). */
+int wt_solver_stream(const wt_solver *h, void **stream);
 
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
  * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and

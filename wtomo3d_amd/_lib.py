@@ -87,12 +87,16 @@ def _declare(L):
     L.wt_solver_profile_read.argtypes = [_vp, i32, ctypes.POINTER(ctypes.c_int64), dp, dp, dp]
     L.wt_solver_profile_read.restype = ctypes.c_int
     L.wt_haskgrn.argtypes = [i32, i32] + [_vp] * 6 + [i32, _vp, _vp, ctypes.c_double, i32, ctypes.c_double,
-                                                     ctypes.c_double, _vp, _vp, _vp, _vp]
+                                                     ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
     L.wt_haskgrn.restype = ctypes.c_int
-    L.wt_planewave_layered.argtypes = [i32] * 3 + [ctypes.c_double, i32, i32, _vp, _vp, _vp, _vp, _vp]
+    L.wt_planewave_layered.argtypes = [i32] * 3 + [ctypes.c_double, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp]
     L.wt_planewave_layered.restype = ctypes.c_int
     L.wt_solver_scattered_source.argtypes = [_vp, _vp, _vp, _vp, i32, i32, _vp, _vp, _vp]
     L.wt_solver_scattered_source.restype = ctypes.c_int
+    L.wt_total_field.argtypes = [_vp, _vp, i64, _vp]
+    L.wt_total_field.restype = ctypes.c_int
+    L.wt_solver_stream.argtypes = [_vp, ctypes.POINTER(ctypes.c_void_p)]
+    L.wt_solver_stream.restype = ctypes.c_int
     L.strumpack_serial_finter.argtypes = [i32p, i32p, i32p, _vp, _vp, _vp, _vp, _vp, i32p]
     L.strumpack_serial_finter.restype = None
     return L

@@ -7,30 +7,37 @@
 
 namespace {
 
+// Stream-ordered scratch: allocation and release are ordered on the caller's
+// stream, so other streams' work is not waited for.
 struct Buf {
     void* p = nullptr;
-    ~Buf() { if (p) (void)hipFree(p); }
-    int alloc(size_t n) { return hipMalloc(&p, n ? n : 1) == hipSuccess ? WT_OK : WT_ERR_NOMEM; }
+    hipStream_t s;
+    explicit Buf(hipStream_t st) : s(st) {}
+    ~Buf() { if (p) (void)hipFreeAsync(p, s); }
+    int alloc(size_t n) { return hipMallocAsync(&p, n ? n : 1, s) == hipSuccess ? WT_OK : WT_ERR_NOMEM; }
 };
 
 template <typename RS>
 int haskgrn_t(int nl, const void* const* layer, int ns, const void* ain, const int32_t* is_s, double omega, int nz,
-              double dz, double zorig, void* d_u, void* d_w, double* d_pic, int32_t* status)
+              double dz, double zorig, void* d_u, void* d_w, double* d_pic, int32_t* status, hipStream_t st)
 {
     std::vector<RS> lay((size_t)6 * nl);
     for (int k = 0; k < 6; ++k)
         for (int m = 0; m < nl; ++m) lay[(size_t)k * nl + m] = ((const RS*)layer[k])[m];
-    Buf dl, da, ds, dst;
+    Buf dl(st), da(st), ds(st), dst(st);
     if (dl.alloc(lay.size() * sizeof(RS)) || da.alloc(ns * sizeof(RS)) || ds.alloc(ns * 4) || dst.alloc(ns * 4))
         return WT_ERR_NOMEM;
-    WT_HIP(hipMemcpy(dl.p, lay.data(), lay.size() * sizeof(RS), hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(da.p, ain, ns * sizeof(RS), hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(ds.p, is_s, ns * 4, hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpyAsync(dl.p, lay.data(), lay.size() * sizeof(RS), hipMemcpyHostToDevice, st));
+    WT_HIP(hipMemcpyAsync(da.p, ain, ns * sizeof(RS), hipMemcpyHostToDevice, st));
+    WT_HIP(hipMemcpyAsync(ds.p, is_s, ns * 4, hipMemcpyHostToDevice, st));
     int rc = wt_launch_haskgrn<RS>(nl, (const RS*)dl.p, ns, (const RS*)da.p, (const int32_t*)ds.p, (RS)omega, nz,
-                                   (RS)dz, (RS)zorig, (cpx<RS>*)d_u, (cpx<RS>*)d_w, d_pic, (int32_t*)dst.p,
-                                   nullptr);
-    if (rc) return rc;
-    WT_HIP(hipMemcpy(status, dst.p, ns * 4, hipMemcpyDeviceToHost));
+                                   (RS)dz, (RS)zorig, (cpx<RS>*)d_u, (cpx<RS>*)d_w, d_pic, (int32_t*)dst.p, st);
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    WT_HIP(hipMemcpyAsync(status, dst.p, ns * 4, hipMemcpyDeviceToHost, st));
+    WT_HIP(hipStreamSynchronize(st));
     return WT_OK;
 }
 
@@ -40,8 +47,9 @@ extern "C" {
 
 int wt_haskgrn(int prec, int nl, const void* vp, const void* vs, const void* rho, const void* top, const void* vpd,
                const void* vsd, int ns, const void* ain, const int32_t* is_s, double omega, int nz, double dz,
-               double zorig, void* d_u, void* d_w, double* d_pic, int32_t* status)
+               double zorig, void* d_u, void* d_w, double* d_pic, int32_t* status, void* stream)
 {
+    const hipStream_t st = (hipStream_t)stream;
     if (!vp || !vs || !rho || !top || !vpd || !vsd || !ain || !is_s || !d_u || !d_w || !d_pic || !status)
         return WT_ERR_ARG;
     if (nl < 1 || nl > WT_HASK_MAXL || ns < 1 || nz < 1) return WT_ERR_ARG;
@@ -50,35 +58,46 @@ int wt_haskgrn(int prec, int nl, const void* vp, const void* vs, const void* rho
     if (nl < 2 && nz > 1) return WT_ERR_ARG;
     const void* layer[6] = {vp, vs, rho, top, vpd, vsd};
     if (prec == WT_PREC_F32)
-        return haskgrn_t<float>(nl, layer, ns, ain, is_s, omega, nz, dz, zorig, d_u, d_w, d_pic, status);
+        return haskgrn_t<float>(nl, layer, ns, ain, is_s, omega, nz, dz, zorig, d_u, d_w, d_pic, status, st);
     if (prec == WT_PREC_F64)
-        return haskgrn_t<double>(nl, layer, ns, ain, is_s, omega, nz, dz, zorig, d_u, d_w, d_pic, status);
+        return haskgrn_t<double>(nl, layer, ns, ain, is_s, omega, nz, dz, zorig, d_u, d_w, d_pic, status, st);
     return WT_ERR_ARG;
 }
 
 int wt_planewave_layered(int nx, int ny, int nz, double dx, int prec, int nrhs, const void* d_u, const void* d_w,
-                         const double* d_kh, const double* phi, void* d_x)
+                         const double* d_kh, const double* phi, void* d_x, void* stream)
 {
+    const hipStream_t st = (hipStream_t)stream;
     if (nx < 1 || ny < 1 || nz < 1 || nrhs < 1 || !d_u || !d_w || !d_kh || !phi || !d_x) return WT_ERR_ARG;
     std::vector<double> cs((size_t)2 * nrhs);
     for (int j = 0; j < nrhs; ++j) {
         cs[2 * j] = std::cos(phi[j]);
         cs[2 * j + 1] = std::sin(phi[j]);
     }
-    Buf dcs;
+    Buf dcs(st);
     if (dcs.alloc(cs.size() * 8)) return WT_ERR_NOMEM;
-    WT_HIP(hipMemcpy(dcs.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpyAsync(dcs.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, st));
     int rc;
     if (prec == WT_PREC_F32)
         rc = wt_launch_expand<float>((cpx<double>*)d_x, nx, ny, nz, dx, nrhs, (const cpx<float>*)d_u,
-                                     (const cpx<float>*)d_w, d_kh, (const double*)dcs.p, nullptr);
+                                     (const cpx<float>*)d_w, d_kh, (const double*)dcs.p, st);
     else if (prec == WT_PREC_F64)
         rc = wt_launch_expand<double>((cpx<double>*)d_x, nx, ny, nz, dx, nrhs, (const cpx<double>*)d_u,
-                                      (const cpx<double>*)d_w, d_kh, (const double*)dcs.p, nullptr);
+                                      (const cpx<double>*)d_w, d_kh, (const double*)dcs.p, st);
     else
-        return WT_ERR_ARG;
+        rc = WT_ERR_ARG;
+    // the host copy of the angles must outlive the upload
+    WT_HIP(hipStreamSynchronize(st));
+    return rc;
+}
+
+int wt_total_field(void* d_uh, const void* d_ubg, int64_t count, void* stream)
+{
+    if (!d_uh || !d_ubg || count < 0) return WT_ERR_ARG;
+    const hipStream_t st = (hipStream_t)stream;
+    int rc = wt_launch_add((cpx<double>*)d_uh, (const cpx<double>*)d_ubg, count, st);
     if (rc) return rc;
-    WT_HIP(hipDeviceSynchronize());
+    WT_HIP(hipStreamSynchronize(st));
     return WT_OK;
 }
 

@@ -34,6 +34,13 @@ int dalloc(T** p, int64_t count)
     return WT_OK;
 }
 
+template <typename T>
+int salloc(T** p, int64_t count, hipStream_t st)
+{
+    return hipMallocAsync((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T), st) == hipSuccess ? WT_OK
+                                                                                             : WT_ERR_NOMEM;
+}
+
 int ensure_ir(wt_solver_impl& s, int nrhs)
 {
     if (s.ir_cap >= nrhs) return WT_OK;
@@ -326,6 +333,13 @@ int wt_solver_planewaves(wt_solver* h, int nrhs, const double* dirs, double kmag
     return rc;
 }
 
+int wt_solver_stream(const wt_solver* h, void** stream)
+{
+    if (!h || !stream) return WT_ERR_ARG;
+    *stream = (void*)h->s.stream;
+    return WT_OK;
+}
+
 int wt_solver_scattered_source(wt_solver* h, const void* lam_bg, const void* mu_bg, const double* rho_bg,
                                int on_device, int nrhs, const void* d_ubg, const double* srcfn, void* d_sfld)
 {
@@ -336,15 +350,16 @@ int wt_solver_scattered_source(wt_solver* h, const void* lam_bg, const void* mu_
     cpx<double>*lb = nullptr, *mb = nullptr, *ab = nullptr, *sf = nullptr;
     double* rb = nullptr;
     int rc = WT_OK;
+    // scratch is stream-ordered: released after the work queued on the
+    // handle's stream, without waiting for other streams
     auto cleanup = [&] {
-        if (lb) (void)hipFree(lb);
-        if (mb) (void)hipFree(mb);
-        if (rb) (void)hipFree(rb);
-        if (ab) (void)hipFree(ab);
-        if (sf) (void)hipFree(sf);
+        for (void* p : {(void*)lb, (void*)mb, (void*)rb, (void*)ab, (void*)sf})
+            if (p) (void)hipFreeAsync(p, s.stream);
+        (void)hipStreamSynchronize(s.stream);
     };
-    if ((rc = dalloc(&lb, nodes)) || (rc = dalloc(&mb, nodes)) || (rc = dalloc(&rb, nodes)) ||
-        (rc = dalloc(&ab, s.nnz)) || (srcfn && (rc = dalloc(&sf, nrhs)))) {
+    if ((rc = salloc(&lb, nodes, s.stream)) || (rc = salloc(&mb, nodes, s.stream)) ||
+        (rc = salloc(&rb, nodes, s.stream)) || (rc = salloc(&ab, s.nnz, s.stream)) ||
+        (srcfn && (rc = salloc(&sf, nrhs, s.stream)))) {
         cleanup();
         return rc;
     }

@@ -18,3 +18,5 @@ int wt_launch_expand(cpx<double>* d_x, int nx, int ny, int nz, double dx, int nr
 int wt_launch_scatsrc(const int32_t* rowptr, const int32_t* colidx, const cpx<double>* a, const cpx<double>* abg,
                       const cpx<double>* u, const cpx<double>* srcfn, cpx<double>* out, int64_t n, int nrhs,
                       hipStream_t s);
+
+int wt_launch_add(cpx<double>* y, const cpx<double>* x, int64_t n, hipStream_t s);

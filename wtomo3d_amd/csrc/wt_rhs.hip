@@ -267,6 +267,14 @@ __global__ __launch_bounds__(256) void scatsrc_kernel(const int32_t* __restrict_
     out[row * nrhs + c] = cpx<double>(f.re * ar - f.im * ai, f.re * ai + f.im * ar);
 }
 
+// u_h += u_bg (makeuh.f:41-59 with totfld)
+__global__ __launch_bounds__(256) void add_kernel(cpx<double>* __restrict__ y, const cpx<double>* __restrict__ x,
+                                                  int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = cpx<double>(y[i].re + x[i].re, y[i].im + x[i].im);
+}
+
 unsigned grid_for(int64_t n)
 {
     const int64_t b = (n + 255) / 256;
@@ -306,6 +314,14 @@ int wt_launch_scatsrc(const int32_t* rowptr, const int32_t* colidx, const cpx<do
     const int64_t blocks = (n + SS_ROWS - 1) / SS_ROWS;
     hipLaunchKernelGGL(scatsrc_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rowptr, colidx, a, abg, u, srcfn, out,
                        n, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_launch_add(cpx<double>* y, const cpx<double>* x, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL(add_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, x, n);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }

@@ -72,37 +72,48 @@ class Profiles:
                 self.kh.to_host(np.float64, (self.ns,)))
 
 
-def profiles(layers, ain, is_s, omega, nz, dz, zorig=0.0, prec=WT_PREC_F64):
+def profiles(layers, ain, is_s, omega, nz, dz, zorig=0.0, prec=WT_PREC_F64, stream=None, out=None):
     """HASKGRN for every source on the device.
 
     layers: dict of per-layer sequences ``vp, vs, rho, top, vpd, vsd`` (the
     last layer the half-space); ain: incidence angles (radians); is_s: 1 for
-    an incident S wave, 0 for P.  Raises on a reference error (a depth above
-    the first layer top)."""
+    an incident S wave, 0 for P.  ``stream``: the HIP stream to run on (a
+    solver's, ``Solver.stream()``); ``out``: (U, W, kh) DeviceBuffers to
+    reuse.  Raises on a reference error (a depth above the first layer top)."""
     ft = np.float32 if prec == WT_PREC_F32 else np.float64
     ct = np.complex64 if prec == WT_PREC_F32 else np.complex128
     arrs = [np.ascontiguousarray(layers[k], ft) for k in ("vp", "vs", "rho", "top", "vpd", "vsd")]
     ain = np.ascontiguousarray(np.atleast_1d(ain), ft)
     ns = ain.size
     iss = np.ascontiguousarray(np.broadcast_to(np.asarray(is_s, np.int32), (ns,)))
-    U = DeviceBuffer(ns * nz * np.dtype(ct).itemsize)
-    W = DeviceBuffer(ns * nz * np.dtype(ct).itemsize)
-    kh = DeviceBuffer(ns * 8)
+    nb = ns * nz * np.dtype(ct).itemsize
+    if out is not None and out[0].nbytes >= nb and out[1].nbytes >= nb and out[2].nbytes >= ns * 8:
+        U, W, kh = out
+    else:
+        U, W, kh = DeviceBuffer(nb), DeviceBuffer(nb), DeviceBuffer(ns * 8)
     status = np.zeros(ns, np.int32)
     check(lib().wt_haskgrn(prec, arrs[0].size, *[a.ctypes.data_as(_vp) for a in arrs], ns,
                            ain.ctypes.data_as(_vp), iss.ctypes.data_as(_vp), float(omega), nz, float(dz),
-                           float(zorig), U.ptr, W.ptr, kh.ptr, status.ctypes.data_as(_vp)), "wt_haskgrn")
+                           float(zorig), U.ptr, W.ptr, kh.ptr, status.ctypes.data_as(_vp), stream), "wt_haskgrn")
     if status.any():
         raise RuntimeError("haskgrn: depth above the first layer top for sources %s" % np.nonzero(status)[0])
     return Profiles(U, W, kh, ns, nz, prec, status)
 
 
-def background_fields(nx, ny, nz, dx, prof, phi):
+def background_fields(nx, ny, nz, dx, prof, phi, stream=None, out=None):
     """3-D background fields of the profiled plane waves: a device block,
-    row-major (n, ns) complex128 in the unknown order."""
+    row-major (n, ns) complex128 in the unknown order (into ``out``, a
+    DeviceBuffer, when given)."""
     assert prof.nz == nz
     phi = np.ascontiguousarray(np.broadcast_to(np.asarray(phi, np.float64), (prof.ns,)))
-    X = DeviceBuffer(3 * nx * ny * nz * prof.ns * 16)
+    nb = 3 * nx * ny * nz * prof.ns * 16
+    X = out if out is not None and out.nbytes >= nb else DeviceBuffer(nb)
     check(lib().wt_planewave_layered(nx, ny, nz, float(dx), prof.prec, prof.ns, prof.U.ptr, prof.W.ptr,
-                                     prof.kh.ptr, phi.ctypes.data_as(_vp), X.ptr), "wt_planewave_layered")
+                                     prof.kh.ptr, phi.ctypes.data_as(_vp), X.ptr, stream), "wt_planewave_layered")
     return X
+
+
+def total_field(uh, ubg, count, stream=None):
+    """uh += ubg over ``count`` complex128 values (DeviceBuffers): the total
+    field of MAKEUH (makeuh.f:41-59)."""
+    check(lib().wt_total_field(uh.ptr, ubg.ptr, int(count), stream), "wt_total_field")

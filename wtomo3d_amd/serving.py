@@ -8,6 +8,15 @@ request and are reused by later requests for the same model and frequency;
 idle factors are evicted least-recently-used when memory is needed.  The
 reference has no serving layer (its driver factors one frequency at a time,
 elwave.f90:147-182); this is the request-level wrapper around jobs 2 and 3.
+
+Each request solves the reference's forward problem for plane-wave sources
+on its model: a heterogeneous model (seeded by the model key) over a layered
+background; Thomson-Haskell profiles of the background (HASKGRN) expanded to
+3-D background fields; the scattered-field source (A_bg - A) u_bg
+(HETFLDSRC); the FP32-factor / FP64-refined solve for the scattered field;
+and the total field u_h + u_bg (MAKEUH).  Sources: P (70 %) or S plane
+waves, incidence 10-35 degrees, azimuth 0-360 degrees, from the request's
+seed.
 """
 import collections
 import ctypes
@@ -16,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import media
+from . import media, rhs
 from ._lib import lib, check, WtSolveStats, grid_params
 from .solver import Solver
 
@@ -49,21 +58,40 @@ class Engine:
         self.closed = False
         self.done = []
         self._models = {}
+        self._mlock = threading.Lock()
         self._est = {}
 
     # -- model / memory helpers --------------------------------------------
     def _model(self, key, dims, dx):
+        """(lam, mu, rho) of the heterogeneous model, its layer stack and the
+        (lam, mu, rho) of its layered background."""
         k = (key, dims, dx)
-        if k not in self._models:
-            vp, vs, rho = media.heterogeneous(dims[0], dims[1], dims[2], dx, seed=int(key))
-            lam, mu, rho1 = media.lame(vp, vs, rho)
-            self._models[k] = (lam, mu, rho1, float(np.median(vp)))
-        return self._models[k]
+        with self._mlock:
+            if k not in self._models:
+                nx, ny, nz = dims
+                vp, vs, rho = media.heterogeneous(nx, ny, nz, dx, seed=int(key))
+                vp1, vs1, rho1 = media.layered(nx, ny, nz, dx, seed=int(key))
+                bg = media.lame(*(np.broadcast_to(a[None, None, :], (ny, nx, nz)) for a in (vp1, vs1, rho1)))
+                self._models[k] = (media.lame(vp, vs, rho), media.layer_stack(nx, ny, nz, dx, seed=int(key)), bg)
+            return self._models[k]
 
     def _estimate(self, dims):
+        """Solver bytes plus the scattered-source scratch (the background
+        operator and model) and the request's background and source blocks."""
         if dims not in self._est:
-            self._est[dims] = int(lib().wt_estimate_bytes(dims[0], dims[1], dims[2], 64, 32, 64))
+            nx, ny, nz = dims
+            n, nnz = 3 * nx * ny * nz, int(lib().wt_graph_nnz(nx, ny, nz))
+            self._est[dims] = int(lib().wt_estimate_bytes(nx, ny, nz, 64, 32, 64)) + nnz * 16 + n * 40 + 2 * n * 64 * 16
         return self._est[dims]
+
+    @staticmethod
+    def sources(r):
+        """Plane-wave sources of a request: incidence (rad), azimuth (rad), S flags."""
+        rng = np.random.default_rng(r.src_seed)
+        ain = rng.uniform(np.deg2rad(10.0), np.deg2rad(35.0), r.nrhs)
+        phi = rng.uniform(0.0, 2 * np.pi, r.nrhs)
+        is_s = (rng.random(r.nrhs) < 0.3).astype(np.int32)
+        return ain, phi, is_s
 
     # -- scheduling ----------------------------------------------------------
     def submit(self, req):
@@ -119,8 +147,8 @@ class Engine:
                     return
             r.start = time.perf_counter()
             try:
+                (lam, mu, rho), layers, bg = self._model(r.model_key, r.dims, r.dx)
                 if e is None:
-                    lam, mu, rho, _ = self._model(r.model_key, r.dims, r.dx)
                     s = Solver(*r.dims)
                     s.set_model(lam, mu, rho)
                     s.assemble(grid_params(r.dims[0], r.dims[1], r.dims[2], r.dx, 2 * np.pi * r.freq,
@@ -132,21 +160,21 @@ class Engine:
                     r.reused = True
                 s = e.solver
                 n = s.n
-                vref = self._model(r.model_key, r.dims, r.dx)[3]
-                bufs = slot_bufs(3 * n * r.nrhs * 16)
-                dirs = np.ascontiguousarray(media.plane_wave_dirs(r.nrhs, r.src_seed), np.float64)
+                st_ = s.stream()
+                x, sf, ub, pu, pw, pk = slot_bufs(n * r.nrhs * 16, r.dims[2] * r.nrhs * 16, r.nrhs * 8)
                 omega = 2 * np.pi * r.freq
-                check(L.wt_solver_planewaves(s._h, r.nrhs, dirs.ctypes.data_as(ctypes.c_void_p), omega / vref,
-                                             r.dx, bufs[0]), "planewaves")
-                check(L.wt_solver_apply(s._h, r.nrhs, bufs[0], bufs[1], 1), "apply")
+                ain, phi, is_s = self.sources(r)
+                prof = rhs.profiles(layers, ain, is_s, omega, r.dims[2], r.dx, 0.0, rhs.WT_PREC_F64, stream=st_,
+                                    out=(pu, pw, pk))
+                rhs.background_fields(r.dims[0], r.dims[1], r.dims[2], r.dx, prof, phi, stream=st_, out=ub)
+                s.scattered_source(bg[0], bg[1], bg[2], ub, out=sf, nrhs=r.nrhs)
                 st = WtSolveStats()
-                check(L.wt_solver_solve(s._h, 32, r.nrhs, bufs[1], bufs[2], 1, self.tol, self.max_iter,
+                check(L.wt_solver_solve(s._h, 32, r.nrhs, sf.ptr, x.ptr, 1, self.tol, self.max_iter,
                                         ctypes.byref(st)), "solve")
+                rhs.total_field(x, ub, n * r.nrhs, stream=st_)
                 r.stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
                 if self.return_x:
-                    x = np.zeros((n, r.nrhs), np.complex128)
-                    check(L.wt_d2h(x.ctypes.data_as(ctypes.c_void_p), bufs[2], x.nbytes), "d2h")
-                    r.x = x
+                    r.x = x.to_host(np.complex128, (n, r.nrhs))
             except Exception as ex:  # reported per request
                 r.error = str(ex)
             r.end = time.perf_counter()
@@ -172,19 +200,19 @@ class Engine:
         bufs = []
 
         def make_slot():
-            cap = [0, [ctypes.c_void_p() for _ in range(3)]]
+            # per-slot device buffers that only grow: solution, source and
+            # background blocks, then the two profiles and the wavenumbers
+            held = [None] * 6
 
-            def get(nbytes):
-                if nbytes > cap[0]:
-                    for p in cap[1]:
-                        if p.value:
-                            L.wt_dev_free(p)
-                    cap[1] = [ctypes.c_void_p() for _ in range(3)]
-                    for p in cap[1]:
-                        check(L.wt_dev_alloc(nbytes // 3, ctypes.byref(p)), "slot alloc")
-                    cap[0] = nbytes
-                return cap[1]
-            bufs.append(cap)
+            def get(block, prof, kh):
+                sizes = (block, block, block, prof, prof, kh)
+                for i, nb in enumerate(sizes):
+                    if held[i] is None or held[i].nbytes < nb:
+                        if held[i] is not None:
+                            held[i].free()
+                        held[i] = rhs.DeviceBuffer(nb)
+                return held
+            bufs.append(held)
             return get
 
         threads = [threading.Thread(target=self._worker, args=(make_slot(),), daemon=True)
@@ -207,10 +235,10 @@ class Engine:
         for t in threads:
             t.join()
         wall = time.perf_counter() - t0
-        for cap in bufs:
-            for p in cap[1]:
-                if p.value:
-                    L.wt_dev_free(p)
+        for held in bufs:
+            for b in held:
+                if b is not None:
+                    b.free()
         return wall
 
     def close(self):

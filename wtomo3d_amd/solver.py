@@ -126,11 +126,19 @@ class Solver:
         stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
         return (X[:, 0] if one else X), stats
 
-    def scattered_source(self, lam_bg, mu_bg, rho_bg, ubg, srcfn=None):
+    def stream(self):
+        """The HIP stream of this handle (a ctypes void pointer)."""
+        st = ctypes.c_void_p()
+        check(lib().wt_solver_stream(self._h, ctypes.byref(st)), "wt_solver_stream")
+        return st
+
+    def scattered_source(self, lam_bg, mu_bg, rho_bg, ubg, srcfn=None, out=None, nrhs=None):
         """HETFLDSRC (hetfldsrc.f90:17-1039): srcfn_j (A_bg - A) u_bg_j with A
         the operator of the last ``assemble`` and A_bg the background model's at
         the same parameters.  ubg: a device block (rhs.DeviceBuffer, row-major
-        (n, nrhs) complex128) or a host array (n, nrhs); returns the same kind."""
+        (n, nrhs) complex128) or a host array (n, nrhs); returns the same kind
+        (into ``out``, a DeviceBuffer, when given).  For a device ubg the
+        column count is ``nrhs`` (default: all the buffer holds)."""
         from .rhs import DeviceBuffer
         host = not isinstance(ubg, DeviceBuffer)
         if host:
@@ -138,7 +146,7 @@ class Solver:
             nrhs = ubg.shape[1]
             ub = DeviceBuffer.from_host(ubg)
         else:
-            nrhs = ubg.nbytes // (self.n * 16)
+            nrhs = int(nrhs) if nrhs is not None else ubg.nbytes // (self.n * 16)
             ub = ubg
         lb = np.ascontiguousarray(lam_bg, np.complex128).ravel()
         mb = np.ascontiguousarray(mu_bg, np.complex128).ravel()
@@ -146,7 +154,8 @@ class Solver:
         sf = None
         if srcfn is not None:
             sf = np.ascontiguousarray(np.broadcast_to(np.asarray(srcfn, np.complex128), (nrhs,)))
-        out = DeviceBuffer(self.n * nrhs * 16)
+        if out is None or out.nbytes < self.n * nrhs * 16:
+            out = DeviceBuffer(self.n * nrhs * 16)
         check(lib().wt_solver_scattered_source(self._h, _p(lb), _p(mb), _p(rb), 0, nrhs, ub.ptr,
                                                None if sf is None else _p(sf), out.ptr),
               "wt_solver_scattered_source")
