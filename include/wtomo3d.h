@@ -117,6 +117,25 @@ int64_t wt_solver_bytes(const wt_solver *h, int factor_prec, int nrhs);
 /* lambda, mu: complex128[nxnynz]; rho: double[nxnynz] (host or device). */
 int wt_solver_set_model(wt_solver *h, const void *lam, const void *mu, const double *rho, int on_device);
 
+/* Complex moduli of an attenuating model at one frequency; replaces
+ * DISPERSION (dispersion.f:1-131; module inputs MODEL_MODULE vpr, vsr, qp,
+ * qs, rho, omega, freqbase):
+ *   c = c_base max(1 + ln(omega_re / (2 pi freqbase)) / (pi Q), 0.1),
+ *   c~ = c (1 - i / (2 Q)),  mu = vs~^2 rho,  lambda = vp~^2 rho - 2 mu,
+ * vp, vs the speeds at freqbase, qpi, qsi = 1/Qp, 1/Qs (the reference's qp,
+ * qs arrays), rho; all double [nodes].  When omega_re or freqbase is not
+ * positive the moduli are those of the base speeds (the reference leaves
+ * them unchanged).  lam, mu: complex128 [nodes].  on_device: every array is
+ * a device pointer; otherwise host arrays.  Runs on the given stream. */
+int wt_dispersion(int64_t nodes, const double *vp, const double *vs, const double *qpi, const double *qsi,
+                  const double *rho, double freqbase, double omega_re, void *lam, void *mu, int on_device,
+                  void *stream);
+
+/* wt_solver_set_model with the moduli of wt_dispersion, formed on the
+ * device from host vp, vs, 1/Qp, 1/Qs, rho. */
+int wt_solver_set_model_q(wt_solver *h, const double *vp, const double *vs, const double *qpi,
+                          const double *qsi, const double *rho, double freqbase, double omega_re);
+
 /* FP64 assembly of the operator at p->omega into the solver's CSR values. */
 int wt_solver_assemble(wt_solver *h, const wt_grid_params *p);
 
@@ -212,6 +231,42 @@ int wt_total_field(void *d_uh, const void *d_ubg, int64_t count, void *stream);
 /* The HIP stream a solver handle runs its work on (for callers that put
  * their own device work in order with it). */
 int wt_solver_stream(const wt_solver *h, void **stream);
+
+/* ---- receivers ---- */
+
+/* Receiver extraction compiled into a sparse matrix; replaces the receiver
+ * and hydrophone loops of MAKEUH (makeuh.f:60-168) with SRCSETUP
+ * (srcsetup.f:1-214, extraction branch) and SRCREG (srcreg.f), whose
+ * inputs are module globals (RECPRM_MODULE xg, yg, zg, gspread, igreg,
+ * receiver, recin; xr, yr, zr, rspread, irreg; MODEL_MODULE da, mu, omega,
+ * dx, freesurf).
+ *   ng receivers at (xg, yg, zg) [m from the first node], gspread, igreg
+ *   (0: nearest node; 1-4: a (2 igreg + 1)^3 Kaiser-windowed sinc region
+ *   for gspread <= 0.5, else an (INT(4 gspread) + 1)^3 discrete Dirac
+ *   region, INT(4 gspread) + 1 <= 9), gfac: complex response per receiver
+ *   (2 ng doubles) or NULL;
+ *   nr hydrophones at (xr, yr, zr) with rspread, irreg: pressure
+ *   (lambda + 2 mu / 3) div u at the nearest node, lam, mu complex128
+ *   [nxnynz] (host), omega the angular frequency;
+ *   freesurf[6] as MODEL_MODULE freesurf (top, +x, bottom, -x, +y, -y):
+ *   taps beyond a free surface are mirrored with a change of sign, others
+ *   dropped.
+ * WT_ERR_ARG for a station whose taps would leave the grid. */
+typedef struct wt_receivers wt_receivers;
+int wt_receivers_create(int nx, int ny, int nz, double dx, const int32_t *freesurf, int ng, const double *xg,
+                        const double *yg, const double *zg, double gspread, int igreg, const double *gfac,
+                        int nr, const double *xr, const double *yr, const double *zr, double rspread,
+                        int irreg, const void *lam, const void *mu, double omega_re, double omega_im,
+                        wt_receivers **out);
+
+/* Output rows: 3 ng (u, v, w per receiver: uest, vest, west) + nr (utest). */
+int wt_receivers_rows(const wt_receivers *r);
+
+/* d_out (rows x nrhs, row-major complex128, device) = E d_field, d_field a
+ * solution block (n x nrhs row-major complex128, device; the total field
+ * for MAKEUH's totfld).  Runs on the given stream and returns when done. */
+int wt_receivers_extract(const wt_receivers *r, int nrhs, const void *d_field, void *d_out, void *stream);
+int wt_receivers_destroy(wt_receivers *r);
 
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
  * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and

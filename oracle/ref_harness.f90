@@ -12,6 +12,8 @@
 !   elwave.f90:171         ASMBLE
 !   haskgrn.f:31           HASKGRN (plane-wave depth profiles)
 !   hetfldsrc.f90:17       HETFLDSRC (scattered-field source)
+!   makeuh.f:1             MAKEUH (total field, receivers, hydrophone pressure)
+!   dispersion.f:1         DISPERSION (attenuation: complex moduli)
 ! Only the oracle tests load the resulting library.
       SUBROUTINE wtref_graph_size(nx, ny, nz, numvert, nzero) &
                  BIND(C, NAME='wtref_graph_size')
@@ -199,4 +201,136 @@
       ug = ugrn1f
       wg = wgrn1f
       DEALLOCATE(vp1d, vs1d, rh1d, h1d, vp1disc, vs1disc, ain, srctyp, ugrn1f, wgrn1f)
+      END SUBROUTINE
+
+! MAKEUH (makeuh.f:1-172) for one source at one frequency: the solution
+! sfld (scattered field, unknown order), the background fields (nz,nx,ny),
+! ng receivers (displacements, optional response factors rfac when recin
+! is set) and nr hydrophones (pressure, with the model's lambda and mu from
+! vp, vs, rh at the hydrophone nodes); every source-receiver pair active.
+      SUBROUTINE wtref_makeuh(nxi, nyi, nzi, dxi, freqi, fst, vp, vs, rh, totf, &
+                              sfld, ub, vb, wb, ngi, xgi, ygi, zgi, gsp, igr, &
+                              reci, rfac, nri, xri, yri, zri, rsp, irr, &
+                              ue, ve, we, pe, ierr) &
+                 BIND(C, NAME='wtref_makeuh')
+      USE ISO_C_BINDING
+      USE MODEL_MODULE, ONLY : da, mu, rho, dap, mup, rhop, nx, ny, nz, iom
+      USE INIT_MODULE, ONLY : totfld
+      USE SRCPRM_MODULE, ONLY : srctyp
+      USE WAVEFIELD_MODULE, ONLY : utbkr, vtbkr, wtbkr, ufac, vfac, wfac
+      USE RECPRM_MODULE, ONLY : xg, yg, zg, xr, yr, zr, sg, sr, receiver, &
+                                uest, vest, west, utest, igreg, irreg, ng, nr, &
+                                recin, gspread, rspread
+      USE INTERFACE_MODULE, ONLY : makeuh
+      IMPLICIT NONE
+      INTEGER(C_INT), INTENT(IN) :: nxi, nyi, nzi, fst, totf, ngi, igr, reci, nri, irr
+      REAL, INTENT(IN) :: dxi, freqi, gsp, rsp
+      REAL, INTENT(IN) :: vp(nzi,nxi,nyi), vs(nzi,nxi,nyi), rh(nzi,nxi,nyi)
+      COMPLEX, INTENT(IN) :: sfld(3*nzi*nxi*nyi)
+      COMPLEX, INTENT(IN) :: ub(nzi,nxi,nyi), vb(nzi,nxi,nyi), wb(nzi,nxi,nyi)
+      REAL, INTENT(IN) :: xgi(*), ygi(*), zgi(*), xri(*), yri(*), zri(*)
+      COMPLEX, INTENT(IN) :: rfac(*)
+      COMPLEX, INTENT(OUT) :: ue(*), ve(*), we(*), pe(*)
+      INTEGER(C_INT), INTENT(OUT) :: ierr
+      INTEGER ierr1
+      COMPLEX, ALLOCATABLE :: sf(:)
+      CALL wtref_model(nxi, nyi, nzi, dxi, freqi, 999.999, fst, 1, 10.0*dxi, 1.0e-3, &
+                       vp, vs, rh, vp, vs, rh)
+      iom = 1
+      totfld = (totf /= 0)
+      IF (ALLOCATED(srctyp)) DEALLOCATE(srctyp)
+      ALLOCATE(srctyp(1))
+      srctyp(1) = 'pp'
+      IF (ALLOCATED(utbkr)) DEALLOCATE(utbkr, vtbkr, wtbkr)
+      ALLOCATE(utbkr(nz,nx,ny), vtbkr(nz,nx,ny), wtbkr(nz,nx,ny))
+      utbkr = ub
+      vtbkr = vb
+      wtbkr = wb
+      IF (ALLOCATED(ufac)) DEALLOCATE(ufac, vfac, wfac)
+      ALLOCATE(ufac(nz,nx,ny), vfac(nz,nx,ny), wfac(nz,nx,ny))
+      ng = ngi
+      nr = nri
+      igreg = igr
+      irreg = irr
+      gspread = gsp
+      rspread = rsp
+      recin = reci
+      IF (ALLOCATED(xg)) DEALLOCATE(xg, yg, zg, sg, receiver, uest, vest, west)
+      ALLOCATE(xg(MAX(ng,1)), yg(MAX(ng,1)), zg(MAX(ng,1)), sg(1,MAX(ng,1)))
+      ALLOCATE(receiver(MAX(ng,1),1), uest(1,MAX(ng,1),1), vest(1,MAX(ng,1),1), west(1,MAX(ng,1),1))
+      IF (ng > 0) THEN
+         xg(1:ng) = xgi(1:ng)
+         yg(1:ng) = ygi(1:ng)
+         zg(1:ng) = zgi(1:ng)
+         receiver(1:ng,1) = rfac(1:ng)
+      ENDIF
+      sg = 1
+      IF (ALLOCATED(xr)) DEALLOCATE(xr, yr, zr, sr, utest)
+      ALLOCATE(xr(MAX(nr,1)), yr(MAX(nr,1)), zr(MAX(nr,1)), sr(1,MAX(nr,1)), utest(1,MAX(nr,1),1))
+      IF (nr > 0) THEN
+         xr(1:nr) = xri(1:nr)
+         yr(1:nr) = yri(1:nr)
+         zr(1:nr) = zri(1:nr)
+      ENDIF
+      sr = 1
+      ALLOCATE(sf(3*nz*nx*ny))
+      sf = sfld
+      CALL MAKEUH(1, sf, ierr1)
+      DEALLOCATE(sf)
+      ierr = ierr1
+      IF (ng > 0) THEN
+         ue(1:ng) = uest(1,1:ng,1)
+         ve(1:ng) = vest(1,1:ng,1)
+         we(1:ng) = west(1,1:ng,1)
+      ENDIF
+      IF (nr > 0) pe(1:nr) = utest(1,1:nr,1)
+      DEALLOCATE(utbkr, vtbkr, wtbkr, ufac, vfac, wfac, srctyp)
+      DEALLOCATE(xg, yg, zg, sg, receiver, uest, vest, west, xr, yr, zr, sr, utest)
+      DEALLOCATE(da, mu, rho, dap, mup, rhop)
+      END SUBROUTINE
+
+! DISPERSION (dispersion.f:1-131) on a grid of speeds vp, vs at the base
+! frequency, inverse quality factors qpi, qsi (the reference's qp, qs) and
+! densities; da and mu start from the lossless moduli as modelio.f90:78-92
+! leaves them.
+      SUBROUTINE wtref_dispersion(nxi, nyi, nzi, vp, vs, qpi, qsi, rh, fbase, omr, lam, mu1) &
+                 BIND(C, NAME='wtref_dispersion')
+      USE ISO_C_BINDING
+      USE MODEL_MODULE, ONLY : da, mu, rho, vpr, vsr, qp, qs, omega, freqbase, nx, ny, nz
+      USE INIT_MODULE, ONLY : ncom
+      IMPLICIT NONE
+      INTEGER(C_INT), INTENT(IN) :: nxi, nyi, nzi
+      REAL, INTENT(IN) :: vp(nzi,nxi,nyi), vs(nzi,nxi,nyi), qpi(nzi,nxi,nyi), qsi(nzi,nxi,nyi), rh(nzi,nxi,nyi)
+      REAL, INTENT(IN) :: fbase, omr
+      COMPLEX, INTENT(OUT) :: lam(nzi,nxi,nyi), mu1(nzi,nxi,nyi)
+      COMPLEX vc
+      INTEGER ix, iy, iz
+      nx = nxi
+      ny = nyi
+      nz = nzi
+      ncom = 0
+      IF (ALLOCATED(da)) DEALLOCATE(da, mu, rho)
+      IF (ALLOCATED(vpr)) DEALLOCATE(vpr, vsr, qp, qs)
+      ALLOCATE(da(nz,nx,ny), mu(nz,nx,ny), rho(nz,nx,ny), vpr(nz,nx,ny), vsr(nz,nx,ny), qp(nz,nx,ny), qs(nz,nx,ny))
+      vpr = vp
+      vsr = vs
+      qp = qpi
+      qs = qsi
+      rho = rh
+      DO iy=1,ny
+         DO ix=1,nx
+            DO iz=1,nz
+               vc = CMPLX(vs(iz,ix,iy), 0.0)
+               mu(iz,ix,iy) = vc*vc*rho(iz,ix,iy)
+               vc = CMPLX(vp(iz,ix,iy), 0.0)
+               da(iz,ix,iy) = vc*vc*rho(iz,ix,iy) - 2.0*mu(iz,ix,iy)
+            ENDDO
+         ENDDO
+      ENDDO
+      omega = CMPLX(omr, 0.0)
+      freqbase = fbase
+      CALL DISPERSION()
+      lam = da
+      mu1 = mu
+      DEALLOCATE(da, mu, rho, vpr, vsr, qp, qs)
       END SUBROUTINE

@@ -10,6 +10,10 @@ hetfldsrc.f90 with the assembly sources, compiled by oracle/Makefile).
   numbering of hetfldsrc.f90:1020 is the unknown order) with a perturbed
   model, its layered background, background fields and srcfn, and the
   reference's scattered-field source in both precisions.
+- makeuh_g121110.npz: a 12x11x10 grid with a scattered field and a
+  background field, receivers (nearest node, Kaiser sinc, Dirac spread; at
+  and just below the free surface so that taps are mirrored) and
+  hydrophones, and the reference's MAKEUH outputs in both precisions.
 
     python tests/golden/make_golden_rhs.py
 """
@@ -90,6 +94,38 @@ def main():
         assert ierr == 0
         res["sfld%d" % prec] = sf
     np.savez_compressed(os.path.join(HERE, "hetfldsrc_g877.npz"), **res)
+
+    # receivers and hydrophones on a 12 x 11 x 10 grid
+    nx, ny, nz, dx, freq = 12, 11, 10, 100.0, 0.4
+    rng = np.random.default_rng(31)
+    vp = 4000.0 + 500.0 * rng.random((ny, nx, nz))
+    vs = vp / 1.8
+    rho = 2500.0 + 100.0 * rng.random((ny, nx, nz))
+    n = 3 * nx * ny * nz
+    sfld = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ub, vb, wb = [rng.standard_normal((ny, nx, nz)) + 1j * rng.standard_normal((ny, nx, nz)) for _ in range(3)]
+    res = dict(vp=vp, vs=vs, rho=rho, sfld=sfld, ub=ub, vb=vb, wb=wb, params=np.array([nx, ny, nz, dx, freq]))
+    sets = [("nearest", 0, 0.5), ("sinc2", 2, 0.5), ("sinc4", 4, 0.5), ("dirac1", 1, 1.0), ("dirac13", 2, 1.3)]
+    for tag, ireg, spread in sets:
+        xs = rng.uniform(0.0, (nx - 1) * dx, 6)
+        ys = rng.uniform(0.0, (ny - 1) * dx, 6)
+        zs = np.concatenate([[0.0, 30.0], rng.uniform(0.0, (nz - 1) * dx, 4)])
+        fac = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+        hx = rng.uniform(dx, (nx - 2) * dx, 3)
+        hy = rng.uniform(dx, (ny - 2) * dx, 3)
+        hz = np.array([0.0, 150.0, 420.0])
+        rec = dict(x=xs, y=ys, z=zs, spread=spread, ireg=ireg, fac=fac)
+        hyd = dict(x=hx, y=hy, z=hz, spread=spread, ireg=ireg)
+        res[tag + "__stations"] = np.stack([xs, ys, zs])
+        res[tag + "__fac"] = fac
+        res[tag + "__hyd"] = np.stack([hx, hy, hz])
+        res[tag + "__set"] = np.array([ireg, spread])
+        for prec in (32, 64):
+            ierr, ue, ve, we, pe = O.ref_makeuh(nx, ny, nz, dx, freq, vp, vs, rho, sfld, ub, vb, wb, rec, hyd, prec=prec)
+            assert ierr == 0
+            res["%s__out%d" % (tag, prec)] = np.concatenate([ue, ve, we, pe])
+    res["sets"] = np.array([t for t, _, _ in sets])
+    np.savez_compressed(os.path.join(HERE, "makeuh_g121110.npz"), **res)
 
 
 if __name__ == "__main__":

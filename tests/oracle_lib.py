@@ -201,3 +201,47 @@ def ref_hetfldsrc(nx, ny, nz, dx, freq, vp, vs, rho, vpb, vsb, rhob, ub, vb, wb,
                       F(pmlr), *[_ptr(a) for a in reals], *[_ptr(a) for a in flds], F(complex(srcfn).real),
                       F(complex(srcfn).imag), _ptr(sf), ctypes.byref(ierr))
     return ierr.value, sf
+
+
+def ref_makeuh(nx, ny, nz, dx, freq, vp, vs, rho, sfld, ub, vb, wb, rec, hyd, fst=True, totfld=True, prec=64):
+    """Reference MAKEUH (makeuh.f:1) through oracle/_ref -> (ierr, uest, vest, west, utest).
+
+    rec: dict x, y, z, spread, ireg, fac (None: recin = 0); hyd: dict x, y, z, spread, ireg."""
+    L = ref(prec)
+    R = ctypes.c_float if prec == 32 else ctypes.c_double
+    ft = np.float32 if prec == 32 else np.float64
+    ct = np.complex64 if prec == 32 else np.complex128
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    F = lambda v: ctypes.byref(R(v))
+    A = lambda v, t: np.ascontiguousarray(v, t)
+    ng, nr = len(rec["x"]), len(hyd["x"])
+    xg, yg, zg = (A(rec[k], ft) for k in ("x", "y", "z"))
+    xr, yr, zr = (A(hyd[k], ft) for k in ("x", "y", "z"))
+    fac = A(rec["fac"] if rec.get("fac") is not None else np.ones(max(ng, 1)), ct)
+    reals = [A(a, ft) for a in (vp, vs, rho)]
+    flds = [A(a, ct) for a in (sfld, ub, vb, wb)]
+    ue, ve, we = (np.zeros(max(ng, 1), ct) for _ in range(3))
+    pe = np.zeros(max(nr, 1), ct)
+    ierr = ctypes.c_int(-1)
+    L.wtref_makeuh(I(nx), I(ny), I(nz), F(dx), F(freq), I(int(fst)), *[_ptr(a) for a in reals], I(int(totfld)),
+                   *[_ptr(a) for a in flds], I(ng), _ptr(xg), _ptr(yg), _ptr(zg), F(rec["spread"]),
+                   I(rec["ireg"]), I(0 if rec.get("fac") is None else 1), _ptr(fac), I(nr), _ptr(xr), _ptr(yr),
+                   _ptr(zr), F(hyd["spread"]), I(hyd["ireg"]), _ptr(ue), _ptr(ve), _ptr(we), _ptr(pe),
+                   ctypes.byref(ierr))
+    return ierr.value, ue[:ng], ve[:ng], we[:ng], pe[:nr]
+
+
+def ref_dispersion(vp, vs, qpi, qsi, rho, freqbase, omega_re, prec=64):
+    """Reference DISPERSION (dispersion.f:1) -> (lambda, mu); arrays (ny, nx, nz)."""
+    L = ref(prec)
+    R = ctypes.c_float if prec == 32 else ctypes.c_double
+    ft = np.float32 if prec == 32 else np.float64
+    ct = np.complex64 if prec == 32 else np.complex128
+    ny, nx, nz = np.shape(vp)
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    F = lambda v: ctypes.byref(R(v))
+    arrs = [np.ascontiguousarray(a, ft) for a in (vp, vs, qpi, qsi, rho)]
+    lam = np.zeros((ny, nx, nz), ct)
+    mu = np.zeros((ny, nx, nz), ct)
+    L.wtref_dispersion(I(nx), I(ny), I(nz), *[_ptr(a) for a in arrs], F(freqbase), F(omega_re), _ptr(lam), _ptr(mu))
+    return lam, mu

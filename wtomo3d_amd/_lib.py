@@ -97,6 +97,20 @@ def _declare(L):
     L.wt_total_field.restype = ctypes.c_int
     L.wt_solver_stream.argtypes = [_vp, ctypes.POINTER(ctypes.c_void_p)]
     L.wt_solver_stream.restype = ctypes.c_int
+    f64 = ctypes.c_double
+    L.wt_receivers_create.argtypes = [i32] * 3 + [f64, _vp, i32, _vp, _vp, _vp, f64, i32, _vp, i32, _vp, _vp, _vp,
+                                                  f64, i32, _vp, _vp, f64, f64, ctypes.POINTER(ctypes.c_void_p)]
+    L.wt_receivers_create.restype = ctypes.c_int
+    L.wt_receivers_rows.argtypes = [_vp]
+    L.wt_receivers_rows.restype = ctypes.c_int
+    L.wt_receivers_extract.argtypes = [_vp, i32, _vp, _vp, _vp]
+    L.wt_receivers_extract.restype = ctypes.c_int
+    L.wt_receivers_destroy.argtypes = [_vp]
+    L.wt_receivers_destroy.restype = ctypes.c_int
+    L.wt_dispersion.argtypes = [i64] + [_vp] * 5 + [f64, f64, _vp, _vp, i32, _vp]
+    L.wt_dispersion.restype = ctypes.c_int
+    L.wt_solver_set_model_q.argtypes = [_vp] + [_vp] * 5 + [f64, f64]
+    L.wt_solver_set_model_q.restype = ctypes.c_int
     L.strumpack_serial_finter.argtypes = [i32p, i32p, i32p, _vp, _vp, _vp, _vp, _vp, i32p]
     L.strumpack_serial_finter.restype = None
     return L

@@ -9,6 +9,7 @@
 
 #include "wt_solver.h"
 #include "wt_rhs.h"
+#include "wt_model.h"
 #include "wt_vec.h"
 
 int64_t wt_nnz(int nx, int ny, int nz);
@@ -225,6 +226,78 @@ int wt_solver_set_model(wt_solver* h, const void* lam, const void* mu, const dou
     WT_HIP(hipMemcpyAsync(s.rho, rho, nodes * 8, k, s.stream));
     WT_HIP(hipStreamSynchronize(s.stream));
     return WT_OK;
+}
+
+namespace {
+// 10 epsilon of the working precision, the reference's threshold on omega
+// (dispersion.f:43, EPSILON(1.0)*10 in its REAL)
+constexpr double kOmegaEps = 10.0 * 2.220446049250313e-16;
+
+int dispersion_dev(int64_t nodes, const double* vp, const double* vs, const double* qpi, const double* qsi,
+                   const double* rho, double freqbase, double omega_re, cpx<double>* lam, cpx<double>* mu,
+                   hipStream_t st)
+{
+    const double pi = 3.141592653589793;
+    const double omegabase = freqbase * 2.0 * pi;
+    const int att = (omega_re > kOmegaEps && omegabase > 0.0) ? 1 : 0;
+    const double fac1 = att ? std::log(omega_re / omegabase) : 0.0;
+    return wt_launch_dispersion(nodes, vp, vs, qpi, qsi, rho, fac1, att, lam, mu, st);
+}
+}  // namespace
+
+int wt_dispersion(int64_t nodes, const double* vp, const double* vs, const double* qpi, const double* qsi,
+                  const double* rho, double freqbase, double omega_re, void* lam, void* mu, int on_device,
+                  void* stream)
+{
+    if (nodes < 1 || !vp || !vs || !qpi || !qsi || !rho || !lam || !mu) return WT_ERR_ARG;
+    const hipStream_t st = (hipStream_t)stream;
+    if (on_device) {
+        int rc = dispersion_dev(nodes, vp, vs, qpi, qsi, rho, freqbase, omega_re, (cpx<double>*)lam, (cpx<double>*)mu,
+                                st);
+        if (rc) return rc;
+        WT_HIP(hipStreamSynchronize(st));
+        return WT_OK;
+    }
+    double* d = nullptr;
+    if (hipMallocAsync((void**)&d, (size_t)nodes * (5 * 8 + 32), st) != hipSuccess) return WT_ERR_NOMEM;
+    const double* src[5] = {vp, vs, qpi, qsi, rho};
+    int rc = WT_OK;
+    for (int k = 0; k < 5 && !rc; ++k)
+        if (hipMemcpyAsync(d + k * nodes, src[k], nodes * 8, hipMemcpyHostToDevice, st) != hipSuccess) rc = WT_ERR_HIP;
+    cpx<double>* dl = (cpx<double>*)(d + 5 * nodes);
+    cpx<double>* dm = dl + nodes;
+    if (!rc) rc = dispersion_dev(nodes, d, d + nodes, d + 2 * nodes, d + 3 * nodes, d + 4 * nodes, freqbase, omega_re,
+                                 dl, dm, st);
+    if (!rc && (hipMemcpyAsync(lam, dl, nodes * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(mu, dm, nodes * 16, hipMemcpyDeviceToHost, st) != hipSuccess))
+        rc = WT_ERR_HIP;
+    (void)hipFreeAsync(d, st);
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = WT_ERR_HIP;
+    return rc;
+}
+
+int wt_solver_set_model_q(wt_solver* h, const double* vp, const double* vs, const double* qpi, const double* qsi,
+                          const double* rho, double freqbase, double omega_re)
+{
+    if (!h || !vp || !vs || !qpi || !qsi || !rho) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    const int64_t nodes = s.n / 3;
+    int rc;
+    if (!s.lam && ((rc = dalloc(&s.lam, nodes)) || (rc = dalloc(&s.mu, nodes)) || (rc = dalloc(&s.rho, nodes))))
+        return rc;
+    double* d = nullptr;
+    if (hipMallocAsync((void**)&d, (size_t)nodes * 4 * 8, s.stream) != hipSuccess) return WT_ERR_NOMEM;
+    const double* src[4] = {vp, vs, qpi, qsi};
+    rc = WT_OK;
+    for (int k = 0; k < 4 && !rc; ++k)
+        if (hipMemcpyAsync(d + k * nodes, src[k], nodes * 8, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+            rc = WT_ERR_HIP;
+    if (!rc && hipMemcpyAsync(s.rho, rho, nodes * 8, hipMemcpyHostToDevice, s.stream) != hipSuccess) rc = WT_ERR_HIP;
+    if (!rc) rc = dispersion_dev(nodes, d, d + nodes, d + 2 * nodes, d + 3 * nodes, s.rho, freqbase, omega_re, s.lam,
+                                 s.mu, s.stream);
+    (void)hipFreeAsync(d, s.stream);
+    if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
+    return rc;
 }
 
 int wt_solver_assemble(wt_solver* h, const wt_grid_params* p)

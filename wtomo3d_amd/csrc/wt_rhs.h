@@ -20,3 +20,6 @@ int wt_launch_scatsrc(const int32_t* rowptr, const int32_t* colidx, const cpx<do
                       hipStream_t s);
 
 int wt_launch_add(cpx<double>* y, const cpx<double>* x, int64_t n, hipStream_t s);
+
+int wt_launch_rows(const int32_t* rowptr, const int32_t* col, const cpx<double>* val, const cpx<double>* X,
+                   cpx<double>* out, int rows, int nrhs, hipStream_t s);

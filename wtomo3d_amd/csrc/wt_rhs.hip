@@ -267,6 +267,27 @@ __global__ __launch_bounds__(256) void scatsrc_kernel(const int32_t* __restrict_
     out[row * nrhs + c] = cpx<double>(f.re * ar - f.im * ai, f.re * ai + f.im * ar);
 }
 
+// out[r, j] = sum_e val_e X[col_e, j] for a short list of sparse rows (the
+// receiver extraction): one wave per row, lanes over the right-hand sides.
+__global__ __launch_bounds__(256) void rows_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                   const cpx<double>* __restrict__ val,
+                                                   const cpx<double>* __restrict__ X, cpx<double>* __restrict__ out,
+                                                   int rows, int nrhs)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= rows) return;
+    const int32_t e0 = rowptr[row], e1 = rowptr[row + 1];
+    for (int j = lane; j < nrhs; j += 64) {
+        double ar = 0.0, ai = 0.0;
+        for (int32_t e = e0; e < e1; ++e) {
+            const cpx<double> a = val[e], x = X[(int64_t)col[e] * nrhs + j];
+            ar += a.re * x.re - a.im * x.im;
+            ai += a.re * x.im + a.im * x.re;
+        }
+        out[(int64_t)row * nrhs + j] = cpx<double>(ar, ai);
+    }
+}
+
 // u_h += u_bg (makeuh.f:41-59 with totfld)
 __global__ __launch_bounds__(256) void add_kernel(cpx<double>* __restrict__ y, const cpx<double>* __restrict__ x,
                                                   int64_t n)
@@ -314,6 +335,16 @@ int wt_launch_scatsrc(const int32_t* rowptr, const int32_t* colidx, const cpx<do
     const int64_t blocks = (n + SS_ROWS - 1) / SS_ROWS;
     hipLaunchKernelGGL(scatsrc_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rowptr, colidx, a, abg, u, srcfn, out,
                        n, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_launch_rows(const int32_t* rowptr, const int32_t* col, const cpx<double>* val, const cpx<double>* X,
+                   cpx<double>* out, int rows, int nrhs, hipStream_t s)
+{
+    if (rows <= 0) return WT_OK;
+    hipLaunchKernelGGL(rows_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, rowptr, col, val, X, out, rows,
+                       nrhs);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }

@@ -94,3 +94,20 @@ def plane_wave_dirs(nrhs, seed, inc=(10.0, 35.0)):
     th = np.deg2rad(rng.uniform(inc[0], inc[1], nrhs))
     ph = np.deg2rad(rng.uniform(0.0, 360.0, nrhs))
     return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)], axis=1)
+
+
+def attenuating(vp, vs, qpi, qsi, rho, freqbase, omega):
+    """Complex (lambda, mu) of an attenuating model at angular frequency
+    omega (DISPERSION, dispersion.f:1-131), formed on the GPU: vp, vs the
+    speeds at `freqbase` Hz, qpi, qsi = 1/Qp, 1/Qs.  Returns complex128
+    arrays flattened z fastest, with rho as float64."""
+    import ctypes
+    from ._lib import lib, check
+    arrs = [np.ascontiguousarray(np.asarray(a, np.float64).ravel()) for a in (vp, vs, qpi, qsi, rho)]
+    n = arrs[0].size
+    lam = np.zeros(n, np.complex128)
+    mu = np.zeros(n, np.complex128)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    check(lib().wt_dispersion(n, *[P(a) for a in arrs], float(freqbase), float(np.real(omega)), P(lam), P(mu), 0,
+                              None), "wt_dispersion")
+    return lam, mu, arrs[4]

@@ -117,3 +117,65 @@ def total_field(uh, ubg, count, stream=None):
     """uh += ubg over ``count`` complex128 values (DeviceBuffers): the total
     field of MAKEUH (makeuh.f:41-59)."""
     check(lib().wt_total_field(uh.ptr, ubg.ptr, int(count), stream), "wt_total_field")
+
+
+class Receivers:
+    """Receivers and hydrophones of one grid as a device extraction matrix
+    (MAKEUH's receiver and hydrophone loops, makeuh.f:60-168).
+
+    rec / hyd: dicts x, y, z (metres from the first node), spread, ireg
+    (rec may carry ``fac``, complex responses); hydrophones need the
+    model's lam, mu (complex, z fastest) and omega.  ``extract(field, nrhs)``
+    returns (uest, vest, west, utest) arrays of shape (count, nrhs)."""
+
+    def __init__(self, nx, ny, nz, dx, rec=None, hyd=None, lam=None, mu=None, omega=0.0,
+                 freesurf=(1, 0, 0, 0, 0, 0)):
+        rec = rec or dict(x=[], y=[], z=[], spread=0.5, ireg=0)
+        hyd = hyd or dict(x=[], y=[], z=[], spread=0.5, ireg=0)
+        A = lambda v: np.ascontiguousarray(v, np.float64)  # noqa: E731
+        self._keep = [A(rec[k]) for k in ("x", "y", "z")] + [A(hyd[k]) for k in ("x", "y", "z")]
+        self.ng, self.nr = len(rec["x"]), len(hyd["x"])
+        fac = None
+        if rec.get("fac") is not None:
+            fac = np.ascontiguousarray(rec["fac"], np.complex128)
+            self._keep.append(fac)
+        fs = np.ascontiguousarray(freesurf, np.int32)
+        lm = [np.ascontiguousarray(a, np.complex128).ravel() if a is not None else None for a in (lam, mu)]
+        self._keep += [fs] + [a for a in lm if a is not None]
+        om = complex(omega)
+        h = ctypes.c_void_p()
+        P = lambda a: None if a is None else a.ctypes.data_as(_vp)  # noqa: E731
+        k = self._keep
+        check(lib().wt_receivers_create(nx, ny, nz, float(dx), P(fs), self.ng, P(k[0]), P(k[1]), P(k[2]),
+                                        float(rec["spread"]), int(rec["ireg"]), P(fac), self.nr, P(k[3]), P(k[4]),
+                                        P(k[5]), float(hyd["spread"]), int(hyd["ireg"]), P(lm[0]), P(lm[1]),
+                                        om.real, om.imag, ctypes.byref(h)), "wt_receivers_create")
+        self._h = h
+        self.rows = lib().wt_receivers_rows(h)
+        self.n = 3 * nx * ny * nz
+
+    def extract(self, field, nrhs=None, stream=None):
+        host = not isinstance(field, DeviceBuffer)
+        if host:
+            field = np.ascontiguousarray(np.asarray(field).reshape(self.n, -1), np.complex128)
+            nrhs = field.shape[1]
+            dev = DeviceBuffer.from_host(field)
+        else:
+            nrhs = int(nrhs) if nrhs is not None else field.nbytes // (self.n * 16)
+            dev = field
+        out = DeviceBuffer(self.rows * nrhs * 16)
+        check(lib().wt_receivers_extract(self._h, nrhs, dev.ptr, out.ptr, stream), "wt_receivers_extract")
+        E = out.to_host(np.complex128, (self.rows, nrhs))
+        g = E[:3 * self.ng].reshape(self.ng, 3, nrhs)
+        return g[:, 0], g[:, 1], g[:, 2], E[3 * self.ng:]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().wt_receivers_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -67,6 +67,14 @@ class Solver:
         assert lam.size == self.n // 3 and mu.size == lam.size and rho.size == lam.size
         check(lib().wt_solver_set_model(self._h, _p(lam), _p(mu), _p(rho), 0), "wt_solver_set_model")
 
+    def set_model_q(self, vp, vs, qpi, qsi, rho, freqbase, omega):
+        """MODEL_MODULE state of an attenuating model at `omega`: the complex
+        moduli of DISPERSION (dispersion.f:1-131) formed on the device from
+        speeds at `freqbase` Hz and 1/Qp, 1/Qs (arrays z fastest)."""
+        arrs = [np.ascontiguousarray(np.asarray(a, np.float64).ravel()) for a in (vp, vs, qpi, qsi, rho)]
+        check(lib().wt_solver_set_model_q(self._h, *[_p(a) for a in arrs], float(freqbase), float(np.real(omega))),
+              "wt_solver_set_model_q")
+
     def assemble(self, params):
         check(lib().wt_solver_assemble(self._h, ctypes.byref(params)), "wt_solver_assemble")
 
