@@ -52,6 +52,19 @@ struct MF<double> {
     __device__ static int ok(int l) { return l >> 4; }
 };
 
+// The problem list lives in device memory, so its operand pointers reach the
+// kernel as generic pointers and would be accessed with flat instructions,
+// which the hardware counts with the LDS operations: every wait for an LDS
+// read would then also wait for the prefetch of the next K slice.  Operands
+// are therefore accessed through global-address-space pointers to 2-wide
+// vectors (one complex value each), which compile to global loads and
+// stores whose completion is tracked separately.
+template <typename R>
+using v2 = R __attribute__((ext_vector_type(2)));
+template <typename R>
+using gv2 = __attribute__((address_space(1))) v2<R>;
+typedef __attribute__((address_space(1))) const int32_t gi32;
+
 constexpr int NT = 256;  // threads per block
 constexpr int BK = 8;    // complex k per slice
 
@@ -86,6 +99,10 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
         else hi = mid - 1;
     }
     const wt_gemm_prob<R> P = probs[lo];
+    const gv2<R>* gA = (const gv2<R>*)P.A;
+    const gv2<R>* gB = (const gv2<R>*)P.B;
+    gv2<R>* gC = (gv2<R>*)P.C;
+    gi32* gperm = (gi32*)P.bperm;
     int64_t t = b - tile_start[lo];
     const int tiles_m = (P.M + BM - 1) / BM, tiles_n = (P.N + BN - 1) / BN;
     // grouped order: GM tile-rows at a time share B columns in cache
@@ -115,25 +132,25 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
     // per-thread load coordinates
     const int ka = tid / TPA, ma = (tid % TPA) * EA;
     const int nbb = tid / TPB, kb = (tid % TPB) * EB;
-    cpx<R> ra[EA], rb[EB];
+    v2<R> ra[EA], rb[EB];
 
     auto load = [&](int kt) {
         const int k0 = kt * BK;
         const bool fullA = (m0 + BM <= P.M) && (k0 + BK <= K);
         const int kk = k0 + ka;
-        const cpx<R>* pa = P.A + (int64_t)kk * P.lda + m0 + ma;
+        const gv2<R>* pa = gA + (int64_t)kk * P.lda + m0 + ma;
         if (fullA) {
 #pragma unroll
             for (int e = 0; e < EA; ++e) ra[e] = pa[e];
         } else {
 #pragma unroll
             for (int e = 0; e < EA; ++e)
-                ra[e] = (kk < K && m0 + ma + e < P.M) ? pa[e] : cpx<R>(R(0), R(0));
+                ra[e] = (kk < K && m0 + ma + e < P.M) ? pa[e] : v2<R>{R(0), R(0)};
         }
         const int nn = n0 + nbb;
-        const bool fullB = (n0 + BN <= P.N) && (k0 + BK <= K) && !P.bperm;
+        const bool fullB = (n0 + BN <= P.N) && (k0 + BK <= K) && !gperm;
         if (fullB) {
-            const cpx<R>* pb = P.B + (int64_t)nn * P.ldb + k0 + kb;
+            const gv2<R>* pb = gB + (int64_t)nn * P.ldb + k0 + kb;
 #pragma unroll
             for (int e = 0; e < EB; ++e) rb[e] = pb[e];
         } else {
@@ -141,19 +158,19 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
             for (int e = 0; e < EB; ++e) {
                 int k = k0 + kb + e;
                 if (k < K && nn < P.N) {
-                    int kr = P.bperm ? P.bperm[k] : k;
-                    rb[e] = P.B[(int64_t)nn * P.ldb + kr];
+                    int kr = gperm ? gperm[k] : k;
+                    rb[e] = gB[(int64_t)nn * P.ldb + kr];
                 } else {
-                    rb[e] = cpx<R>(R(0), R(0));
+                    rb[e] = v2<R>{R(0), R(0)};
                 }
             }
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < EA; ++e) { sAr[buf][ka][ma + e] = ra[e].re; sAi[buf][ka][ma + e] = ra[e].im; }
+        for (int e = 0; e < EA; ++e) { sAr[buf][ka][ma + e] = ra[e].x; sAi[buf][ka][ma + e] = ra[e].y; }
 #pragma unroll
-        for (int e = 0; e < EB; ++e) { sBr[buf][kb + e][nbb] = rb[e].re; sBi[buf][kb + e][nbb] = rb[e].im; }
+        for (int e = 0; e < EB; ++e) { sBr[buf][kb + e][nbb] = rb[e].x; sBi[buf][kb + e][nbb] = rb[e].y; }
     };
 
     if (nkt > 0) {
@@ -206,17 +223,18 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
                 if (m < P.M && n < P.N) {
                     R t1 = acc[0][i][j][v], t2 = acc[1][i][j][v], t3 = acc[2][i][j][v];
                     R re = t1 - t2, im = (t3 - t1) - t2;
-                    cpx<R>* pc = P.C + (int64_t)n * P.ldc + m;
+                    gv2<R>* pc = gC + (int64_t)n * P.ldc + m;
                     if (mode == WT_GEMM_SET) {
-                        *pc = cpx<R>(re, im);
+                        *pc = v2<R>{re, im};
                     } else {
-                        cpx<R> c = *pc;
-                        if (mode == WT_GEMM_SUB) *pc = cpx<R>(c.re - re, c.im - im);
-                        else *pc = cpx<R>(c.re + re, c.im + im);
+                        const v2<R> c = *pc;
+                        if (mode == WT_GEMM_SUB) *pc = v2<R>{c.x - re, c.y - im};
+                        else *pc = v2<R>{c.x + re, c.y + im};
                     }
                 }
             }
 }
+
 
 }  // namespace
 
