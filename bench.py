@@ -89,6 +89,24 @@ def cpu_baseline(nx, ny, nz, nrhs, budget):
                       % (len(sample), sample[:3], t, rate / 1e9, total, t_tree)}
 
 
+GEMM_KERNEL = "gemm_batched_kernel<float, 0>"
+
+
+def gemm_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter
+    summary (profiles/gemm_traffic.json, two rocprofv3 --pmc passes of this
+    benchmark scaled by a copy-kernel calibration), when it covers it."""
+    path = os.path.join(ROOT, "profiles", "gemm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != GEMM_KERNEL:
+        return None
+    return {"bytes_per_launch": t["bytes_per_dispatch"], "source": t.get("source", path)}
+
+
 def progress(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -175,6 +193,19 @@ def main():
     s.profile(False)
     X32 = np.zeros((n, nrhs), np.complex128)
     L.wt_d2h(X32.ctypes.data_as(ctypes.c_void_p), dX, X32.nbytes)
+    # the same step with the right-hand sides and solutions in host memory
+    # (the boundary's host-buffer mode): the PCIe-inclusive rate, not `value`
+    progress("host-buffer step")
+    Xh = np.zeros_like(B)
+    L.wt_dev_sync()
+    t0 = time.perf_counter()
+    W._lib.check(L.wt_solver_assemble(s._h, ctypes.byref(g)), "assemble")
+    W._lib.check(L.wt_solver_factor(s._h, 32), "factor")
+    W._lib.check(L.wt_solver_solve(s._h, 32, nrhs, B.ctypes.data_as(ctypes.c_void_p),
+                                   Xh.ctypes.data_as(ctypes.c_void_p), 0, a.tol, 30, ctypes.byref(st)), "solve")
+    L.wt_dev_sync()
+    t_host = time.perf_counter() - t0
+    del Xh
     s.release(32)
 
     fp64 = None
@@ -199,6 +230,7 @@ def main():
 
     g_fac = prof["gemm_factor"]
     achieved = g_fac["flops"] / (g_fac["ms"] * 1e-3) / 1e12 if g_fac["ms"] > 0 else 0.0
+    traffic = gemm_traffic()
     asm = prof["assemble"]
     asm_gbs = asm["bytes"] / (asm["ms"] * 1e-3) / 1e9 if asm["ms"] > 0 else 0.0
     progress("CPU baseline")
@@ -214,11 +246,16 @@ def main():
                                "FP64 IR to 1e-8" % (nx, ny, nz, nrhs),
                    "nx": nx, "ny": ny, "nz": nz, "nrhs": nrhs, "unknowns": info["n"], "nnz": info["nnz"],
                    "fronts": info["fronts"], "factor_flops": info["flops"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "batched complex GEMM (3M, fp32 MFMA) of the factorization",
-                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+        "roofline": {"bound": "mfma", "kernel": GEMM_KERNEL + ": the factorization's square-tile complex "
+                     "GEMMs (3M, fp32 MFMA)", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "flops_per_launch": g_fac["flops"] / max(g_fac["launches"], 1),
+                     "avg_launch_ms": g_fac["ms"] / max(g_fac["launches"], 1),
                      "launches_per_step": g_fac["launches"] / a.steps,
-                     "ms_per_step": g_fac["ms"] / a.steps},
+                     "ms_per_step": g_fac["ms"] / a.steps,
+                     "executed_frac_3m": 0.75 * achieved / PEAK_FP32_TFLOPS},
         "cpu_baseline": cpu,
         "rel_residual": max(res32), "refinement_solves": max(it32),
         "fp64": fp64,
@@ -231,6 +268,7 @@ def main():
     }
     if fp64 and "rhs_per_s" in fp64:
         out["fp32ir_over_fp64"] = value / fp64["rhs_per_s"]
+    out["pcie_inclusive_rhs_per_s"] = nrhs / t_host
     L.wt_dev_free(dB)
     L.wt_dev_free(dX)
     s.close()

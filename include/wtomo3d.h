@@ -269,10 +269,12 @@ int wt_receivers_extract(const wt_receivers *r, int nrhs, const void *d_field, v
 int wt_receivers_destroy(wt_receivers *r);
 
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
- * disable (0) and reset.  Classes: 0 factor GEMMs (panels, trailing and
- * contribution-block updates), 1 solve GEMMs, 2 diagonal-tile LU, 3
- * extend-add, 4 fills / value scatter, 5 assembly, 6 FP64 residual, 7
- * reorder / gather, 8 super-panel triangular solves.  _read returns launches, summed kernel milliseconds,
+ * disable (0) and reset.  Classes: 0 factor GEMMs in the square-tile
+ * kernel (trailing and contribution-block updates, wide panels), 1 solve
+ * GEMMs, 2 diagonal-tile LU, 3 extend-add, 4 fills / value scatter, 5
+ * assembly, 6 FP64 residual, 7 reorder / gather, 8 super-panel triangular
+ * solves, 9 factor GEMMs in the narrow-tile kernel (launches whose
+ * problems all have N <= 64).  _read returns launches, summed kernel milliseconds,
  * algorithmic flops (8 M N K per complex GEMM, 8 per residual complex
  * multiply-add) and algorithmic bytes.  No counterpart in the reference
  * (STRUMPACK's verbose statistics, strumpack_finter.c:32-33, are printed). */

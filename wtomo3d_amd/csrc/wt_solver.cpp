@@ -336,8 +336,8 @@ int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
             });
             break;
         case WT_STEP_GEMM:
-            rc = P.run(st, WT_PROF_GEMM_FACTOR, num.fplan.launches[t.gemm].flops, 0,
-                       [&] { return num.fplan.run(t.gemm, st); });
+            rc = P.run(st, num.fplan.launches[t.gemm].var ? WT_PROF_GEMM_FACTOR_NARROW : WT_PROF_GEMM_FACTOR,
+                       num.fplan.launches[t.gemm].flops, 0, [&] { return num.fplan.run(t.gemm, st); });
             break;
         default: rc = WT_ERR_STATE;
         }

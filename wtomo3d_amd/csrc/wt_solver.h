@@ -116,7 +116,7 @@ struct wt_numeric {
 // wt_solver_profile): launches, summed kernel time and algorithmic work.
 enum wt_prof_class {
     WT_PROF_GEMM_FACTOR, WT_PROF_GEMM_SOLVE, WT_PROF_DIAG, WT_PROF_EXTADD, WT_PROF_FILL,
-    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_TRSM, WT_PROF_NCLASS
+    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_TRSM, WT_PROF_GEMM_FACTOR_NARROW, WT_PROF_NCLASS
 };
 struct wt_prof {
     bool on = false;

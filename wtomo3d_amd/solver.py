@@ -106,7 +106,7 @@ class Solver:
         check(lib().wt_solver_release(self._h, prec), "wt_solver_release")
 
     PROFILE_CLASSES = ("gemm_factor", "gemm_solve", "diag_tile", "extend_add", "fill_scatter",
-                       "assemble", "residual", "reorder", "panel_trsm")
+                       "assemble", "residual", "reorder", "panel_trsm", "gemm_factor_narrow")
 
     def profile(self, enable=True):
         """Enable (and reset) per-kernel HIP-event timing on the solver stream."""
