@@ -13,8 +13,17 @@
 // walks the 27 neighbour blocks with a wave-uniform label, so the whole wave
 // evaluates one formula at a time (no divergence).  The interior stencil is
 // the stretched form with r1 = 1, r2 = 0 (identical values; the reference
-// notes the identity at fdfd3d.f:56-57).  Output rows are written straight to
-// their CSR offsets, which follow from closed forms, so no atomics or scans.
+// notes the identity at fdfd3d.f:56-57).  CSR offsets follow from closed
+// forms, so no atomics or scans.
+//
+// The kernel is bound by its stores.  A node's output is 3 rows of up to 81
+// entries, and the rows of consecutive z-nodes are 3 rows apart, so one lane
+// per node storing its own entries touches 64 cache lines per instruction
+// with 48-byte pieces.  The labels of each neighbour column (oy, ox) are
+// therefore staged in LDS and stored as the rows' contiguous 3*czn-entry
+// runs (144 bytes in FP64), consecutive lanes on consecutive entries.  The
+// run length is what sets the store rate: longer runs need more labels staged
+// per node than LDS holds for 64 FP64 nodes.
 #include "wt_common.h"
 
 #pragma clang fp contract(off)
@@ -291,36 +300,99 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
 // LDS tile: 3 (x) * 3 (y) columns of ASM_WAVE + 2 z-nodes.
 constexpr int TZ = ASM_WAVE + 2;
 
+// Output staging: the three labels of one (oy, ox) neighbour column for every
+// node of the segment, [node][row][oz][component].
+constexpr int STG = 27;
+
+// Block id -> (z segment, x, y), with consecutive ids of one XCD (blocks go
+// round-robin to the 8 XCDs) mapped to consecutive columns, so the blocks
+// sharing an L2 share input columns and write adjacent output.
+__device__ inline void segment_of_block(int64_t nblk, int nzs, int nx, int& zs, int& ix, int& iy)
+{
+    int64_t b = blockIdx.x;
+    int64_t q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
+    b = x * q + (x < r ? x : r) + i;
+    zs = (int)(b % nzs);
+    int64_t c = b / nzs;
+    ix = (int)(c % nx) + 1;
+    iy = (int)(c / nx) + 1;
+}
+
 template <typename R>
 __global__ __launch_bounds__(ASM_WAVE) void assemble_values_kernel(
     wt_grid_params p, const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ mu,
-    const R* __restrict__ rho, cpx<R>* __restrict__ out, int* __restrict__ err)
+    const R* __restrict__ rho, cpx<R>* __restrict__ out, int* __restrict__ err, int64_t nblk)
 {
-    __shared__ cpx<R> s_lam[9][TZ];
-    __shared__ cpx<R> s_mu[9][TZ];
+    // The lambda/mu tiles are dead once the node coefficients are formed; the
+    // output staging reuses their space.
+    constexpr int IN_BYTES = 2 * 9 * TZ * (int)sizeof(cpx<R>);
+    constexpr int STG_BYTES = ASM_WAVE * STG * (int)sizeof(cpx<R>);
+    __shared__ __align__(16) char s_raw[IN_BYTES > STG_BYTES ? IN_BYTES : STG_BYTES];
     __shared__ R s_rho[9][TZ];
+    cpx<R>(*s_lam)[TZ] = (cpx<R>(*)[TZ])s_raw;
+    cpx<R>(*s_mu)[TZ] = (cpx<R>(*)[TZ])(s_raw + 9 * TZ * sizeof(cpx<R>));
+    cpx<R>* stage = (cpx<R>*)s_raw;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const int z0 = blockIdx.x * ASM_WAVE;  // 0-based first z of this segment
-    const int ix = blockIdx.y + 1, iy = blockIdx.z + 1;
+    int zs, ix, iy;
+    segment_of_block(nblk, (nz + ASM_WAVE - 1) / ASM_WAVE, nx, zs, ix, iy);
+    const int z0 = zs * ASM_WAVE;  // 0-based first z of this segment
     const int lane = threadIdx.x;
 
-    // Stage the clamped 3x3 column neighbourhood, z in [z0-1, z0+ASM_WAVE].
-    for (int e = lane; e < 9 * TZ; e += ASM_WAVE) {
-        int col = e / TZ, zz = e - col * TZ;
-        int xg = ix + (col / 3) - 1, yg = iy + (col % 3) - 1;
-        xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
-        yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
-        int zg = z0 + zz;  // 1-based z of tile slot zz is z0 + zz
-        zg = zg < 1 ? 1 : (zg > nz ? nz : zg);
-        int64_t g = (int64_t)(zg - 1) + (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz;
-        s_lam[col][zz] = lam[g];
-        s_mu[col][zz] = mu[g];
-        s_rho[col][zz] = rho[g];
+    // Stage the clamped 3x3 column neighbourhood, z in [z0-1, z0+ASM_WAVE]
+    // (tile slot zz holds 1-based z = z0 + zz).  Each column is a contiguous
+    // run of the z-fastest model arrays; every load is issued before the
+    // first LDS write, so the wave waits for memory once.
+    {
+        auto colbase = [&](int col) {
+            int xg = ix + (col / 3) - 1, yg = iy + (col % 3) - 1;
+            xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
+            yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
+            return (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz;
+        };
+        auto zclamp = [&](int zz) {
+            int zg = z0 + zz;
+            return zg < 1 ? 1 : (zg > nz ? nz : zg);
+        };
+        cpx<R> la[9], mv[9];
+        R rh[9];
+        const int zg = zclamp(lane);
+#pragma unroll
+        for (int col = 0; col < 9; ++col) {
+            const int64_t g = colbase(col) + zg - 1;
+            la[col] = lam[g];
+            mv[col] = mu[g];
+            rh[col] = rho[g];
+        }
+#pragma unroll
+        for (int col = 0; col < 9; ++col) {
+            s_lam[col][lane] = la[col];
+            s_mu[col][lane] = mv[col];
+            s_rho[col][lane] = rh[col];
+        }
+        // the two slots past the wave width
+        if (lane < TZ - ASM_WAVE) {
+            const int zz = ASM_WAVE + lane, zh = zclamp(zz);
+#pragma unroll
+            for (int col = 0; col < 9; ++col) {
+                const int64_t g = colbase(col) + zh - 1;
+                la[col] = lam[g];
+                mv[col] = mu[g];
+                rh[col] = rho[g];
+            }
+#pragma unroll
+            for (int col = 0; col < 9; ++col) {
+                s_lam[col][zz] = la[col];
+                s_mu[col][zz] = mv[col];
+                s_rho[col][zz] = rh[col];
+            }
+        }
     }
     __syncthreads();
 
-    const int iz = z0 + lane + 1;
-    if (iz > nz) return;
+    // Lanes past the end of the column evaluate the last node again and
+    // write nothing (every lane takes part in the staged output).
+    const bool live = z0 + lane + 1 <= nz;
+    const int iz = live ? z0 + lane + 1 : nz;
     const int ip = p.ipml;
     // Region class, fdfd3d.f:50-115.
     const bool inx = ix > ip && ix <= nx - ip, iny = iy > ip && iy <= ny - ip;
@@ -329,13 +401,13 @@ __global__ __launch_bounds__(ASM_WAVE) void assemble_values_kernel(
     else if (iz == nz || ix == 1 || ix == nx || iy == 1 || iy == ny) mode = 2;
     else if (iz <= ip) {
         if (iz == 1 && !p.fst) mode = 2;
-        else if (iz == 1 && !p.usemin) { atomicOr(err, 2); return; }
+        else if (iz == 1 && !p.usemin) { if (live) atomicOr(err, 2); }
         else if (!p.fst) mode = 1;
         else if (inx && iny) mode = 0;
         else { mode = 1; izp = ip + 1; }
     }
     const bool topfree = p.fst && iz == 1;
-    const int tz = lane + 1;  // tile z-slot of this node
+    const int tz = iz - z0;  // tile z-slot of this node
 
     NodeCoef<R> c;
     cpx<R> cen_mc;
@@ -396,7 +468,7 @@ __global__ __launch_bounds__(ASM_WAVE) void assemble_values_kernel(
             bool ok = pml_axis(ix, nx, ip, pmld, dx, pmlf, vp, eyeom, r1x, r2x);
             ok = pml_axis(iy, ny, ip, pmld, dx, pmlf, vp, eyeom, r1y, r2y) && ok;
             ok = pml_axis(izp, nz, ip, pmld, dx, pmlf, vp, eyeom, r1z, r2z) && ok;
-            if (!ok) atomicOr(err, 1);
+            if (!ok && live) atomicOr(err, 1);
         }
         c.rx1 = (r2x / dx2) / R(4);
         c.ry1 = (r2y / dx2) / R(4);
@@ -409,39 +481,64 @@ __global__ __launch_bounds__(ASM_WAVE) void assemble_values_kernel(
         c.ryz = ((r1y * r1z) / dxx4) / R(4);
     }
 
-    // Row layout of this node: 3 rows of 3*nnb entries, neighbours ascending.
-    const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), czn = 1 + (iz > 1) + (iz < nz);
-    const int rowlen = 3 * cxn * cyn * czn;
-    cpx<R>* base = out + node_offset(iz, ix, iy, nx, ny, nz);
+    // Row layout: a node has 3 rows of 3*cxn*cyn*czn entries, its neighbours
+    // ascending (y, then x, then z).  The nodes of a column segment follow
+    // each other in the CSR arrays, and in each row the neighbour column
+    // (oy, ox) owns one run of 3*czn entries.  Each column's three labels are
+    // staged in LDS and written as those runs: a store instruction covers two
+    // nodes (54 lanes, lane = node parity h, row r, entry e), so consecutive
+    // lanes store consecutive entries.
+    //
+    // With czn(iz) = 3 - [iz = 1] - [iz = nz], the first entry of node n of
+    // the segment (iz = z0 + 1 + n) is 9 cxy (3 n - [z0 = 0, n >= 1]) and a
+    // row holds 3 cxy czn entries, so entry (r, e) of the run of column rank
+    // j = ry cxn + rx lies at 9 cxy (3 n - [z0 = 0, n >= 1]) + czn K + e with
+    // K = 3 cxy r + 3 j.
+    const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), cxy = cxn * cyn;
+    const int zlast = nz - z0 < ASM_WAVE ? nz - z0 : ASM_WAVE;  // live nodes
+    cpx<R>* const seg = out + node_offset(z0 + 1, ix, iy, nx, ny, nz);
+    const int fh = lane / STG, frem = lane - fh * STG, fr = frem / 9, fe = frem - fr * 9;
+    const bool dir = mode == 2;
+    __syncthreads();  // lambda/mu tiles dead from here on
 
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
-        const bool exists = (iy + oy >= 1) && (iy + oy <= ny) && (ix + ox >= 1) && (ix + ox <= nx) &&
-                            (iz + oz >= 1) && (iz + oz <= nz);
-        // density the mass lumping uses at this neighbour (clamped gather,
-        // top layer zeroed at the free surface, setcoefs3d.f:187-191)
-        R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
-        if (topfree && oz == -1) ro = R(0);
-        cpx<R> v[9];
-        label_block<R>(k, c, ro, v);
-        if (mode == 2) {
+    for (int col = 0; col < 9; ++col) {
+        const int oy = col / 3 - 1, ox = col % 3 - 1;
+        if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
 #pragma unroll
-            for (int e = 0; e < 9; ++e)
-                v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
+        for (int oz = -1; oz <= 1; ++oz) {
+            const int k = 3 * col + oz + 1;
+            // density the mass lumping uses at this neighbour (clamped gather,
+            // top layer zeroed at the free surface, setcoefs3d.f:187-191)
+            R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
+            if (topfree && oz == -1) ro = R(0);
+            cpx<R> v[9];
+            label_block<R>(k, c, ro, v);
+            if (dir) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int e = 0; e < 3; ++e) stage[lane * STG + r * 9 + (oz + 1) * 3 + e] = v[3 * r + e];
         }
-        if (exists) {
-            const int rank = ((oy > -1) ? (iy > 1) + (oy > 0) : 0) * cxn * czn +
-                             ((ox > -1) ? (ix > 1) + (ox > 0) : 0) * czn +
-                             ((oz > -1) ? (iz > 1) + (oz > 0) : 0);
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                cpx<R>* dst = base + r * rowlen + 3 * rank;
-                dst[0] = v[3 * r + 0];
-                dst[1] = v[3 * r + 1];
-                dst[2] = v[3 * r + 2];
+        __syncthreads();
+        const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
+        const int K = 3 * cxy * fr + 3 * (ry * cxn + rx);
+        if (fh < 2) {
+#pragma unroll 4
+            for (int n = fh; n < zlast; n += 2) {
+                const int izn = z0 + 1 + n;
+                const int czn = 1 + (izn > 1) + (izn < nz);
+                if (fe < 3 * czn) {
+                    const int dst = 9 * cxy * (3 * n - (z0 == 0 && n > 0 ? 1 : 0)) + czn * K + fe;
+                    seg[dst] = stage[n * STG + frem + (izn == 1 ? 3 : 0)];
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -507,9 +604,11 @@ int wt_launch_values(const wt_grid_params& p, const void* d_lam, const void* d_m
 {
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return WT_ERR_ARG;
     if (!p.usemin && p.fst) return WT_ERR_UNSUPPORTED;
-    dim3 grid((p.nz + ASM_WAVE - 1) / ASM_WAVE, p.nx, p.ny);
-    hipLaunchKernelGGL(assemble_values_kernel<R>, grid, dim3(ASM_WAVE), 0, stream, p,
-                       (const cpx<R>*)d_lam, (const cpx<R>*)d_mu, (const R*)d_rho, (cpx<R>*)d_vals, d_err);
+    const int64_t nblk = (int64_t)((p.nz + ASM_WAVE - 1) / ASM_WAVE) * p.nx * p.ny;
+    if (nblk > 0x7fffffff) return WT_ERR_ARG;
+    hipLaunchKernelGGL(assemble_values_kernel<R>, dim3((unsigned)nblk), dim3(ASM_WAVE), 0, stream, p,
+                       (const cpx<R>*)d_lam, (const cpx<R>*)d_mu, (const R*)d_rho, (cpx<R>*)d_vals, d_err,
+                       nblk);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
