@@ -299,6 +299,67 @@ int strumpack_parallel_finter(void);
  * kernel that replaces STRUMPACK's dense front updates. */
 int wt_gemm_host(int prec, int M, int N, int K, const void *A, int64_t lda, const void *B, int64_t ldb,
                  void *C, int64_t ldc, int mode, const int32_t *bperm, int reps, double *ms);
+/* The same with the tile variant chosen by the caller: 0 (128 x 128 fp32,
+ * 128 x 64 fp64), 1 (256 x 64 fp32), 2 (64 x 64), or -1 for the default rule
+ * (1 when N <= 64, else 0).  No counterpart in the reference. */
+int wt_gemm_host_var(int prec, int var, int M, int N, int K, const void *A, int64_t lda, const void *B,
+                     int64_t ldb, void *C, int64_t ldc, int mode, const int32_t *bperm, int reps, double *ms);
+
+/* ---------------- batches of independent small models ---------------- */
+/* The reference solves one model at one frequency per pass of its driver
+ * loop: DISPERSION, ASMBLE, then STRUMPACK_SERIAL_FINTER jobs 2 and 3
+ * (elwave.f90:147-186).  A batch runs that sequence for many models at once,
+ * each with its own grid, medium and frequency: the models form one
+ * block-diagonal system whose elimination forest is factored level by level
+ * (every launch carries the fronts of all models at that depth), captured in
+ * HIP graphs and replayed.  Unknowns of the batch are the models' unknowns
+ * concatenated in model order (wt_batch_offsets). */
+typedef struct wt_batch wt_batch;
+
+/* dims: nmodels triples (nx, ny, nz).  Models are grouped, in order, so that
+ * each group's CSR has at most max_group_nnz entries (<= 0: 2^31 - 1, the
+ * 32-bit row-pointer limit); every group is factored on its own stream. */
+int wt_batch_create(int nmodels, const int32_t *dims, int leaf_nodes, int64_t max_group_nnz, wt_batch **out);
+int wt_batch_destroy(wt_batch *b);
+/* info[8]: unknowns, CSR entries, fronts, levels (deepest tree), largest
+ * front pivots, groups, factor entries, contribution-block entries;
+ * *flops: the factorization's 8 (p^3/3 + p^2 u + p u^2) summed. */
+int wt_batch_info(const wt_batch *b, int64_t *info, double *flops);
+/* unk_off[nmodels + 1]: first unknown of each model; group_of[nmodels]. */
+int wt_batch_offsets(const wt_batch *b, int64_t *unk_off, int32_t *group_of);
+/* Capture the factorization and the refinement step in HIP graphs (default
+ * on) or launch eagerly. */
+int wt_batch_set_graphs(wt_batch *b, int enable);
+/* MODEL_MODULE state of every model (as wt_solver_set_model), the models'
+ * arrays concatenated in model order. */
+int wt_batch_set_models(wt_batch *b, const void *lam, const void *mu, const double *rho, int on_device);
+/* ASMBLE of every model with its own parameters params[nmodels] (grid sizes
+ * must match the batch's; each carries its own frequency). */
+int wt_batch_assemble(wt_batch *b, const wt_grid_params *params);
+/* Numeric factorization of every model (job 2), WT_PREC_F32 or _F64. */
+int wt_batch_factor(wt_batch *b, int factor_prec);
+/* Tile-shape autotuner: factors once per GEMM tile variant, timing every
+ * launch, and keeps each launch's fastest variant for later factorizations
+ * (the batch is left factored).  *speedup: GEMM time under the default
+ * choice over the tuned time.  No counterpart in the reference. */
+int wt_batch_autotune(wt_batch *b, int factor_prec, double *speedup);
+/* The union's symbolic analysis on the host only (no GPU): info[8] =
+ * unknowns, CSR entries, fronts, levels, largest front pivots, factor
+ * entries, contribution-block entries, solve-update entries. */
+int wt_batch_symbolic_info(int nmodels, const int32_t *dims, int leaf_nodes, int64_t *info, double *flops);
+/* The CSR pattern of SREL3D + SPARSE_ADJ2CRS (srgraphs.f:638-734,
+ * sparse.f:13-49) computed on the host (wt_graph computes it on the GPU). */
+int wt_graph_host(int nx, int ny, int nz, int base, int32_t *rowptr, int32_t *colidx);
+/* y = A x with every model's FP64 CSR values; x, y row-major (total
+ * unknowns) x nrhs, nrhs <= 64. */
+int wt_batch_apply(wt_batch *b, int nrhs, const void *x, void *y, int on_device);
+/* Solve with FP64 iterative refinement (job 3 for many sources): rhs and x
+ * are row-major (total unknowns) x nrhs, nrhs <= 64; model k uses its first
+ * ncol[k] columns (the others are ignored on input and zero on output).
+ * Refines until every model's columns reach |b - A x| / |b| <= tol or
+ * max_iter solves; stats[nmodels] per model. */
+int wt_batch_solve(wt_batch *b, int factor_prec, int nrhs, const int32_t *ncol, const void *rhs, void *x,
+                   int on_device, double tol, int max_iter, wt_solve_stats *stats);
 
 /* ---------------- device memory helpers ---------------- */
 int wt_dev_alloc(size_t bytes, void **d_ptr);

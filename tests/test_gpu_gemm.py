@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0):
+def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0, var=-1):
     rng = np.random.default_rng(seed)
     ct = np.complex64 if prec == 32 else np.complex128
     lda, ldb, ldc = lda or M, ldb or K, ldc or M
@@ -26,8 +26,8 @@ def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0):
     ref = {0: Cm - Am @ Bm, 1: Am @ Bm, 2: Cm + Am @ Bm}[mode]
     Cout = C.copy()
     ms = ctypes.c_double()
-    rc = W.lib().wt_gemm_host(prec, M, N, K, P(A), lda, P(B), ldb, P(Cout), ldc, mode,
-                              P(bp) if perm else None, 1, ctypes.byref(ms))
+    rc = W.lib().wt_gemm_host_var(prec, var, M, N, K, P(A), lda, P(B), ldb, P(Cout), ldc, mode,
+                                  P(bp) if perm else None, 1, ctypes.byref(ms))
     assert rc == 0
     got = Cout[:, :M].T.astype(np.complex128)
     scale = np.abs(Am) @ np.abs(Bm) + np.abs(Cm)
@@ -49,4 +49,15 @@ def test_gemm_modes_shapes(prec, shape, mode):
 @pytest.mark.parametrize("prec", [32, 64])
 def test_gemm_row_permuted_b(prec):
     err = _run(prec, 64, 200, 64, 1, perm=True, seed=3)
+    assert err <= (2e-6 if prec == 32 else 1e-14)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("var", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (65, 65, 8)])
+def test_gemm_tile_variants(prec, var, shape):
+    """Every tile variant (the autotuner picks among them per launch) gives
+    the same product; mode SUB with padded leading dimensions."""
+    M, N, K = shape
+    err = _run(prec, M, N, K, 0, lda=M + 1, ldb=K + 2, ldc=M + 3, seed=11, var=var)
     assert err <= (2e-6 if prec == 32 else 1e-14)

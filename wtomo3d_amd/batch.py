@@ -1,0 +1,170 @@
+"""Batches of independent small models (the wt_batch_* C ABI).
+
+The reference runs DISPERSION, ASMBLE and STRUMPACK_SERIAL_FINTER jobs 2/3
+once per model and frequency (elwave.f90:147-186).  A ``Batch`` does that for
+many models at once, each with its own grid, medium and frequency: the
+models are one block-diagonal system whose elimination forest is factored
+level by level on the GPU, with the factorization and each refinement step
+captured in HIP graphs.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib, check, grid_params, WtGridParams, WtSolveStats
+from . import media
+
+_vp = ctypes.c_void_p
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+class Batch:
+    """dims: sequence of (nx, ny, nz).  max_group_nnz > 0 caps the CSR
+    entries of one group (a block-diagonal system on its own stream)."""
+
+    def __init__(self, dims, leaf_nodes=64, max_group_nnz=0):
+        d = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 3))
+        self.dims = [tuple(int(v) for v in r) for r in d]
+        self.m = len(self.dims)
+        h = ctypes.c_void_p()
+        check(lib().wt_batch_create(self.m, _p(d), int(leaf_nodes), int(max_group_nnz), ctypes.byref(h)),
+              "wt_batch_create")
+        self._h = h
+        self.offsets = np.zeros(self.m + 1, np.int64)
+        self.group_of = np.zeros(self.m, np.int32)
+        check(lib().wt_batch_offsets(h, _p(self.offsets), _p(self.group_of)), "wt_batch_offsets")
+        self.n = int(self.offsets[-1])
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().wt_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        v = (ctypes.c_int64 * 8)()
+        fl = ctypes.c_double()
+        check(lib().wt_batch_info(self._h, v, ctypes.byref(fl)), "wt_batch_info")
+        keys = ("n", "nnz", "fronts", "levels", "max_p", "groups", "factor_len", "cb_len")
+        out = {k: int(v[i]) for i, k in enumerate(keys)}
+        out["flops"] = fl.value
+        return out
+
+    def set_graphs(self, enable=True):
+        check(lib().wt_batch_set_graphs(self._h, int(bool(enable))), "wt_batch_set_graphs")
+
+    def set_models(self, lams, mus, rhos):
+        """Per-model (lambda, mu) complex and rho real arrays, z fastest."""
+        lam = np.ascontiguousarray(np.concatenate([np.asarray(a, np.complex128).ravel() for a in lams]))
+        mu = np.ascontiguousarray(np.concatenate([np.asarray(a, np.complex128).ravel() for a in mus]))
+        rho = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).ravel() for a in rhos]))
+        if lam.size * 3 != self.n or mu.size != lam.size or rho.size != lam.size:
+            raise ValueError("model arrays do not match the batch's grids")
+        check(lib().wt_batch_set_models(self._h, _p(lam), _p(mu), _p(rho), 0), "wt_batch_set_models")
+
+    def assemble(self, params):
+        arr = (WtGridParams * self.m)(*params)
+        check(lib().wt_batch_assemble(self._h, arr), "wt_batch_assemble")
+
+    def factor(self, prec=32):
+        check(lib().wt_batch_factor(self._h, prec), "wt_batch_factor")
+
+    def autotune(self, prec=32):
+        sp = ctypes.c_double()
+        check(lib().wt_batch_autotune(self._h, prec, ctypes.byref(sp)), "wt_batch_autotune")
+        return sp.value
+
+    def _pack(self, blocks):
+        cols = [1 if np.asarray(b).ndim == 1 else np.asarray(b).shape[1] for b in blocks]
+        nrhs = max(cols)
+        B = np.zeros((self.n, nrhs), np.complex128)
+        for k, b in enumerate(blocks):
+            b = np.asarray(b, np.complex128).reshape(self.offsets[k + 1] - self.offsets[k], -1)
+            B[self.offsets[k]:self.offsets[k + 1], :b.shape[1]] = b
+        return B, np.asarray(cols, np.int32), nrhs
+
+    def _unpack(self, X, cols, shapes_1d):
+        out = []
+        for k in range(self.m):
+            x = X[self.offsets[k]:self.offsets[k + 1], :cols[k]].copy()
+            out.append(x[:, 0] if shapes_1d[k] else x)
+        return out
+
+    def apply(self, xs):
+        """A_k x_k for every model (x_k: (n_k,) or (n_k, c))."""
+        X, cols, nrhs = self._pack(xs)
+        Y = np.zeros_like(X)
+        check(lib().wt_batch_apply(self._h, nrhs, _p(X), _p(Y), 0), "wt_batch_apply")
+        return self._unpack(Y, cols, [np.asarray(x).ndim == 1 for x in xs])
+
+    def solve(self, bs, prec=32, tol=1e-10, max_iter=30):
+        """Solves A_k x_k = b_k for every model; returns (xs, stats)."""
+        B, cols, nrhs = self._pack(bs)
+        X = np.zeros_like(B)
+        st = (WtSolveStats * self.m)()
+        check(lib().wt_batch_solve(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st),
+              "wt_batch_solve")
+        stats = [{k: getattr(s, k) for k, _ in WtSolveStats._fields_} for s in st]
+        return self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs]), stats
+
+
+def symbolic_info(dims, leaf_nodes=64):
+    """Host-only symbolic analysis of a batch's union (no GPU): dict as
+    ``Batch.info`` without the group count."""
+    d = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 3))
+    v = (ctypes.c_int64 * 8)()
+    fl = ctypes.c_double()
+    check(lib().wt_batch_symbolic_info(d.shape[0], _p(d), int(leaf_nodes), v, ctypes.byref(fl)),
+          "wt_batch_symbolic_info")
+    keys = ("n", "nnz", "fronts", "levels", "max_p", "factor_len", "cb_len", "w_len")
+    out = {k: int(v[i]) for i, k in enumerate(keys)}
+    out["flops"] = fl.value
+    return out
+
+
+def small_models(nmodels=256, seed=1004, side=(16, 40), freq=(0.05, 0.5), nrhs=(1, 16), dx=250.0,
+                 pml_nodes=5):
+    """The batched-small-models workload: every side of every grid uniform
+    in ``side`` nodes, a seeded layered medium per model (as ``media.layered``),
+    a frequency uniform in ``freq`` Hz and a right-hand-side count uniform in
+    ``nrhs``.  Returns a list of dicts (dims, vp, vs, rho, freq, nrhs,
+    params); the PML is ``pml_nodes`` intervals wide on every side but the
+    free surface."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nmodels):
+        nx, ny, nz = (int(v) for v in rng.integers(side[0], side[1] + 1, 3))
+        f = float(rng.uniform(freq[0], freq[1]))
+        c = int(rng.integers(nrhs[0], nrhs[1] + 1))
+        vp1, vs1, rho1 = media.layered(nx, ny, nz, dx, seed * 1000 + k)
+        shape = (ny, nx, nz)
+        vp = np.broadcast_to(vp1[None, None, :], shape)
+        vs = np.broadcast_to(vs1[None, None, :], shape)
+        rho = np.broadcast_to(rho1[None, None, :], shape)
+        p = grid_params(nx, ny, nz, dx, 2 * np.pi * f, pmld=pml_nodes * dx)
+        out.append(dict(dims=(nx, ny, nz), vp=vp, vs=vs, rho=rho, freq=f, nrhs=c, params=p, dx=dx,
+                        seed=seed * 1000 + k))
+    return out
+
+
+def plane_wave_block(model, seed_offset=0):
+    """Plane-wave displacement fields (n_k, nrhs_k) of a small model at its
+    frequency (known solutions: the right-hand sides are A_k U_k)."""
+    nx, ny, nz = model["dims"]
+    vpm = float(np.median(model["vp"]))
+    return media.plane_wave_fields(nx, ny, nz, model["dx"], 2 * np.pi * model["freq"], vpm, model["nrhs"],
+                                   model["seed"] + seed_offset)

@@ -42,6 +42,12 @@ int salloc(T** p, int64_t count, hipStream_t st)
                                                                                              : WT_ERR_NOMEM;
 }
 
+int ensure_ir(wt_solver_impl& s, int nrhs);
+}  // namespace
+
+int wt_solver_ir_buffers(wt_solver_impl& s, int nrhs) { return ensure_ir(s, nrhs); }
+
+namespace {
 int ensure_ir(wt_solver_impl& s, int nrhs)
 {
     if (s.ir_cap >= nrhs) return WT_OK;

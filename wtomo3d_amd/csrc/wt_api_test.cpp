@@ -6,10 +6,11 @@
 
 namespace {
 template <typename R>
-int run_gemm(int M, int N, int K, const void* A, int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc,
-             int mode, const int32_t* bperm, int reps, double* ms)
+int run_gemm(int var, int M, int N, int K, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+             int64_t ldc, int mode, const int32_t* bperm, int reps, double* ms)
 {
-    if (M < 1 || N < 1 || K < 1 || lda < M || ldc < M || reps < 1) return WT_ERR_ARG;
+    if (M < 1 || N < 1 || K < 1 || lda < M || ldc < M || reps < 1 || var < -1 || var >= WT_GEMM_NVAR)
+        return WT_ERR_ARG;
     const int64_t cs = sizeof(cpx<R>);
     const int64_t na = lda * K, nbv = ldb * N, nc = ldc * N;
     cpx<R>*dA = nullptr, *dB = nullptr, *dC = nullptr;
@@ -26,7 +27,7 @@ int run_gemm(int M, int N, int K, const void* A, int64_t lda, const void* B, int
         wt_gemm_prob<R> p;
         p.A = dA; p.B = dB; p.C = dC; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
         p.M = M; p.N = N; p.K = K; p.mode = mode; p.bperm = dp; p.pad = 0;
-        const int var = N <= 64 ? 1 : 0;
+        if (var < 0) var = N <= 64 ? 1 : 0;
         int64_t ts[2] = {0, wt_gemm_tiles<R>(M, N, var)};
         if (hipMemcpy(dA, A, na * cs, hipMemcpyHostToDevice) || hipMemcpy(dB, B, nbv * cs, hipMemcpyHostToDevice) ||
             (bperm && hipMemcpy(dp, bperm, K * 4, hipMemcpyHostToDevice)) ||
@@ -53,10 +54,18 @@ int run_gemm(int M, int N, int K, const void* A, int64_t lda, const void* B, int
 }
 }  // namespace
 
+// var: tile variant (0..WT_GEMM_NVAR-1), or -1 for the default rule
+extern "C" int wt_gemm_host_var(int prec, int var, int M, int N, int K, const void* A, int64_t lda, const void* B,
+                                int64_t ldb, void* C, int64_t ldc, int mode, const int32_t* bperm, int reps,
+                                double* ms)
+{
+    if (prec == WT_PREC_F32) return run_gemm<float>(var, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
+    if (prec == WT_PREC_F64) return run_gemm<double>(var, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
+    return WT_ERR_ARG;
+}
+
 extern "C" int wt_gemm_host(int prec, int M, int N, int K, const void* A, int64_t lda, const void* B, int64_t ldb,
                             void* C, int64_t ldc, int mode, const int32_t* bperm, int reps, double* ms)
 {
-    if (prec == WT_PREC_F32) return run_gemm<float>(M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
-    if (prec == WT_PREC_F64) return run_gemm<double>(M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
-    return WT_ERR_ARG;
+    return wt_gemm_host_var(prec, -1, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
 }

@@ -19,6 +19,7 @@ struct wt_dfront {
     int32_t parent, level, slot, pad;
 };
 
+int wt_zero_i32(int* p, int n, hipStream_t s);
 template <typename R>
 int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s);
 template <typename Ri, typename R>

@@ -463,6 +463,22 @@ __global__ __launch_bounds__(256) void solve_gather_upd_kernel(const wt_dfront* 
 
 }  // namespace
 
+static __global__ void zero_i32_kernel(int* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// A fill kernel rather than a memset node, so that captured sequences
+// replay it as an ordinary kernel.
+int wt_zero_i32(int* p, int n, hipStream_t s)
+{
+    if (n <= 0) return WT_OK;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
 template <typename R>
 int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s)
 {

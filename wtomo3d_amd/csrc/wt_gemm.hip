@@ -8,9 +8,10 @@
 // The MFMA computes D = X Y with X taken from B^T and Y from A^T, so D is a
 // tile of C^T: each lane's accumulators then run along the rows of the
 // column-major C, and the epilogue stores are contiguous across lanes.
-// 256 threads = 4 waves in a 2 x 2 grid; fp32 uses v_mfma_f32_32x32x2f32
-// (block 128 x 128, wave 64 x 64), fp64 uses v_mfma_f64_16x16x4f64 (block
-// 128 x 64, wave 64 x 32).  K is consumed in slices of 8 complex values,
+// 256 threads = 4 waves; fp32 uses v_mfma_f32_32x32x2f32 (default block
+// 128 x 128 as 2 x 2 waves of 64 x 64), fp64 uses v_mfma_f64_16x16x4f64
+// (default block 128 x 64, waves of 64 x 32); the other tile variants are
+// listed with wt_gemm_tile.  K is consumed in slices of 8 complex values,
 // staged through double-buffered LDS as separate real and imaginary planes.
 #include "wt_gemm.h"
 
@@ -243,12 +244,21 @@ int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, 
                    hipStream_t stream)
 {
     if (nprob <= 0 || ntiles <= 0) return WT_OK;
-    if (var)
+    if (ntiles > 0x7fffffff) return WT_ERR_ARG;
+    switch (var) {
+    case 1:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 1>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                            d_tile_start, nprob, ntiles);
-    else
+        break;
+    case 2:
+        hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                           d_tile_start, nprob, ntiles);
+        break;
+    default:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 0>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                            d_tile_start, nprob, ntiles);
+        break;
+    }
     WT_HIP(hipGetLastError());
     return WT_OK;
 }

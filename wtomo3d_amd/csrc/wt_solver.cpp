@@ -12,7 +12,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "wt_vec.h"
 
@@ -42,11 +44,12 @@ int wt_gemm_plan<R>::upload()
 {
     dfree(d_probs);
     dfree(d_tiles);
-    if (dalloc(&d_probs, (int64_t)probs.size()) || dalloc(&d_tiles, (int64_t)tiles.size())) return WT_ERR_NOMEM;
+    tstride = (int64_t)tiles[0].size();
+    if (dalloc(&d_probs, (int64_t)probs.size()) || dalloc(&d_tiles, WT_GEMM_NVAR * tstride)) return WT_ERR_NOMEM;
     if (!probs.empty())
         WT_HIP(hipMemcpy(d_probs, probs.data(), probs.size() * sizeof(probs[0]), hipMemcpyHostToDevice));
-    if (!tiles.empty())
-        WT_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    for (int v = 0; v < WT_GEMM_NVAR && tstride > 0; ++v)
+        WT_HIP(hipMemcpy(d_tiles + v * tstride, tiles[v].data(), tstride * sizeof(int64_t), hipMemcpyHostToDevice));
     return WT_OK;
 }
 template <typename R>
@@ -55,13 +58,26 @@ void wt_gemm_plan<R>::release()
     dfree(d_probs);
     dfree(d_tiles);
     probs.clear();
-    tiles.clear();
+    for (int v = 0; v < WT_GEMM_NVAR; ++v) tiles[v].clear();
     launches.clear();
+    tstride = 0;
+}
+
+template <typename R>
+void wt_numeric<R>::drop_graphs()
+{
+    if (fgraph) (void)hipGraphExecDestroy(fgraph);
+    fgraph = nullptr;
+    if (sgraph) (void)hipGraphExecDestroy(sgraph);
+    sgraph = nullptr;
+    sgraph_nrhs = 0;
 }
 
 template <typename R>
 void wt_numeric<R>::release()
 {
+    drop_graphs();
+    tuned = false;
     dfree(factor);
     dfree(cb[0]);
     dfree(cb[1]);
@@ -82,6 +98,8 @@ void wt_numeric<R>::release()
     ready = false;
 }
 
+static int upload_symbolic(wt_solver_impl& s);
+
 int wt_solver_init(wt_solver_impl& s, int nx, int ny, int nz, int leaf_nodes)
 {
     if (nx < 2 || ny < 2 || nz < 2) return WT_ERR_ARG;
@@ -99,6 +117,80 @@ int wt_solver_init(wt_solver_impl& s, int nx, int ny, int nz, int leaf_nodes)
     if ((rc = wt_symbolic_build(nx, ny, nz, leaf_nodes, WT_NB, hr.data(), hc.data(), s.S))) return rc;
     hc.clear();
     hc.shrink_to_fit();
+    return upload_symbolic(s);
+}
+
+// Independent grids as one block-diagonal system (a forest of elimination
+// trees advancing level by level together): per-grid patterns and symbolic
+// analyses on the host, in parallel, then their union.  hr/hc receive the
+// union's CSR pattern (0-based) when given.
+int wt_union_build(const std::vector<wt_grid_dims>& grids, int leaf_nodes, wt_symbolic& U,
+                   std::vector<int32_t>* hr, std::vector<int32_t>* hc)
+{
+    const int m = (int)grids.size();
+    if (m < 1) return WT_ERR_ARG;
+    std::vector<int64_t> noff(m + 1, 0), eoff(m + 1, 0);
+    for (int k = 0; k < m; ++k) {
+        const wt_grid_dims& g = grids[k];
+        if (g.nx < 2 || g.ny < 2 || g.nz < 2) return WT_ERR_ARG;
+        noff[k + 1] = noff[k] + 3LL * g.nx * g.ny * g.nz;
+        eoff[k + 1] = eoff[k] + wt_nnz(g.nx, g.ny, g.nz);
+    }
+    if (eoff[m] >= (1LL << 31)) return WT_ERR_ARG;
+    if (hr) hr->assign(noff[m] + 1, 0);
+    if (hc) hc->assign(eoff[m], 0);
+    std::vector<wt_symbolic> parts(m);
+    std::vector<int> res(m, WT_OK);
+    {
+        const unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; ++t)
+            th.emplace_back([&, t] {
+                std::vector<int32_t> r, c;
+                for (int k = (int)t; k < m; k += (int)nth) {
+                    const wt_grid_dims& g = grids[k];
+                    const int64_t n = noff[k + 1] - noff[k], e = eoff[k + 1] - eoff[k];
+                    r.resize(n + 1);
+                    c.resize(e);
+                    int rc = wt_pattern_host(g.nx, g.ny, g.nz, 0, 0, r.data(), c.data());
+                    if (!rc) rc = wt_symbolic_build(g.nx, g.ny, g.nz, leaf_nodes, WT_NB, r.data(), c.data(), parts[k]);
+                    res[k] = rc;
+                    if (rc) continue;
+                    if (hr)
+                        for (int64_t i = 0; i < n; ++i) (*hr)[noff[k] + i] = (int32_t)(r[i] + eoff[k]);
+                    if (hc)
+                        for (int64_t i = 0; i < e; ++i) (*hc)[eoff[k] + i] = (int32_t)(c[i] + noff[k]);
+                }
+            });
+        for (auto& x : th) x.join();
+    }
+    for (int k = 0; k < m; ++k)
+        if (res[k]) return res[k];
+    if (hr) (*hr)[noff[m]] = (int32_t)eoff[m];
+    std::vector<const wt_symbolic*> pp(m);
+    for (int k = 0; k < m; ++k) pp[k] = &parts[k];
+    return wt_symbolic_union(pp, U);
+}
+
+int wt_solver_init_union(wt_solver_impl& s, const std::vector<wt_grid_dims>& grids, int leaf_nodes)
+{
+    std::vector<int32_t> hr, hc;
+    int rc = wt_union_build(grids, leaf_nodes, s.S, &hr, &hc);
+    if (rc) return rc;
+    WT_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.n = s.S.n;
+    s.nnz = s.S.nnz;
+    if ((rc = dalloc(&s.rowptr, s.n + 1)) || (rc = dalloc(&s.colidx, s.nnz))) return rc;
+    WT_HIP(hipMemcpy(s.rowptr, hr.data(), (s.n + 1) * 4, hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpy(s.colidx, hc.data(), s.nnz * 4, hipMemcpyHostToDevice));
+    return upload_symbolic(s);
+}
+
+// Device copies of the symbolic data (front table, per-level lists, CSR
+// scatter map, permutation, update sets, child->parent maps).
+static int upload_symbolic(wt_solver_impl& s)
+{
+    int rc;
     const wt_symbolic& S = s.S;
 
     // device front table
@@ -297,21 +389,21 @@ static int alloc_numeric(wt_solver_impl& s, wt_numeric<R>& num)
     return WT_OK;
 }
 
+// Enqueues the whole numeric factorization on the solver stream.  When
+// gemm_ms is given, every GEMM launch is bracketed by events (ev, two per
+// launch index) for the autotuner.
 template <typename R>
-int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
+static int enqueue_factor(wt_solver_impl& s, wt_numeric<R>& num, std::vector<hipEvent_t>* ev)
 {
-    if (!s.have_vals) return WT_ERR_STATE;
-    int rc = alloc_numeric(s, num);
-    if (rc) return rc;
     hipStream_t st = s.stream;
     const wt_symbolic& S = s.S;
     wt_prof& P = s.prof;
     const double cs = sizeof(cpx<R>);
-    WT_HIP(hipMemsetAsync(num.status, 0, sizeof(int), st));
+    int rc;
+    if ((rc = wt_zero_i32(num.status, 1, st))) return rc;
     if ((rc = P.run(st, WT_PROF_FILL, 0, S.factor_len * cs,
                     [&] { return wt_fill_zero<R>(num.factor, S.factor_len, st); })))
         return rc;
-    if (!s.rscale && (rc = dalloc(&s.rscale, s.n))) return rc;
     if ((rc = P.run(st, WT_PROF_FILL, 0, s.nnz * 16.0 + s.n * 12.0,
                     [&] { return wt_row_scale(s.rowptr, s.vals, s.n, s.rscale, st); })))
         return rc;
@@ -336,19 +428,125 @@ int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
             });
             break;
         case WT_STEP_GEMM:
+            if (ev) (void)hipEventRecord((*ev)[2 * t.gemm], st);
             rc = P.run(st, num.fplan.launches[t.gemm].var ? WT_PROF_GEMM_FACTOR_NARROW : WT_PROF_GEMM_FACTOR,
                        num.fplan.launches[t.gemm].flops, 0, [&] { return num.fplan.run(t.gemm, st); });
+            if (ev) (void)hipEventRecord((*ev)[2 * t.gemm + 1], st);
             break;
         default: rc = WT_ERR_STATE;
         }
         if (rc) return rc;
     }
+    return WT_OK;
+}
+
+template <typename R>
+static int factor_prepare(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    if (!s.have_vals) return WT_ERR_STATE;
+    int rc = alloc_numeric(s, num);
+    if (rc) return rc;
+    if (!s.rscale && (rc = dalloc(&s.rscale, s.n))) return rc;
+    return WT_OK;
+}
+
+template <typename R>
+static int factor_finish(wt_solver_impl& s, wt_numeric<R>& num)
+{
     int hs = 0;
-    WT_HIP(hipMemcpyAsync(&hs, num.status, sizeof(int), hipMemcpyDeviceToHost, st));
-    WT_HIP(hipStreamSynchronize(st));
-    P.flush();
+    WT_HIP(hipMemcpyAsync(&hs, num.status, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    WT_HIP(hipStreamSynchronize(s.stream));
+    s.prof.flush();
     s.factored_prec = sizeof(R) == 4 ? 32 : 64;
     return hs ? WT_ERR_PIVOT : WT_OK;
+}
+
+template <typename R>
+int wt_solver_factor_launch(wt_solver_impl& s, wt_numeric<R>& num);
+
+// With s.use_graphs (and profiling off) the sequence is captured once into
+// a HIP graph and replayed: thousands of small launches per factorization
+// of a batch of small models become one graph launch.
+template <typename R>
+int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    int rc = wt_solver_factor_launch(s, num);
+    return rc ? rc : factor_finish(s, num);
+}
+
+template <typename R>
+int wt_solver_factor_wait(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    return factor_finish(s, num);
+}
+
+// Enqueues the factorization (graph or eager) without waiting for it.
+template <typename R>
+int wt_solver_factor_launch(wt_solver_impl& s, wt_numeric<R>& num)
+{
+    int rc = factor_prepare(s, num);
+    if (rc) return rc;
+    if (s.use_graphs && !s.prof.on) {
+        if (!num.fgraph) {
+            WT_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_factor(s, num, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s.stream, &g);
+            if (rc == WT_OK && e != hipSuccess) rc = WT_ERR_HIP;
+            if (rc == WT_OK && hipGraphInstantiate(&num.fgraph, g, nullptr, nullptr, 0) != hipSuccess) {
+                num.fgraph = nullptr;
+                rc = WT_ERR_HIP;
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+        }
+        WT_HIP(hipGraphLaunch(num.fgraph, s.stream));
+        return WT_OK;
+    }
+    return enqueue_factor(s, num, nullptr);
+}
+
+// Tile-shape autotuner: one full factorization per GEMM tile variant, each
+// launch timed with events; every launch then keeps its fastest variant.
+// Each run is a complete factorization (the variants compute identical
+// results), so the factor is valid afterwards.  *speedup: summed GEMM time
+// under the default choice over the summed time under the tuned choice.
+template <typename R>
+int wt_solver_autotune(wt_solver_impl& s, wt_numeric<R>& num, double* speedup)
+{
+    int rc = factor_prepare(s, num);
+    if (rc) return rc;
+    const size_t nl = num.fplan.launches.size();
+    std::vector<int> dflt(nl), best(nl);
+    std::vector<float> tbest(nl, INFINITY), tdef(nl, 0.f);
+    for (size_t i = 0; i < nl; ++i) dflt[i] = best[i] = num.fplan.launches[i].var;
+    std::vector<hipEvent_t> ev(2 * nl, nullptr);
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = WT_ERR_HIP;
+    const bool prof = s.prof.on;
+    s.prof.on = false;
+    for (int v = 0; v < WT_GEMM_NVAR && rc == WT_OK; ++v) {
+        for (auto& L : num.fplan.launches) L.var = v;
+        if ((rc = enqueue_factor(s, num, &ev)) || (rc = factor_finish(s, num))) break;
+        for (size_t i = 0; i < nl; ++i) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) != hipSuccess) continue;
+            if (v == dflt[i]) tdef[i] = t;
+            if (t < tbest[i]) { tbest[i] = t; best[i] = v; }
+        }
+    }
+    s.prof.on = prof;
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (size_t i = 0; i < nl; ++i) num.fplan.launches[i].var = rc == WT_OK ? best[i] : dflt[i];
+    num.drop_graphs();
+    if (rc == WT_OK && speedup) {
+        double a = 0, b = 0;
+        for (size_t i = 0; i < nl; ++i) { a += tdef[i]; b += std::isfinite(tbest[i]) ? tbest[i] : tdef[i]; }
+        *speedup = b > 0 ? a / b : 1.0;
+    }
+    num.tuned = rc == WT_OK;
+    return rc;
 }
 
 template <typename R>
@@ -357,6 +555,9 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
     const wt_symbolic& S = s.S;
     const int nl = (int)S.levels.size();
     const int64_t n = S.n, NB2 = WT_NB2;
+    if (num.sgraph) (void)hipGraphExecDestroy(num.sgraph);
+    num.sgraph = nullptr;
+    num.sgraph_nrhs = 0;
     dfree(num.y);
     dfree(num.w[0]);
     dfree(num.w[1]);
@@ -518,6 +719,12 @@ int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R
 
 template int wt_solver_factor_run<float>(wt_solver_impl&, wt_numeric<float>&);
 template int wt_solver_factor_run<double>(wt_solver_impl&, wt_numeric<double>&);
+template int wt_solver_factor_launch<float>(wt_solver_impl&, wt_numeric<float>&);
+template int wt_solver_factor_launch<double>(wt_solver_impl&, wt_numeric<double>&);
+template int wt_solver_factor_wait<float>(wt_solver_impl&, wt_numeric<float>&);
+template int wt_solver_factor_wait<double>(wt_solver_impl&, wt_numeric<double>&);
+template int wt_solver_autotune<float>(wt_solver_impl&, wt_numeric<float>&, double*);
+template int wt_solver_autotune<double>(wt_solver_impl&, wt_numeric<double>&, double*);
 template int wt_solver_prepare_solve<float>(wt_solver_impl&, wt_numeric<float>&, int);
 template int wt_solver_prepare_solve<double>(wt_solver_impl&, wt_numeric<double>&, int);
 template int wt_solver_solve_block<float>(wt_solver_impl&, wt_numeric<float>&, int, cpx<float>*);

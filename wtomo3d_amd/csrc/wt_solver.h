@@ -10,29 +10,35 @@
 #include "wt_symbolic.h"
 
 // One launch of the batched GEMM: a contiguous range of the plan's problems.
+// Tile counts are kept for every tile variant; var is the variant used (the
+// N <= 64 rule by default, or the autotuner's measured choice).
 struct wt_gemm_launch_rec {
-    int64_t prob_off, tile_off, ntiles;
+    int64_t prob_off, tile_off;
+    int64_t ntiles[WT_GEMM_NVAR];
     int32_t nprob;
-    int32_t var;   // tile variant (1 when every problem has N <= 64)
+    int32_t var;
     double flops;  // 8 M N K summed over the launch's problems
 };
 
 template <typename R>
 struct wt_gemm_plan {
     std::vector<wt_gemm_prob<R>> probs;
-    std::vector<int64_t> tiles;
+    std::vector<int64_t> tiles[WT_GEMM_NVAR];  // per-variant prefix sums, same indexing
     std::vector<wt_gemm_launch_rec> launches;
     wt_gemm_prob<R>* d_probs = nullptr;
-    int64_t* d_tiles = nullptr;
+    int64_t* d_tiles = nullptr;  // WT_GEMM_NVAR arrays of tiles[0].size(), variant-major
+    int64_t tstride = 0;
     wt_gemm_launch_rec cur{};
     void begin()
     {
         cur.prob_off = (int64_t)probs.size();
-        cur.tile_off = (int64_t)tiles.size();
-        cur.ntiles = 0;
+        cur.tile_off = (int64_t)tiles[0].size();
+        for (int v = 0; v < WT_GEMM_NVAR; ++v) {
+            cur.ntiles[v] = 0;
+            tiles[v].push_back(0);
+        }
         cur.nprob = 0;
         cur.flops = 0;
-        tiles.push_back(0);
     }
     void add(const cpx<R>* A, int64_t lda, const cpx<R>* B, int64_t ldb, cpx<R>* C, int64_t ldc, int64_t M,
              int64_t N, int64_t K, int mode, const int32_t* bperm = nullptr)
@@ -42,29 +48,23 @@ struct wt_gemm_plan {
         p.A = A; p.B = B; p.C = C; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
         p.M = (int32_t)M; p.N = (int32_t)N; p.K = (int32_t)K; p.mode = mode; p.bperm = bperm; p.pad = 0;
         probs.push_back(p);
-        cur.ntiles += wt_gemm_tiles<R>(M, N, 0);
+        for (int v = 0; v < WT_GEMM_NVAR; ++v) {
+            cur.ntiles[v] += wt_gemm_tiles<R>(M, N, v);
+            tiles[v].push_back(cur.ntiles[v]);
+        }
         cur.flops += 8.0 * (double)M * (double)N * (double)K;
-        tiles.push_back(cur.ntiles);
         ++cur.nprob;
     }
     // returns the launch index, or -1 when nothing was added
     int end()
     {
         if (cur.nprob == 0) {
-            tiles.pop_back();
+            for (int v = 0; v < WT_GEMM_NVAR; ++v) tiles[v].pop_back();
             return -1;
         }
         cur.var = 1;
         for (int64_t i = cur.prob_off; i < (int64_t)probs.size(); ++i)
             if (probs[i].N > 64) cur.var = 0;
-        if (cur.var) {  // recount tiles for the narrow variant
-            int64_t t = 0;
-            for (int64_t i = cur.prob_off, j = 1; i < (int64_t)probs.size(); ++i, ++j) {
-                t += wt_gemm_tiles<R>(probs[i].M, probs[i].N, 1);
-                tiles[cur.tile_off + j] = t;
-            }
-            cur.ntiles = t;
-        }
         launches.push_back(cur);
         return (int)launches.size() - 1;
     }
@@ -73,7 +73,8 @@ struct wt_gemm_plan {
     int run(int i, hipStream_t s) const
     {
         const wt_gemm_launch_rec& L = launches[i];
-        return wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.tile_off, L.nprob, L.ntiles, L.var, s);
+        return wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.var * tstride + L.tile_off, L.nprob,
+                                 L.ntiles[L.var], L.var, s);
     }
 };
 
@@ -109,6 +110,13 @@ struct wt_numeric {
     std::vector<wt_step> fwd, bwd;
     std::vector<int32_t> hsp;   // super-panel entries {front, K0, K1}
     int32_t* d_sp = nullptr;
+    // captured sequences: the factorization, and one refinement step of a
+    // batch solve (for sgraph_nrhs columns)
+    hipGraphExec_t fgraph = nullptr;
+    hipGraphExec_t sgraph = nullptr;
+    int sgraph_nrhs = 0;
+    bool tuned = false;  // GEMM tile variants chosen by measurement
+    void drop_graphs();
     void release();
 };
 
@@ -216,12 +224,28 @@ struct wt_solver_impl {
     wt_numeric<double> f64;
     int factored_prec = 0;
     double last_factor_ms = 0;
+    bool use_graphs = false;  // capture and replay the factorization
+};
+
+struct wt_grid_dims {
+    int nx, ny, nz;
 };
 
 int wt_solver_init(wt_solver_impl& s, int nx, int ny, int nz, int leaf_nodes);
+int wt_solver_init_union(wt_solver_impl& s, const std::vector<wt_grid_dims>& grids, int leaf_nodes);
+int wt_union_build(const std::vector<wt_grid_dims>& grids, int leaf_nodes, wt_symbolic& U,
+                   std::vector<int32_t>* hr, std::vector<int32_t>* hc);
 void wt_solver_free(wt_solver_impl& s);
 template <typename R>
 int wt_solver_factor_run(wt_solver_impl& s, wt_numeric<R>& num);
+template <typename R>
+int wt_solver_factor_launch(wt_solver_impl& s, wt_numeric<R>& num);
+template <typename R>
+int wt_solver_factor_wait(wt_solver_impl& s, wt_numeric<R>& num);
+template <typename R>
+int wt_solver_autotune(wt_solver_impl& s, wt_numeric<R>& num, double* speedup);
+// Refinement buffers (xb, rb, bb, partial, norms) for nrhs columns.
+int wt_solver_ir_buffers(wt_solver_impl& s, int nrhs);
 template <typename R>
 int wt_solver_prepare_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs);
 template <typename R>

@@ -231,3 +231,110 @@ int wt_symbolic_build(int nx, int ny, int nz, int leaf_nodes, int nb, const int3
     }
     return WT_OK;
 }
+
+int wt_pattern_host(int nx, int ny, int nz, int32_t row_base, int32_t col_base, int32_t* rowptr, int32_t* colidx)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || !rowptr || !colidx) return WT_ERR_ARG;
+    int64_t e = 0, row = 0;
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz) {
+                const int64_t g = iz + (int64_t)ix * nz + (int64_t)iy * nx * nz;
+                for (int r = 0; r < 3; ++r) {
+                    rowptr[row++] = (int32_t)(e + row_base);
+                    for (int oy = -1; oy <= 1; ++oy) {
+                        if (iy + oy < 0 || iy + oy >= ny) continue;
+                        for (int ox = -1; ox <= 1; ++ox) {
+                            if (ix + ox < 0 || ix + ox >= nx) continue;
+                            for (int oz = -1; oz <= 1; ++oz) {
+                                if (iz + oz < 0 || iz + oz >= nz) continue;
+                                const int64_t gn = g + oz + (int64_t)ox * nz + (int64_t)oy * nx * nz;
+                                for (int c = 0; c < 3; ++c) colidx[e++] = (int32_t)(3 * gn + c + col_base);
+                            }
+                        }
+                    }
+                }
+            }
+    rowptr[row] = (int32_t)(e + row_base);
+    return WT_OK;
+}
+
+int wt_symbolic_union(const std::vector<const wt_symbolic*>& parts, wt_symbolic& U)
+{
+    U = wt_symbolic();
+    if (parts.empty()) return WT_ERR_ARG;
+    U.nb = parts[0]->nb;
+    int maxlev = 0;
+    int64_t no = 0, nnzo = 0, fo = 0, to = 0, po = 0, uo = 0, mo = 0;
+    int32_t foff = 0;
+    size_t nf = 0, nu = 0, nm = 0;
+    for (const wt_symbolic* P : parts) {
+        if (P->nb != U.nb || (int64_t)P->dst.size() != P->nnz) return WT_ERR_ARG;
+        nf += P->fronts.size();
+        nu += P->upd_idx.size();
+        nm += P->relmap.size();
+        nnzo += P->nnz;
+        no += P->n;
+    }
+    U.fronts.reserve(nf);
+    U.upd_idx.reserve(nu);
+    U.relmap.reserve(nm);
+    U.dst.reserve((size_t)nnzo);
+    U.perm.reserve((size_t)no);
+    U.iperm.reserve((size_t)no);
+    no = nnzo = 0;
+    for (const wt_symbolic* P : parts) {
+        for (const wt_front& F0 : P->fronts) {
+            wt_front F = F0;
+            if (F.parent >= 0) F.parent += foff;
+            for (int k = 0; k < 2; ++k)
+                if (F.child[k] >= 0) F.child[k] += foff;
+            F.pstart += no;
+            F.uoff += uo;
+            F.moff += mo;
+            F.Loff += fo;
+            F.Uoff += fo;
+            F.tileoff += to;
+            F.pivoff += po;
+            maxlev = std::max(maxlev, F.level);
+            U.fronts.push_back(F);
+        }
+        for (int64_t v : P->perm) U.perm.push_back(v + no);
+        for (int64_t v : P->iperm) U.iperm.push_back(v + no);
+        for (int64_t v : P->upd_idx) U.upd_idx.push_back(v + no);
+        U.relmap.insert(U.relmap.end(), P->relmap.begin(), P->relmap.end());
+        for (int64_t v : P->dst) U.dst.push_back(v + fo);
+        no += P->n;
+        nnzo += P->nnz;
+        fo += P->factor_len;
+        to += P->tile_len;
+        po += P->piv_len;
+        uo += (int64_t)P->upd_idx.size();
+        mo += (int64_t)P->relmap.size();
+        foff += (int32_t)P->fronts.size();
+        U.max_p = std::max(U.max_p, P->max_p);
+        U.max_u = std::max(U.max_u, P->max_u);
+        U.flops += P->flops;
+    }
+    U.n = no;
+    U.nnz = nnzo;
+    U.factor_len = fo;
+    U.tile_len = to;
+    U.piv_len = po;
+    // levels of the forest, and the per-level contribution-block and
+    // solve-update offsets (the same layout rule as one tree)
+    U.levels.assign(maxlev + 1, {});
+    for (int f = 0; f < (int)U.fronts.size(); ++f) U.levels[U.fronts[f].level].push_back(f);
+    for (int l = 0; l <= maxlev; ++l) {
+        int64_t co = 0, wo = 0;
+        for (int f : U.levels[l]) {
+            U.fronts[f].CBoff = co;
+            co += U.fronts[f].u * U.fronts[f].u;
+            U.fronts[f].Woff = wo;
+            wo += U.fronts[f].u;
+        }
+        U.cb_len[l & 1] = std::max(U.cb_len[l & 1], co);
+        U.w_len[l & 1] = std::max(U.w_len[l & 1], wo);
+    }
+    return WT_OK;
+}

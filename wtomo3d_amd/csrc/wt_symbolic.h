@@ -56,3 +56,17 @@ struct wt_symbolic {
 // pattern (0-based) of the grid; dst is filled when they are given.
 int wt_symbolic_build(int nx, int ny, int nz, int leaf_nodes, int nb, const int32_t* rowptr,
                       const int32_t* colidx, wt_symbolic& S);
+
+// The CSR pattern of one grid on the host (same as the device pattern
+// kernel): rows in y > x > z > component order, columns ascending, with
+// row_base added to every row pointer and col_base to every column index.
+// rowptr has 3 nx ny nz + 1 entries.
+int wt_pattern_host(int nx, int ny, int nz, int32_t row_base, int32_t col_base, int32_t* rowptr,
+                    int32_t* colidx);
+
+// Symbolic data of the block-diagonal union of independent systems: the
+// forest of their elimination trees, levels aligned by depth from each root,
+// unknowns, CSR entries and factor storage concatenated in part order (part
+// k's unknowns start at the sum of the previous parts' n).  Every part must
+// have been built with its CSR pattern (dst filled) and the same nb.
+int wt_symbolic_union(const std::vector<const wt_symbolic*>& parts, wt_symbolic& U);

@@ -24,3 +24,11 @@ int wt_convert_c128_c64(const cpx<double>* a, cpx<float>* b, int64_t n, hipStrea
 int wt_widen_c64(const cpx<float>* a, cpx<double>* b, int64_t n, hipStream_t s);
 int wt_planewave(cpx<double>* x, int nx, int ny, int nz, double dx, double kmag, const double* d_dirs, int nrhs,
                  hipStream_t s);
+// Per-(segment, column) squared norms of a row-major n x nrhs block whose
+// rows are cut at d_seg_off[0..nseg] (device): out[g * nrhs + c];
+// partial holds nseg * nblk * nrhs doubles.
+int wt_seg_colnorm2(const cpx<double>* v, const int64_t* d_seg_off, int nseg, int nrhs, double* partial, int nblk,
+                    double* out, hipStream_t s);
+// Zeroes columns c >= d_ncol[g] in the rows of every segment g.
+int wt_seg_mask_cols(cpx<double>* v, const int64_t* d_seg_off, const int32_t* d_ncol, int nseg, int64_t n, int nrhs,
+                     hipStream_t s);

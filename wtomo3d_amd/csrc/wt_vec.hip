@@ -200,6 +200,58 @@ __global__ void planewave_kernel(cpx<double>* __restrict__ x, int nx, int ny, in
     }
 }
 
+// Per-(segment, column) |v|^2 of a row-major block whose rows are cut into
+// contiguous segments (the unknowns of the independent systems of a batch):
+// block (x, c, g) sums rows of segment g in its x-th share.
+__global__ __launch_bounds__(256) void seg_colnorm_kernel(const cpx<double>* __restrict__ v,
+                                                          const int64_t* __restrict__ seg_off, int nrhs,
+                                                          double* __restrict__ partial)
+{
+    __shared__ double s[256];
+    const int c = blockIdx.y, g = blockIdx.z;
+    const int64_t a = seg_off[g], n = seg_off[g + 1] - a;
+    const int64_t rows_per_blk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = a + (int64_t)blockIdx.x * rows_per_blk;
+    const int64_t r1 = r0 + rows_per_blk < a + n ? r0 + rows_per_blk : a + n;
+    double acc = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+        cpx<double> x = v[i * nrhs + c];
+        acc += x.re * x.re + x.im * x.im;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[((int64_t)g * gridDim.x + blockIdx.x) * nrhs + c] = s[0];
+}
+
+__global__ void seg_reduce_kernel(const double* __restrict__ partial, int nblk, int nrhs, double* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+    if (c >= nrhs) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += partial[((int64_t)g * nblk + b) * nrhs + c];
+    out[(int64_t)g * nrhs + c] = acc;
+}
+
+// Zeroes the columns c >= ncol[g] of the rows of segment g.
+__global__ void seg_mask_kernel(cpx<double>* __restrict__ v, const int64_t* __restrict__ seg_off,
+                                const int32_t* __restrict__ ncol, int nseg, int nrhs)
+{
+    const int64_t n = seg_off[nseg];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (seg_off[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        for (int c = ncol[lo]; c < nrhs; ++c) v[i * nrhs + c] = cpx<double>(0.0, 0.0);
+    }
+}
+
 }  // namespace
 
 static unsigned grid1(int64_t n)
@@ -302,3 +354,24 @@ template int wt_residual<float>(const int32_t*, const int32_t*, const cpx<float>
                                 const cpx<double>*, cpx<double>*, double*, double*, int64_t, int, hipStream_t);
 template int wt_copy<double>(const cpx<double>*, cpx<double>*, int64_t, hipStream_t);
 template int wt_copy<float>(const cpx<float>*, cpx<float>*, int64_t, hipStream_t);
+
+int wt_seg_colnorm2(const cpx<double>* v, const int64_t* d_seg_off, int nseg, int nrhs, double* partial, int nblk,
+                    double* out, hipStream_t s)
+{
+    if (nseg < 1 || nrhs < 1 || nblk < 1 || nseg > 65535 || nrhs > 65535) return WT_ERR_ARG;
+    hipLaunchKernelGGL(seg_colnorm_kernel, dim3((unsigned)nblk, nrhs, nseg), dim3(256), 0, s, v, d_seg_off, nrhs,
+                       partial);
+    WT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(seg_reduce_kernel, dim3((nrhs + 63) / 64, nseg), dim3(64), 0, s, partial, nblk, nrhs, out);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
+int wt_seg_mask_cols(cpx<double>* v, const int64_t* d_seg_off, const int32_t* d_ncol, int nseg, int64_t n, int nrhs,
+                     hipStream_t s)
+{
+    if (nseg < 1 || n <= 0) return WT_OK;
+    hipLaunchKernelGGL(seg_mask_kernel, dim3(grid1(n)), dim3(256), 0, s, v, d_seg_off, d_ncol, nseg, nrhs);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
