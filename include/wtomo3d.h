@@ -318,8 +318,12 @@ typedef struct wt_batch wt_batch;
 
 /* dims: nmodels triples (nx, ny, nz).  Models are grouped, in order, so that
  * each group's CSR has at most max_group_nnz entries (<= 0: 2^31 - 1, the
- * 32-bit row-pointer limit); every group is factored on its own stream. */
-int wt_batch_create(int nmodels, const int32_t *dims, int leaf_nodes, int64_t max_group_nnz, wt_batch **out);
+ * 32-bit row-pointer limit) and, when max_group_bytes > 0, its numeric state
+ * (complex64 factor, contribution blocks, 16-column solve and refinement
+ * blocks, from the symbolic analysis) at most max_group_bytes; every group
+ * is factored on its own stream. */
+int wt_batch_create(int nmodels, const int32_t *dims, int leaf_nodes, int64_t max_group_nnz, int64_t max_group_bytes,
+                    wt_batch **out);
 int wt_batch_destroy(wt_batch *b);
 /* info[8]: unknowns, CSR entries, fronts, levels (deepest tree), largest
  * front pivots, groups, factor entries, contribution-block entries;
@@ -350,6 +354,12 @@ int wt_batch_symbolic_info(int nmodels, const int32_t *dims, int leaf_nodes, int
 /* The CSR pattern of SREL3D + SPARSE_ADJ2CRS (srgraphs.f:638-734,
  * sparse.f:13-49) computed on the host (wt_graph computes it on the GPU). */
 int wt_graph_host(int nx, int ny, int nz, int base, int32_t *rowptr, int32_t *colidx);
+/* Factor and solve group after group (arguments as wt_batch_solve),
+ * releasing each group's factors before the next: only one group's numeric
+ * state is resident, so a batch larger than device memory runs with groups
+ * bounded by max_group_bytes. */
+int wt_batch_run(wt_batch *b, int factor_prec, int nrhs, const int32_t *ncol, const void *rhs, void *x,
+                 int on_device, double tol, int max_iter, wt_solve_stats *stats);
 /* y = A x with every model's FP64 CSR values; x, y row-major (total
  * unknowns) x nrhs, nrhs <= 64. */
 int wt_batch_apply(wt_batch *b, int nrhs, const void *x, void *y, int on_device);

@@ -78,4 +78,4 @@ def test_small_models_workload_shape():
 def test_batch_rejects_bad_dims():
     h = ctypes.c_void_p()
     d = np.array([4, 4, 1], np.int32)
-    assert W.lib().wt_batch_create(1, P(d), 64, 0, ctypes.byref(h)) != 0
+    assert W.lib().wt_batch_create(1, P(d), 64, 0, 0, ctypes.byref(h)) != 0

@@ -147,3 +147,21 @@ def test_batch_ignores_columns_beyond_each_count():
         assert np.all(xk[:, ncol[k]:] == 0)
         assert _rel(xk[:, :ncol[k]], U[k][:, :ncol[k]]) <= 1e-8
         assert st[k].nrhs == ncol[k]
+
+
+def test_batch_run_releases_groups_and_matches_solve():
+    """Memory-bounded groups run one after the other (wt_batch_run): every
+    model's solution equals the resident-factor solve of the same batch."""
+    mods = _setup(seed=81)
+    with _batch(mods) as b1:
+        U = _fields(mods, NCOLS, seed=23)
+        B = b1.apply(U)
+        b1.factor(32)
+        X1, _ = b1.solve(B, prec=32, tol=1e-11, max_iter=30)
+    with _batch(mods, max_group_bytes=1) as b2:  # one model per group
+        assert b2.info()["groups"] == len(mods)
+        X2, st = b2.run(B, prec=32, tol=1e-11, max_iter=30)
+        X3, _ = b2.run(B, prec=32, tol=1e-11, max_iter=30)  # factors rebuilt
+    for k in range(len(mods)):
+        assert st[k]["rel_residual"] <= 1e-11
+        assert _rel(X2[k], X1[k]) <= 1e-10 and np.array_equal(X2[k], X3[k])

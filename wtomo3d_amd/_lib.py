@@ -75,7 +75,7 @@ def _declare(L):
                                                ctypes.POINTER(ctypes.c_double)]
     L.wt_gemm_host_var.restype = ctypes.c_int
     pstats = ctypes.POINTER(WtSolveStats)
-    L.wt_batch_create.argtypes = [i32, _vp, i32, i64, ctypes.POINTER(ctypes.c_void_p)]
+    L.wt_batch_create.argtypes = [i32, _vp, i32, i64, i64, ctypes.POINTER(ctypes.c_void_p)]
     L.wt_batch_destroy.argtypes = [_vp]
     L.wt_batch_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.wt_batch_offsets.argtypes = [_vp, _vp, _vp]
@@ -86,12 +86,13 @@ def _declare(L):
     L.wt_batch_autotune.argtypes = [_vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.wt_batch_solve.argtypes = [_vp, i32, i32, _vp, _vp, _vp, i32, ctypes.c_double, i32, pstats]
     L.wt_batch_apply.argtypes = [_vp, i32, _vp, _vp, i32]
+    L.wt_batch_run.argtypes = [_vp, i32, i32, _vp, _vp, _vp, i32, ctypes.c_double, i32, pstats]
     L.wt_batch_symbolic_info.argtypes = [i32, _vp, i32, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_double)]
     L.wt_graph_host.argtypes = [i32] * 4 + [_vp, _vp]
     for name in ("wt_batch_create", "wt_batch_destroy", "wt_batch_info", "wt_batch_offsets", "wt_batch_set_graphs",
                  "wt_batch_set_models", "wt_batch_assemble", "wt_batch_factor", "wt_batch_autotune",
-                 "wt_batch_solve", "wt_batch_apply", "wt_batch_symbolic_info", "wt_graph_host"):
+                 "wt_batch_solve", "wt_batch_apply", "wt_batch_symbolic_info", "wt_graph_host", "wt_batch_run"):
         getattr(L, name).restype = ctypes.c_int
     L.wt_solver_planewaves.argtypes = [_vp, i32, _vp, ctypes.c_double, ctypes.c_double, _vp]
     L.wt_solver_planewaves.restype = ctypes.c_int

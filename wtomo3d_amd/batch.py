@@ -23,15 +23,16 @@ def _p(a):
 
 class Batch:
     """dims: sequence of (nx, ny, nz).  max_group_nnz > 0 caps the CSR
-    entries of one group (a block-diagonal system on its own stream)."""
+    entries of one group (a block-diagonal system on its own stream);
+    max_group_bytes > 0 caps a group's numeric state (see ``run``)."""
 
-    def __init__(self, dims, leaf_nodes=64, max_group_nnz=0):
+    def __init__(self, dims, leaf_nodes=64, max_group_nnz=0, max_group_bytes=0):
         d = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 3))
         self.dims = [tuple(int(v) for v in r) for r in d]
         self.m = len(self.dims)
         h = ctypes.c_void_p()
-        check(lib().wt_batch_create(self.m, _p(d), int(leaf_nodes), int(max_group_nnz), ctypes.byref(h)),
-              "wt_batch_create")
+        check(lib().wt_batch_create(self.m, _p(d), int(leaf_nodes), int(max_group_nnz), int(max_group_bytes),
+                                    ctypes.byref(h)), "wt_batch_create")
         self._h = h
         self.offsets = np.zeros(self.m + 1, np.int64)
         self.group_of = np.zeros(self.m, np.int32)
@@ -118,6 +119,17 @@ class Batch:
         st = (WtSolveStats * self.m)()
         check(lib().wt_batch_solve(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st),
               "wt_batch_solve")
+        stats = [{k: getattr(s, k) for k, _ in WtSolveStats._fields_} for s in st]
+        return self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs]), stats
+
+    def run(self, bs, prec=32, tol=1e-10, max_iter=30):
+        """Factor and solve group after group, releasing each group's factors
+        before the next (``solve``'s arguments and results)."""
+        B, cols, nrhs = self._pack(bs)
+        X = np.zeros_like(B)
+        st = (WtSolveStats * self.m)()
+        check(lib().wt_batch_run(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st),
+              "wt_batch_run")
         stats = [{k: getattr(s, k) for k, _ in WtSolveStats._fields_} for s in st]
         return self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs]), stats
 

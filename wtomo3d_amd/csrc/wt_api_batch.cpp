@@ -62,6 +62,14 @@ void free_group(group* g)
     delete g;
 }
 
+// Device bytes of the numeric state of a symbolic analysis: factor,
+// contribution blocks, inverse tiles, pivots, solve and refinement blocks.
+int64_t numeric_bytes(const wt_symbolic& S, int64_t cs, int64_t nrhs)
+{
+    return (S.factor_len + S.cb_len[0] + S.cb_len[1] + S.tile_len) * cs + S.piv_len * 4 +
+           (S.n + S.w_len[0] + S.w_len[1]) * nrhs * cs + 3 * S.n * nrhs * 16;
+}
+
 int seg_buffers(group& g, int nrhs)
 {
     if (g.seg_cap >= nrhs) return WT_OK;
@@ -171,6 +179,37 @@ int refine_group(group& g, wt_numeric<R>& num, int nrhs, const int32_t* ncol, do
 
 }  // namespace
 
+// Numeric buffers shared by the groups of wt_batch_run (one group resident
+// at a time), grown to the largest group's needs and kept across runs.
+struct num_pool {
+    static constexpr int NB = 9;  // factor, cb0, cb1, tiles, perm, status, y, w0, w1
+    void* p[NB] = {};
+    int64_t cap[NB] = {};
+    int ensure(const int64_t* need)
+    {
+        for (int i = 0; i < NB; ++i) {
+            if (cap[i] >= need[i] && p[i]) continue;
+            (void)hipFree(p[i]);
+            p[i] = nullptr;
+            cap[i] = 0;
+            if (hipMalloc(&p[i], (size_t)(need[i] > 0 ? need[i] : 16)) != hipSuccess) {
+                p[i] = nullptr;
+                return WT_ERR_NOMEM;
+            }
+            cap[i] = need[i] > 0 ? need[i] : 16;
+        }
+        return WT_OK;
+    }
+    void release()
+    {
+        for (int i = 0; i < NB; ++i) {
+            (void)hipFree(p[i]);
+            p[i] = nullptr;
+            cap[i] = 0;
+        }
+    }
+};
+
 struct wt_batch {
     int nmodels = 0;
     std::vector<wt_grid_dims> dims;
@@ -178,7 +217,79 @@ struct wt_batch {
     std::vector<group*> groups;
     std::vector<int> group_of;
     bool have_model = false;
+    num_pool pool;
 };
+
+namespace {
+template <typename R>
+wt_numeric<R>& numeric_of(wt_solver_impl& s);
+template <>
+wt_numeric<float>& numeric_of<float>(wt_solver_impl& s) { return s.f32; }
+template <>
+wt_numeric<double>& numeric_of<double>(wt_solver_impl& s) { return s.f64; }
+
+// Points a group's numeric state at the shared pool.
+template <typename R>
+void lend(num_pool& P, wt_numeric<R>& num)
+{
+    if (num.ready && !num.ext) num.release();
+    num.ext = true;
+    num.factor = (cpx<R>*)P.p[0];
+    num.cb[0] = (cpx<R>*)P.p[1];
+    num.cb[1] = (cpx<R>*)P.p[2];
+    num.tiles = (cpx<R>*)P.p[3];
+    num.perm = (int32_t*)P.p[4];
+    num.status = (int*)P.p[5];
+    num.y = (cpx<R>*)P.p[6];
+    num.w[0] = (cpx<R>*)P.p[7];
+    num.w[1] = (cpx<R>*)P.p[8];
+}
+
+template <typename R>
+int run_groups(wt_batch* b, int nrhs, const int32_t* ncol, const void* rhs, void* x, int on_device, double tol,
+               int max_iter, wt_solve_stats* stats)
+{
+    const int64_t cs = sizeof(cpx<R>);
+    int64_t need[num_pool::NB] = {};
+    for (group* g : b->groups) {
+        const wt_symbolic& S = g->s.S;
+        const int64_t v[num_pool::NB] = {S.factor_len * cs, S.cb_len[0] * cs, S.cb_len[1] * cs, S.tile_len * cs,
+                                         S.piv_len * 4, 4, S.n * nrhs * cs, S.w_len[0] * nrhs * cs,
+                                         S.w_len[1] * nrhs * cs};
+        for (int i = 0; i < num_pool::NB; ++i) need[i] = need[i] > v[i] ? need[i] : v[i];
+    }
+    int rc = b->pool.ensure(need);
+    if (rc) return rc;
+    const hipMemcpyKind in = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind outk = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    int result = WT_OK;
+    for (group* g : b->groups) {
+        wt_solver_impl& s = g->s;
+        wt_numeric<R>& num = numeric_of<R>(s);
+        if (!s.have_vals) return WT_ERR_STATE;
+        lend(b->pool, num);
+        rc = wt_solver_factor_run<R>(s, num);
+        if (rc == WT_OK) rc = wt_solver_ir_buffers(s, nrhs);
+        if (rc == WT_OK) {
+            const int64_t row0 = b->unk_off[g->m0];
+            WT_HIP(hipMemcpyAsync(s.bb, (const cpx<double>*)rhs + row0 * nrhs, s.n * nrhs * 16, in, s.stream));
+            WT_HIP(hipMemcpyAsync(g->d_ncol, ncol + g->m0, g->nm() * 4, hipMemcpyHostToDevice, s.stream));
+            rc = wt_seg_mask_cols(s.bb, g->d_seg, g->d_ncol, g->nm(), s.n, nrhs, s.stream);
+            if (rc == WT_OK) rc = refine_group<R>(*g, num, nrhs, ncol + g->m0, tol, max_iter, stats + g->m0);
+            if (rc == WT_OK || rc == WT_ERR_PIVOT) {
+                WT_HIP(hipMemcpyAsync((cpx<double>*)x + row0 * nrhs, s.xb, s.n * nrhs * 16, outk, s.stream));
+                WT_HIP(hipStreamSynchronize(s.stream));
+            }
+        }
+        // the pool passes to the next group
+        WT_HIP(hipStreamSynchronize(s.stream));
+        num.release();
+        if (rc == WT_ERR_PIVOT) result = rc;
+        else if (rc) return rc;
+    }
+    return result;
+}
+}  // namespace
 
 extern "C" {
 
@@ -186,16 +297,19 @@ int wt_batch_destroy(wt_batch* b)
 {
     if (!b) return WT_OK;
     for (group* g : b->groups) free_group(g);
+    b->pool.release();
     delete b;
     return WT_OK;
 }
 
-int wt_batch_create(int nmodels, const int32_t* dims, int leaf_nodes, int64_t max_group_nnz, wt_batch** out)
+int wt_batch_create(int nmodels, const int32_t* dims, int leaf_nodes, int64_t max_group_nnz, int64_t max_group_bytes,
+                    wt_batch** out)
 {
     if (!out || !dims || nmodels < 1) return WT_ERR_ARG;
     *out = nullptr;
     const int64_t cap = (1LL << 31) - 1;
     if (max_group_nnz <= 0 || max_group_nnz > cap) max_group_nnz = cap;
+    if (leaf_nodes <= 0) leaf_nodes = 64;
     wt_batch* b = new (std::nothrow) wt_batch();
     if (!b) return WT_ERR_NOMEM;
     b->nmodels = nmodels;
@@ -212,6 +326,20 @@ int wt_batch_create(int nmodels, const int32_t* dims, int leaf_nodes, int64_t ma
         b->node_off[k + 1] = b->node_off[k] + (int64_t)d.nx * d.ny * d.nz;
         b->unk_off[k + 1] = 3 * b->node_off[k + 1];
     }
+    // numeric-state bytes of each model (complex64 factor, 16 right-hand
+    // sides) when groups are bounded by memory
+    std::vector<int64_t> mb(nmodels, 0);
+    if (max_group_bytes > 0)
+        for (int k = 0; k < nmodels; ++k) {
+            wt_symbolic S;
+            int rc = wt_symbolic_build(b->dims[k].nx, b->dims[k].ny, b->dims[k].nz, leaf_nodes, WT_NB, nullptr,
+                                       nullptr, S);
+            if (rc) {
+                delete b;
+                return rc;
+            }
+            mb[k] = numeric_bytes(S, 8, 16);
+        }
     // greedy groups of consecutive models
     int rc = WT_OK;
     for (int k = 0; k < nmodels && rc == WT_OK;) {
@@ -219,11 +347,13 @@ int wt_batch_create(int nmodels, const int32_t* dims, int leaf_nodes, int64_t ma
         if (!g) { rc = WT_ERR_NOMEM; break; }
         b->groups.push_back(g);
         g->m0 = k;
-        int64_t e = 0;
+        int64_t e = 0, by = 0;
         while (k < nmodels) {
             const int64_t ek = wt_nnz(b->dims[k].nx, b->dims[k].ny, b->dims[k].nz);
-            if (k > g->m0 && e + ek > max_group_nnz) break;
+            if (k > g->m0 && (e + ek > max_group_nnz || (max_group_bytes > 0 && by + mb[k] > max_group_bytes)))
+                break;
             e += ek;
+            by += mb[k];
             b->group_of[k] = (int)b->groups.size() - 1;
             ++k;
         }
@@ -238,7 +368,7 @@ int wt_batch_create(int nmodels, const int32_t* dims, int leaf_nodes, int64_t ma
             g->unk_off[i + 1] = 3 * g->node_off[i + 1];
             g->nnz_off[i + 1] = g->nnz_off[i] + wt_nnz(gd[i].nx, gd[i].ny, gd[i].nz);
         }
-        if ((rc = wt_solver_init_union(g->s, gd, leaf_nodes > 0 ? leaf_nodes : 64))) break;
+        if ((rc = wt_solver_init_union(g->s, gd, leaf_nodes))) break;
         if ((rc = dalloc(&g->d_seg, g->nm() + 1)) || (rc = dalloc(&g->d_ncol, g->nm()))) break;
         if (hipMemcpy(g->d_seg, g->unk_off.data(), (g->nm() + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
             rc = WT_ERR_HIP;
@@ -399,6 +529,17 @@ int wt_batch_symbolic_info(int nmodels, const int32_t* dims, int leaf_nodes, int
 int wt_graph_host(int nx, int ny, int nz, int base, int32_t* rowptr, int32_t* colidx)
 {
     return wt_pattern_host(nx, ny, nz, base, base, rowptr, colidx);
+}
+
+int wt_batch_run(wt_batch* b, int factor_prec, int nrhs, const int32_t* ncol, const void* rhs, void* x,
+                 int on_device, double tol, int max_iter, wt_solve_stats* stats)
+{
+    if (!b || !ncol || !rhs || !x || !stats || nrhs < 1 || nrhs > 64 || max_iter < 1) return WT_ERR_ARG;
+    if (factor_prec != WT_PREC_F32 && factor_prec != WT_PREC_F64) return WT_ERR_ARG;
+    for (int k = 0; k < b->nmodels; ++k)
+        if (ncol[k] < 0 || ncol[k] > nrhs) return WT_ERR_ARG;
+    return factor_prec == WT_PREC_F32 ? run_groups<float>(b, nrhs, ncol, rhs, x, on_device, tol, max_iter, stats)
+                                      : run_groups<double>(b, nrhs, ncol, rhs, x, on_device, tol, max_iter, stats);
 }
 
 int wt_batch_apply(wt_batch* b, int nrhs, const void* x, void* y, int on_device)

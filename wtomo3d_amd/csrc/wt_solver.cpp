@@ -78,6 +78,12 @@ void wt_numeric<R>::release()
 {
     drop_graphs();
     tuned = false;
+    if (ext) {
+        factor = cb[0] = cb[1] = tiles = y = w[0] = w[1] = nullptr;
+        perm = nullptr;
+        status = nullptr;
+        ext = false;
+    }
     dfree(factor);
     dfree(cb[0]);
     dfree(cb[1]);
@@ -375,9 +381,11 @@ static int alloc_numeric(wt_solver_impl& s, wt_numeric<R>& num)
     if (num.ready) return WT_OK;
     const wt_symbolic& S = s.S;
     int rc;
-    if ((rc = dalloc(&num.factor, S.factor_len)) || (rc = dalloc(&num.cb[0], S.cb_len[0])) ||
-        (rc = dalloc(&num.cb[1], S.cb_len[1])) || (rc = dalloc(&num.tiles, S.tile_len)) ||
-        (rc = dalloc(&num.perm, S.piv_len)) || (rc = dalloc(&num.status, 1))) {
+    if (num.ext) {
+        if (!num.factor || !num.cb[0] || !num.cb[1] || !num.tiles || !num.perm || !num.status) return WT_ERR_STATE;
+    } else if ((rc = dalloc(&num.factor, S.factor_len)) || (rc = dalloc(&num.cb[0], S.cb_len[0])) ||
+               (rc = dalloc(&num.cb[1], S.cb_len[1])) || (rc = dalloc(&num.tiles, S.tile_len)) ||
+               (rc = dalloc(&num.perm, S.piv_len)) || (rc = dalloc(&num.status, 1))) {
         num.release();
         return rc;
     }
@@ -558,18 +566,23 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
     if (num.sgraph) (void)hipGraphExecDestroy(num.sgraph);
     num.sgraph = nullptr;
     num.sgraph_nrhs = 0;
-    dfree(num.y);
-    dfree(num.w[0]);
-    dfree(num.w[1]);
+    if (!num.ext) {
+        dfree(num.y);
+        dfree(num.w[0]);
+        dfree(num.w[1]);
+    }
     dfree(num.d_sp);
     num.hsp.clear();
     num.splan.release();
     num.fwd.clear();
     num.bwd.clear();
     int rc;
-    if ((rc = dalloc(&num.y, n * nrhs)) || (rc = dalloc(&num.w[0], S.w_len[0] * nrhs)) ||
-        (rc = dalloc(&num.w[1], S.w_len[1] * nrhs)))
+    if (num.ext) {
+        if (!num.y || !num.w[0] || !num.w[1]) return WT_ERR_STATE;
+    } else if ((rc = dalloc(&num.y, n * nrhs)) || (rc = dalloc(&num.w[0], S.w_len[0] * nrhs)) ||
+               (rc = dalloc(&num.w[1], S.w_len[1] * nrhs))) {
         return rc;
+    }
     cpx<R>* y = num.y;
     auto nsp = [&](const wt_front& F) { return (F.p + NB2 - 1) / NB2; };
     // forward substitution, bottom-up; per super-panel: the panel's tiles in

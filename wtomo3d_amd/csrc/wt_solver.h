@@ -116,6 +116,10 @@ struct wt_numeric {
     hipGraphExec_t sgraph = nullptr;
     int sgraph_nrhs = 0;
     bool tuned = false;  // GEMM tile variants chosen by measurement
+    // factor, cb, tiles, perm, status, y and w are the caller's buffers
+    // (a pool shared by the groups of a batch run): never allocated or freed
+    // here, and y, w must hold the solve blocks of the largest nrhs used
+    bool ext = false;
     void drop_graphs();
     void release();
 };
