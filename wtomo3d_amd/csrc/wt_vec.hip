@@ -73,7 +73,13 @@ __global__ __launch_bounds__(256) void from_elim_kernel(const cpx<R>* __restrict
 // Per-block partial sums of |r|^2 per column go to `partial`.
 constexpr int SP_ROWS = 4;  // rows per block (one per wave)
 
-template <typename Ra>
+// r = b - A x (or A x when b is null) for a row-major block of nrhs <= CW
+// columns: each wave handles 64 / CW rows, CW lanes per row (one per
+// column), so few right-hand sides still fill the wave.  Every row of the
+// elastic CSR holds its entries in triplets (the three components of one
+// neighbour node, consecutive columns), so one column index serves three
+// entries; the terms are added in the same order as one entry at a time.
+template <typename Ra, int CW>
 __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ colidx,
                                                        const cpx<Ra>* __restrict__ vals,
@@ -81,37 +87,48 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
                                                        const cpx<double>* __restrict__ b, cpx<double>* __restrict__ r,
                                                        double* __restrict__ partial, int64_t n, int nrhs)
 {
-    __shared__ double s_part[SP_ROWS][64];
-    const int wave = threadIdx.x >> 6, c = threadIdx.x & 63;
+    constexpr int RPW = 64 / CW;
+    __shared__ double s_part[SP_ROWS * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, sub = lane / CW, c = lane % CW;
     // each of the 8 XCDs gets a contiguous range of rows, so the x rows of
     // 27-point neighbours are shared through its L2
     int64_t blk = blockIdx.x;
     {
-        const int64_t nb = gridDim.x, q = nb / 8, r = nb % 8, x = blk % 8, i = blk / 8;
-        blk = x * q + (x < r ? x : r) + i;
+        const int64_t nb = gridDim.x, q = nb / 8, rr = nb % 8, xx = blk % 8, i = blk / 8;
+        blk = xx * q + (xx < rr ? xx : rr) + i;
     }
-    const int64_t row = blk * SP_ROWS + wave;
+    const int64_t row = (blk * SP_ROWS + wave) * RPW + sub;
     double nrm = 0.0;
     if (row < n && c < nrhs) {
-        // b == null: r = A x instead of b - A x
         cpx<double> acc = b ? b[row * nrhs + c] : cpx<double>(0.0, 0.0);
         const double sg = b ? -1.0 : 1.0;
-        const int32_t e1 = rowptr[row + 1];
-        for (int32_t e = rowptr[row]; e < e1; ++e) {
-            const cpx<Ra> a = vals[e];
-            const cpx<double> xv = x[(int64_t)colidx[e] * nrhs + c];
+        auto term = [&](const cpx<Ra> a, const cpx<double> xv) {
             acc.re += sg * ((double)a.re * xv.re - (double)a.im * xv.im);
             acc.im += sg * ((double)a.re * xv.im + (double)a.im * xv.re);
+        };
+        const int32_t e0 = rowptr[row], e1 = rowptr[row + 1];
+        if ((e1 - e0) % 3 == 0) {
+            for (int32_t e = e0; e < e1; e += 3) {
+                const cpx<double>* xp = x + (int64_t)colidx[e] * nrhs + c;
+                const cpx<Ra> a0 = vals[e], a1 = vals[e + 1], a2 = vals[e + 2];
+                const cpx<double> x0 = xp[0], x1 = xp[nrhs], x2 = xp[2 * nrhs];
+                term(a0, x0);
+                term(a1, x1);
+                term(a2, x2);
+            }
+        } else {
+            for (int32_t e = e0; e < e1; ++e) term(vals[e], x[(int64_t)colidx[e] * nrhs + c]);
         }
         r[row * nrhs + c] = acc;
         nrm = acc.re * acc.re + acc.im * acc.im;
     }
-    s_part[wave][c] = nrm;
+    s_part[threadIdx.x] = nrm;
     __syncthreads();
-    if (wave == 0 && c < nrhs) {
-        double s = 0.0;
-        for (int w = 0; w < SP_ROWS; ++w) s += s_part[w][c];
-        partial[blk * nrhs + c] = s;
+    if ((int)threadIdx.x < nrhs) {
+        double t = 0.0;
+        for (int w = 0; w < SP_ROWS; ++w)
+            for (int k = 0; k < RPW; ++k) t += s_part[w * 64 + k * CW + threadIdx.x];
+        partial[blk * nrhs + threadIdx.x] = t;
     }
 }
 
@@ -287,10 +304,20 @@ int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* val
                 const cpx<double>* b, cpx<double>* r, double* partial, double* rnorm2, int64_t n, int nrhs,
                 hipStream_t s)
 {
-    if (nrhs > 64) return WT_ERR_ARG;  // one wave per row covers up to 64 columns
-    int64_t nb = wt_residual_blocks(n);
-    hipLaunchKernelGGL(residual_kernel<Ra>, dim3((unsigned)nb), dim3(256), 0, s, rowptr, colidx, vals, x, b, r,
-                       partial, n, nrhs);
+    if (nrhs < 1 || nrhs > 64) return WT_ERR_ARG;  // one wave per row covers up to 64 columns
+    int cw = 1;
+    while (cw < nrhs) cw <<= 1;
+    const int64_t rows_per_blk = SP_ROWS * (64 / cw);
+    const int64_t nb = (n + rows_per_blk - 1) / rows_per_blk;  // <= wt_residual_blocks(n)
+    switch (cw) {
+#define WT_RES(CW)                                                                                              \
+    case CW:                                                                                                    \
+        hipLaunchKernelGGL((residual_kernel<Ra, CW>), dim3((unsigned)nb), dim3(256), 0, s, rowptr, colidx, vals, \
+                           x, b, r, partial, n, nrhs);                                                          \
+        break;
+        WT_RES(1) WT_RES(2) WT_RES(4) WT_RES(8) WT_RES(16) WT_RES(32) WT_RES(64)
+#undef WT_RES
+    }
     WT_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
     WT_HIP(hipGetLastError());
