@@ -16,6 +16,11 @@ Right-hand sides: b_j = A u_j with u_j a homogeneous P plane wave (incidence
 10-35 deg, azimuth 0-360 deg, seeded), a synthetic source with the
 operator's scaling.
 
+After the headline: the layered 64x64x48 sweep over 8 frequencies with 32
+Haskell plane waves each (``sweep``, ``--no-sweep`` skips it), and the
+batched small-model workload (256 models of uneven size, each its own
+frequency and 1-16 right-hand sides; ``batched``, ``--no-batch``).
+
 Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on
 one GPU (frequencies, models and right-hand-side blocks are independent and
 would shard across GPUs without communication).
@@ -52,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-latency sample")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched small-model sample")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the multi-frequency layered sample")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="requests/s of the serving sample")
     ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the serving sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -125,6 +132,18 @@ def serving_sample(rate, seconds):
     rep = serving.latency_report(reqs, wall)
     rep.update(rate_per_s=rate, window_s=seconds, sides="32-80", slots=4)
     return rep
+
+
+def sweep_sample():
+    """The layered multi-frequency workload (wtomo3d_amd.sweep.benchmark)."""
+    from wtomo3d_amd import sweep
+    return sweep.benchmark()
+
+
+def batch_sample():
+    """The batched small-model workload (wtomo3d_amd.batch.benchmark)."""
+    from wtomo3d_amd import batch
+    return batch.benchmark(progress=progress)
 
 
 def main():
@@ -272,6 +291,12 @@ def main():
     L.wt_dev_free(dB)
     L.wt_dev_free(dX)
     s.close()
+    if not a.no_sweep:
+        progress("multi-frequency layered sweep")
+        out["sweep"] = sweep_sample()
+    if not a.no_batch:
+        progress("batched small models")
+        out["batched"] = batch_sample()
     print(json.dumps(out))
 
 

@@ -1,0 +1,143 @@
+"""Frequency sweep of one model with plane-wave sources.
+
+The reference's driver loops over the frequencies of a project and, at each
+one, applies DISPERSION, assembles with ASMBLE and factors with
+STRUMPACK_SERIAL_FINTER job 2 (elwave.f90:147-186); its per-source solve
+phase is empty there (elwave.f90:182).  ``sweep`` runs that loop on the GPU
+and completes it: at each frequency the Thomson-Haskell P/SV plane waves of
+the layered background (HASKGRN, haskgrn.f:31-672, expanded in 3-D) are
+formed on the device, the right-hand sides are b = A u for those fields, and
+all of them are solved as one block with FP64 iterative refinement.  The
+solution is then u itself, so every frequency reports its relative
+wavefield error beside ||A x - b|| / ||b||.
+"""
+import ctypes
+import json
+import time
+
+import numpy as np
+
+from ._lib import lib, check, grid_params, WtSolveStats
+from . import media, rhs
+from .solver import Solver
+
+
+def layered_workload(nx=64, ny=64, nz=48, dx=250.0, seed=1002, freqs=None, nsrc=32, inc=(10.0, 35.0),
+                     pml_nodes=10):
+    """The layered multi-frequency workload: a seeded 6-layer model
+    (``media.layered``), 8 frequencies 0.10-0.45 Hz by default, and per
+    frequency ``nsrc`` plane waves, P or SV at random, incidence uniform in
+    ``inc`` degrees and azimuth uniform in 0-360 degrees (seeded)."""
+    if freqs is None:
+        freqs = np.round(np.arange(0.10, 0.4501, 0.05), 4)
+    vp1, vs1, rho1 = media.layered(nx, ny, nz, dx, seed)
+    shape = (ny, nx, nz)
+    vp = np.broadcast_to(vp1[None, None, :], shape)
+    vs = np.broadcast_to(vs1[None, None, :], shape)
+    rho = np.broadcast_to(rho1[None, None, :], shape)
+    rng = np.random.default_rng(seed + 17)
+    sources = []
+    for _ in freqs:
+        sources.append(dict(ain=np.deg2rad(rng.uniform(inc[0], inc[1], nsrc)),
+                            is_s=(rng.random(nsrc) < 0.5).astype(np.int32),
+                            phi=np.deg2rad(rng.uniform(0.0, 360.0, nsrc))))
+    return dict(dims=(nx, ny, nz), dx=dx, vp=vp, vs=vs, rho=rho, layers=media.layer_stack(nx, ny, nz, dx, seed),
+                freqs=[float(f) for f in freqs], sources=sources, pmld=pml_nodes * dx)
+
+
+def sweep(w, prec=32, tol=1e-8, max_iter=30, compare_fp64=False, leaf_nodes=64):
+    """Runs the workload ``w`` (``layered_workload``) frequency by frequency
+    on one solver handle.  Returns a list of per-frequency dicts: times of
+    assembly + factorization, right-hand-side generation and solve, the
+    refinement statistics, the relative wavefield error against the plane
+    waves and, with ``compare_fp64``, the complex128 path's time and its
+    difference to the refined solution."""
+    nx, ny, nz = w["dims"]
+    dx = w["dx"]
+    lam, mu, rho = media.lame(w["vp"], w["vs"], w["rho"])
+    L = lib()
+    out = []
+    with Solver(nx, ny, nz, leaf_nodes) as s:
+        s.set_model(lam, mu, rho)
+        st_ptr = s.stream()
+        n = s.n
+        nsrc = len(w["sources"][0]["ain"])
+        nb = n * nsrc * 16
+        U, B, X = rhs.DeviceBuffer(nb), rhs.DeviceBuffer(nb), rhs.DeviceBuffer(nb)
+        prof_bufs = None
+        for f, src in zip(w["freqs"], w["sources"]):
+            omega = 2 * np.pi * f
+            p = grid_params(nx, ny, nz, dx, omega, pmld=w["pmld"])
+            L.wt_dev_sync()
+            t0 = time.perf_counter()
+            s.assemble(p)
+            s.factor(prec)
+            L.wt_dev_sync()
+            t1 = time.perf_counter()
+            prof = rhs.profiles(w["layers"], src["ain"], src["is_s"], omega, nz, dx, 0.0, 64, stream=st_ptr,
+                                out=prof_bufs)
+            prof_bufs = (prof.U, prof.W, prof.kh)
+            rhs.background_fields(nx, ny, nz, dx, prof, src["phi"], stream=st_ptr, out=U)
+            check(L.wt_solver_apply(s._h, nsrc, U.ptr, B.ptr, 1), "wt_solver_apply")
+            L.wt_dev_sync()
+            t2 = time.perf_counter()
+            st = WtSolveStats()
+            check(L.wt_solver_solve(s._h, prec, nsrc, B.ptr, X.ptr, 1, float(tol), int(max_iter), ctypes.byref(st)),
+                  "wt_solver_solve")
+            L.wt_dev_sync()
+            t3 = time.perf_counter()
+            Uh = U.to_host(np.complex128, (n, nsrc))
+            Xh = X.to_host(np.complex128, (n, nsrc))
+            werr = float(np.max(np.linalg.norm(Xh - Uh, axis=0) / np.linalg.norm(Uh, axis=0)))
+            rec = dict(freq=f, nrhs=nsrc, factor_s=t1 - t0, rhs_s=t2 - t1, solve_s=t3 - t2,
+                       rel_residual=st.rel_residual, iterations=st.iterations, rel_wavefield_err=werr)
+            if compare_fp64 and prec != 64:
+                t4 = time.perf_counter()
+                s.factor(64)
+                st64 = WtSolveStats()
+                check(L.wt_solver_solve(s._h, 64, nsrc, B.ptr, X.ptr, 1, 0.0, 1, ctypes.byref(st64)),
+                      "wt_solver_solve")
+                L.wt_dev_sync()
+                rec["fp64_s"] = time.perf_counter() - t4
+                X64 = X.to_host(np.complex128, (n, nsrc))
+                rec["fp64_rel_residual"] = st64.rel_residual
+                rec["vs_fp64_rel_diff"] = float(np.max(np.linalg.norm(Xh - X64, axis=0) /
+                                                       np.linalg.norm(X64, axis=0)))
+                s.release(64)
+            out.append(rec)
+    return out
+
+
+def benchmark(nx=64, ny=64, nz=48, nfreq=8):
+    """The layered multi-frequency workload: 64x64x48, dx = 250 m, a seeded
+    6-layer model, 10-node PML with a free surface, 8 frequencies 0.10-0.45
+    Hz, 32 Thomson-Haskell P/SV plane waves per frequency (incidence 10-35
+    deg, azimuth 0-360 deg).  Per frequency: assembly + complex64 factor,
+    the plane-wave fields on the GPU, b = A u, block solve with FP64
+    refinement to 1e-8; then the complex128 factor + one solve of the same
+    block for comparison."""
+    w = layered_workload(nx, ny, nz, freqs=np.round(np.arange(nfreq) * 0.05 + 0.10, 4))
+    sweep(layered_workload(nx, ny, nz, freqs=[0.3], nsrc=32), prec=32)  # plans and buffers (untimed)
+    recs = sweep(w, prec=32, tol=1e-8, max_iter=30, compare_fp64=True)
+    t32 = sum(r["factor_s"] + r["solve_s"] for r in recs)
+    t64 = sum(r["fp64_s"] for r in recs)
+    nrhs = sum(r["nrhs"] for r in recs)
+    return {"workload": "layered %dx%dx%d dx=250m, %d frequencies from 0.10 Hz by 0.05 Hz, 32 Haskell P/SV "
+                        "plane waves each, complex64 factor + FP64 IR to 1e-8 vs complex128" % (nx, ny, nz, nfreq),
+            "rhs": nrhs, "rhs_per_s": nrhs / t32, "fp64_rhs_per_s": nrhs / t64, "fp32ir_over_fp64": t64 / t32,
+            "s_fp32ir": t32, "s_fp64": t64, "rhs_generation_s": sum(r["rhs_s"] for r in recs),
+            "max_rel_residual": max(r["rel_residual"] for r in recs),
+            "max_rel_wavefield_err": max(r["rel_wavefield_err"] for r in recs),
+            "max_vs_fp64_rel_diff": max(r["vs_fp64_rel_diff"] for r in recs),
+            "per_frequency": recs}
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="run the layered multi-frequency workload once and print JSON")
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--ny", type=int, default=64)
+    ap.add_argument("--nz", type=int, default=48)
+    ap.add_argument("--nfreq", type=int, default=8)
+    a = ap.parse_args()
+    print(json.dumps(benchmark(a.nx, a.ny, a.nz, a.nfreq)))
