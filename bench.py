@@ -5,7 +5,9 @@ Workload (the heterogeneous 96x96x64 configuration): dx = 200 m, a seeded
 Gardner density) plus a Gaussian random field (5 % std, 2 km correlation)
 on Vp and Vs with the density perturbed by 0.3x; 10-node PML (pmld = 10 dx,
 ipml = 11) with a free surface; 0.3 Hz; 64 plane-wave right-hand sides as
-one block.  One step = FP64 assembly of the operator at the frequency +
+one block.  Before the warm-up the factorization's GEMM tile shapes are
+autotuned (one factorization per tile variant; ``--no-autotune`` keeps the
+defaults).  One step = FP64 assembly of the operator at the frequency +
 complex64 multifrontal factorization + block solve with FP64 iterative
 refinement to max_j ||b_j - A x_j|| / ||b_j|| <= 1e-8.  The model, the CSR
 pattern, the symbolic analysis and the right-hand sides are resident in HBM
@@ -58,6 +60,8 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-latency sample")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-model sample")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default GEMM tile shapes (no tuning factorizations before the warm-up)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the multi-frequency layered sample")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="requests/s of the serving sample")
     ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the serving sample")
@@ -193,6 +197,14 @@ def main():
         W._lib.check(L.wt_solver_solve(s._h, prec, nrhs, dB, dX, 1, tol, maxit, ctypes.byref(st)), "solve")
         return st.rel_residual, st.iterations
 
+    tune = None
+    if not a.no_autotune:
+        progress("autotuning the factorization's GEMM tile shapes")
+        t0 = time.perf_counter()
+        sp = s.autotune(32)
+        tune = {"gemm_speedup": sp, "s": time.perf_counter() - t0,
+                "note": "one factorization per tile variant before the warm-up; every GEMM launch keeps its "
+                        "fastest variant"}
     progress("symbolic %.2f s; warmup" % t_symbolic)
     for _ in range(a.warmup):
         step(32, a.tol, 30)
@@ -249,6 +261,12 @@ def main():
 
     g_fac = prof["gemm_factor"]
     achieved = g_fac["flops"] / (g_fac["ms"] * 1e-3) / 1e12 if g_fac["ms"] > 0 else 0.0
+    # every factorization GEMM, whichever tile variant the autotuner gave it
+    g_nar = prof["gemm_factor_narrow"]
+    g_all_ms = g_fac["ms"] + g_nar["ms"]
+    g_all_tf = (g_fac["flops"] + g_nar["flops"]) / (g_all_ms * 1e-3) / 1e12 if g_all_ms > 0 else 0.0
+    all_gemm = {"achieved": g_all_tf, "frac": g_all_tf / PEAK_FP32_TFLOPS, "ms_per_step": g_all_ms / a.steps,
+                "launches_per_step": (g_fac["launches"] + g_nar["launches"]) / a.steps}
     traffic = gemm_traffic()
     asm = prof["assemble"]
     asm_gbs = asm["bytes"] / (asm["ms"] * 1e-3) / 1e9 if asm["ms"] > 0 else 0.0
@@ -265,8 +283,8 @@ def main():
                                "FP64 IR to 1e-8" % (nx, ny, nz, nrhs),
                    "nx": nx, "ny": ny, "nz": nz, "nrhs": nrhs, "unknowns": info["n"], "nnz": info["nnz"],
                    "fronts": info["fronts"], "factor_flops": info["flops"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": GEMM_KERNEL + ": the factorization's square-tile complex "
-                     "GEMMs (3M, fp32 MFMA)", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "kernel": GEMM_KERNEL + ": the factorization's complex GEMMs on the "
+                     "128x128 tile (3M, fp32 MFMA); all_factor_gemms counts every tile variant", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
@@ -274,13 +292,15 @@ def main():
                      "avg_launch_ms": g_fac["ms"] / max(g_fac["launches"], 1),
                      "launches_per_step": g_fac["launches"] / a.steps,
                      "ms_per_step": g_fac["ms"] / a.steps,
-                     "executed_frac_3m": 0.75 * achieved / PEAK_FP32_TFLOPS},
+                     "executed_frac_3m": 0.75 * achieved / PEAK_FP32_TFLOPS,
+                     "all_factor_gemms": all_gemm},
         "cpu_baseline": cpu,
         "rel_residual": max(res32), "refinement_solves": max(it32),
         "fp64": fp64,
         "assembly": {"gbs": asm_gbs, "frac_hbm": asm_gbs / PEAK_HBM_GBS, "ms": asm["ms"] / max(asm["launches"], 1)},
         "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()},
         "symbolic_s": t_symbolic,
+        "autotune": tune,
         "p50_latency_ms": serve["p50_latency_ms"] if serve else None,
         "p99_latency_ms": serve["p99_latency_ms"] if serve else None,
         "serving": serve,

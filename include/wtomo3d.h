@@ -148,6 +148,12 @@ int wt_solver_get_values(wt_solver *h, void *vals);
  * (64): replaces STRUMPACK_factor (strumpack_finter.c:84).  Returns
  * WT_ERR_PIVOT when a diagonal tile had an exactly zero pivot. */
 int wt_solver_factor(wt_solver *h, int factor_prec);
+/* Tile-shape autotuner of the factorization: factors once per GEMM tile
+ * variant with every launch timed, then keeps each launch's fastest variant
+ * for the later factorizations of this handle (the handle is left
+ * factored).  *speedup: GEMM time under the default choice over the tuned
+ * time.  No counterpart in the reference. */
+int wt_solver_autotune(wt_solver *h, int factor_prec, double *speedup);
 int wt_solver_release(wt_solver *h, int factor_prec);
 
 /* Solve A X = B for nrhs right-hand sides (row-major n x nrhs complex128,

@@ -115,6 +115,29 @@ def test_refactor_same_handle_new_frequency():
     s.close()
 
 
+def test_autotuned_tiles_keep_the_solution():
+    """The tile-shape autotuner (one factorization per GEMM tile variant)
+    leaves the handle factored, and the tuned factorization solves to the
+    same refined solution."""
+    nx, ny, nz = 16, 14, 12
+    prm, lam, mu, rho, ir, jc, vals = problem(nx, ny, nz, seed=8)
+    s = Solver(nx, ny, nz)
+    s.set_values(vals)
+    A = SO.csr(ir, jc, s.values())
+    B, _ = rhs_block(A, 3, 21)
+    s.factor(32)
+    X0, _ = s.solve(B, prec=32, tol=1e-12, max_iter=30)
+    sp = s.autotune(32)
+    assert sp > 0.5
+    X1, st1 = s.solve(B, prec=32, tol=1e-12, max_iter=30)
+    s.factor(32)
+    X2, st2 = s.solve(B, prec=32, tol=1e-12, max_iter=30)
+    s.close()
+    for X, st in ((X1, st1), (X2, st2)):
+        assert st["rel_residual"] <= 1e-12
+        assert np.linalg.norm(X - X0) / np.linalg.norm(X0) <= 1e-10
+
+
 def test_strumpack_serial_finter_dropin():
     """Jobs 1/2/3/-1 with the reference's argument list (strumpack_finter.c:9)."""
     nx, ny, nz = 8, 7, 9

@@ -372,6 +372,14 @@ int wt_solver_factor(wt_solver* h, int factor_prec)
     return WT_ERR_ARG;
 }
 
+int wt_solver_autotune(wt_solver* h, int factor_prec, double* speedup)
+{
+    if (!h) return WT_ERR_ARG;
+    if (factor_prec == WT_PREC_F32) return wt_solver_autotune<float>(h->s, h->s.f32, speedup);
+    if (factor_prec == WT_PREC_F64) return wt_solver_autotune<double>(h->s, h->s.f64, speedup);
+    return WT_ERR_ARG;
+}
+
 int wt_solver_apply(wt_solver* h, int nrhs, const void* x, void* y, int on_device)
 {
     if (!h || !x || !y || nrhs < 1) return WT_ERR_ARG;

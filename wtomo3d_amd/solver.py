@@ -102,6 +102,13 @@ class Solver:
     def factor(self, prec=32):
         check(lib().wt_solver_factor(self._h, prec), "wt_solver_factor")
 
+    def autotune(self, prec=32):
+        """Times every factorization GEMM launch under each tile variant and
+        keeps the fastest; returns the GEMM speedup over the default tiles."""
+        sp = ctypes.c_double()
+        check(lib().wt_solver_autotune(self._h, prec, ctypes.byref(sp)), "wt_solver_autotune")
+        return sp.value
+
     def release(self, prec):
         check(lib().wt_solver_release(self._h, prec), "wt_solver_release")
 
