@@ -471,6 +471,8 @@ static int factor_finish(wt_solver_impl& s, wt_numeric<R>& num)
 
 template <typename R>
 int wt_solver_factor_launch(wt_solver_impl& s, wt_numeric<R>& num);
+template <typename R>
+static int tune_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs);
 
 // With s.use_graphs (and profiling off) the sequence is captured once into
 // a HIP graph and replayed: thousands of small launches per factorization
@@ -548,12 +550,14 @@ int wt_solver_autotune(wt_solver_impl& s, wt_numeric<R>& num, double* speedup)
         if (e) (void)hipEventDestroy(e);
     for (size_t i = 0; i < nl; ++i) num.fplan.launches[i].var = rc == WT_OK ? best[i] : dflt[i];
     num.drop_graphs();
+    num.tuned = rc == WT_OK;
+    // a solve plan built earlier is tuned now; later ones when they are built
+    if (rc == WT_OK && num.snrhs > 0) rc = tune_solve_plan(s, num, num.snrhs);
     if (rc == WT_OK && speedup) {
         double a = 0, b = 0;
         for (size_t i = 0; i < nl; ++i) { a += tdef[i]; b += std::isfinite(tbest[i]) ? tbest[i] : tdef[i]; }
         *speedup = b > 0 ? a / b : 1.0;
     }
-    num.tuned = rc == WT_OK;
     return rc;
 }
 
@@ -674,22 +678,74 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
     return WT_OK;
 }
 
+// Tile-shape tuning of the block solve's GEMM launches, as for the
+// factorization: one solve per tile variant with every launch timed.  It
+// runs on whatever y holds; the caller's next solve overwrites y first.
+template <typename R>
+static int tune_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
+{
+    const size_t nl = num.splan.launches.size();
+    std::vector<int> best(nl);
+    std::vector<float> tbest(nl, INFINITY);
+    for (size_t i = 0; i < nl; ++i) best[i] = num.splan.launches[i].var;
+    std::vector<hipEvent_t> ev(2 * nl, nullptr);
+    int rc = WT_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = WT_ERR_HIP;
+    const bool prof = s.prof.on;
+    s.prof.on = false;
+    for (int v = 0; v < WT_GEMM_NVAR && rc == WT_OK; ++v) {
+        for (auto& L : num.splan.launches) L.var = v;
+        if ((rc = enqueue_solve(s, num, nrhs, &ev))) break;
+        if (hipStreamSynchronize(s.stream) != hipSuccess) { rc = WT_ERR_HIP; break; }
+        for (size_t i = 0; i < nl; ++i) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess && t < tbest[i]) {
+                tbest[i] = t;
+                best[i] = v;
+            }
+        }
+    }
+    s.prof.on = prof;
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (size_t i = 0; i < nl; ++i) num.splan.launches[i].var = best[i];
+    if (num.sgraph) (void)hipGraphExecDestroy(num.sgraph);
+    num.sgraph = nullptr;
+    num.sgraph_nrhs = 0;
+    return rc;
+}
+
 template <typename R>
 int wt_solver_prepare_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
 {
     if (!num.ready) return WT_ERR_STATE;
     if (num.snrhs == nrhs) return WT_OK;
-    return build_solve_plan(s, num, nrhs);
+    int rc = build_solve_plan(s, num, nrhs);
+    if (rc == WT_OK && num.tuned) rc = tune_solve_plan(s, num, nrhs);
+    return rc;
 }
 
 // In-place block solve of A x = y for nrhs columns in elimination order
 // (y column-major, ld = n).  Uses the solver's own y buffer when y == null.
 template <typename R>
+static int enqueue_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, std::vector<hipEvent_t>* ev);
+
+template <typename R>
 int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R>* /*unused*/)
 {
     if (!num.ready) return WT_ERR_STATE;
     int rc;
-    if (num.snrhs != nrhs && (rc = build_solve_plan(s, num, nrhs))) return rc;
+    if (num.snrhs != nrhs && (rc = wt_solver_prepare_solve(s, num, nrhs))) return rc;
+    return enqueue_solve(s, num, nrhs, nullptr);
+}
+
+// The block solve's launches; with ev, every GEMM launch is bracketed by
+// events (two per launch index) for the autotuner.
+template <typename R>
+static int enqueue_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, std::vector<hipEvent_t>* ev)
+{
+    int rc;
     hipStream_t st = s.stream;
     wt_prof& P = s.prof;
     for (int pass = 0; pass < 2; ++pass) {
@@ -712,8 +768,10 @@ int wt_solver_solve_block(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, cpx<R
                 });
                 break;
             case WT_STEP_GEMM:
+                if (ev) (void)hipEventRecord((*ev)[2 * t.gemm], st);
                 rc = P.run(st, WT_PROF_GEMM_SOLVE, num.splan.launches[t.gemm].flops, 0,
                            [&] { return num.splan.run(t.gemm, st); });
+                if (ev) (void)hipEventRecord((*ev)[2 * t.gemm + 1], st);
                 break;
             case WT_STEP_SPFWD:
             case WT_STEP_SPBWD:
