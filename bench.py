@@ -6,7 +6,9 @@ Gardner density) plus a Gaussian random field (5 % std, 2 km correlation)
 on Vp and Vs with the density perturbed by 0.3x; 10-node PML (pmld = 10 dx,
 ipml = 11) with a free surface; 0.3 Hz; 64 plane-wave right-hand sides as
 one block.  Before the warm-up the factorization's GEMM tile shapes are
-autotuned (one factorization per tile variant; ``--no-autotune`` keeps the
+autotuned: the package's tuning table for the workload is applied when it
+exists (``wtomo3d_amd/tuning/``), otherwise one factorization per tile
+variant is measured (``--retune`` forces that, ``--no-autotune`` keeps the
 defaults).  One step = FP64 assembly of the operator at the frequency +
 complex64 multifrontal factorization + block solve with FP64 iterative
 refinement to max_j ||b_j - A x_j|| / ||b_j|| <= 1e-8.  The model, the CSR
@@ -61,7 +63,11 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-latency sample")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-model sample")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default GEMM tile shapes (no tuning factorizations before the warm-up)")
+                    help="keep the default GEMM tile shapes (no tuning table, no tuning factorizations)")
+    ap.add_argument("--retune", action="store_true",
+                    help="measure the tile shapes even when the package holds a tuning table for this workload")
+    ap.add_argument("--save-tuning", metavar="PATH", default=None,
+                    help="write the tuned tile shapes (factorization and block solve) to PATH as JSON")
     ap.add_argument("--no-sweep", action="store_true", help="skip the multi-frequency layered sample")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="requests/s of the serving sample")
     ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the serving sample")
@@ -198,16 +204,32 @@ def main():
         return st.rel_residual, st.iterations
 
     tune = None
-    if not a.no_autotune:
+    table = os.path.join(ROOT, "wtomo3d_amd", "tuning", "fp32_%dx%dx%d_nrhs%d.json" % (nx, ny, nz, nrhs))
+    if not a.no_autotune and not a.retune and os.path.exists(table):
+        with open(table) as f:
+            tt = json.load(f)
+        try:
+            s.set_tiles(tt["factor"], 32, 0)
+            s.set_tiles(tt["solve"], 32, 1, nrhs)
+            tune = {"source": os.path.relpath(table, ROOT), "gemm_speedup_when_measured": tt.get("gemm_speedup"),
+                    "note": "tile shapes measured by the autotuner on MI355X and stored in the package"}
+        except RuntimeError:
+            tune = None  # the plans changed since the table was written: measure again
+    if not a.no_autotune and tune is None:
         progress("autotuning the factorization's GEMM tile shapes")
         t0 = time.perf_counter()
         sp = s.autotune(32)
         tune = {"gemm_speedup": sp, "s": time.perf_counter() - t0,
-                "note": "one factorization per tile variant before the warm-up; every GEMM launch keeps its "
-                        "fastest variant"}
+                "note": "one factorization per tile variant before the warm-up (the block solve's tile shapes "
+                        "when its plan is built); every GEMM launch keeps its fastest variant"}
     progress("symbolic %.2f s; warmup" % t_symbolic)
     for _ in range(a.warmup):
         step(32, a.tol, 30)
+    if a.save_tuning:
+        with open(a.save_tuning, "w") as f:
+            json.dump({"workload": "%dx%dx%d, %d RHS, complex64 factor" % (nx, ny, nz, nrhs),
+                       "gemm_speedup": (tune or {}).get("gemm_speedup"),
+                       "factor": s.get_tiles(32, 0).tolist(), "solve": s.get_tiles(32, 1, nrhs).tolist()}, f)
     progress("timed steps")
     L.wt_dev_sync()
     s.profile(True)

@@ -154,6 +154,14 @@ int wt_solver_factor(wt_solver *h, int factor_prec);
  * factored).  *speedup: GEMM time under the default choice over the tuned
  * time.  No counterpart in the reference. */
 int wt_solver_autotune(wt_solver *h, int factor_prec, double *speedup);
+/* The GEMM tile variant of every launch of the factorization (which = 0)
+ * or of the block solve for nrhs columns (which = 1), so that a tuning can
+ * be stored and applied again without re-measuring.  get: vars may be NULL
+ * to query *count (plans are built on demand: the factorization's needs
+ * assembled values); set: count must equal the plan's launch count.  No
+ * counterpart in the reference. */
+int wt_solver_get_tiles(wt_solver *h, int factor_prec, int which, int nrhs, int32_t *vars, int64_t *count);
+int wt_solver_set_tiles(wt_solver *h, int factor_prec, int which, int nrhs, const int32_t *vars, int64_t count);
 int wt_solver_release(wt_solver *h, int factor_prec);
 
 /* Solve A X = B for nrhs right-hand sides (row-major n x nrhs complex128,

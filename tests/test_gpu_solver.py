@@ -138,6 +138,34 @@ def test_autotuned_tiles_keep_the_solution():
         assert np.linalg.norm(X - X0) / np.linalg.norm(X0) <= 1e-10
 
 
+def test_tile_table_round_trip():
+    """Stored tile shapes (wt_solver_get_tiles / wt_solver_set_tiles) apply
+    to a fresh handle of the same grid and give the tuned handle's solution;
+    a table of the wrong length is refused."""
+    nx, ny, nz = 14, 12, 10
+    prm, lam, mu, rho, ir, jc, vals = problem(nx, ny, nz, seed=9)
+    A = None
+    with Solver(nx, ny, nz) as s:
+        s.set_values(vals)
+        A = SO.csr(ir, jc, s.values())
+        B, _ = rhs_block(A, 3, 31)
+        s.autotune(32)
+        X1, _ = s.solve(B, prec=32, tol=1e-12, max_iter=30)
+        tf, ts = s.get_tiles(32, 0), s.get_tiles(32, 1, 3)
+    assert tf.size > 0 and ts.size > 0 and set(np.unique(tf)) <= {0, 1, 2}
+    with Solver(nx, ny, nz) as s:
+        s.set_values(vals)
+        s.set_tiles(tf, 32, 0)
+        with pytest.raises(RuntimeError):
+            s.set_tiles(tf[:-1], 32, 0)
+        s.factor(32)
+        s.set_tiles(ts, 32, 1, 3)
+        assert np.array_equal(s.get_tiles(32, 0), tf) and np.array_equal(s.get_tiles(32, 1, 3), ts)
+        X2, st = s.solve(B, prec=32, tol=1e-12, max_iter=30)
+    assert st["rel_residual"] <= 1e-12
+    assert np.linalg.norm(X2 - X1) / np.linalg.norm(X1) <= 1e-10
+
+
 def test_strumpack_serial_finter_dropin():
     """Jobs 1/2/3/-1 with the reference's argument list (strumpack_finter.c:9)."""
     nx, ny, nz = 8, 7, 9

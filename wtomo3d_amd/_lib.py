@@ -89,6 +89,10 @@ def _declare(L):
     L.wt_batch_run.argtypes = [_vp, i32, i32, _vp, _vp, _vp, i32, ctypes.c_double, i32, pstats]
     L.wt_solver_autotune.argtypes = [_vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.wt_solver_autotune.restype = ctypes.c_int
+    L.wt_solver_get_tiles.argtypes = [_vp, i32, i32, i32, _vp, ctypes.POINTER(ctypes.c_int64)]
+    L.wt_solver_get_tiles.restype = ctypes.c_int
+    L.wt_solver_set_tiles.argtypes = [_vp, i32, i32, i32, _vp, i64]
+    L.wt_solver_set_tiles.restype = ctypes.c_int
     L.wt_batch_symbolic_info.argtypes = [i32, _vp, i32, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_double)]
     L.wt_graph_host.argtypes = [i32] * 4 + [_vp, _vp]

@@ -380,6 +380,26 @@ int wt_solver_autotune(wt_solver* h, int factor_prec, double* speedup)
     return WT_ERR_ARG;
 }
 
+int wt_solver_get_tiles(wt_solver* h, int factor_prec, int which, int nrhs, int32_t* vars, int64_t* count)
+{
+    if (!h || !count) return WT_ERR_ARG;
+    if (factor_prec == WT_PREC_F32)
+        return wt_solver_plan_tiles<float>(h->s, h->s.f32, which, nrhs, vars, nullptr, count);
+    if (factor_prec == WT_PREC_F64)
+        return wt_solver_plan_tiles<double>(h->s, h->s.f64, which, nrhs, vars, nullptr, count);
+    return WT_ERR_ARG;
+}
+
+int wt_solver_set_tiles(wt_solver* h, int factor_prec, int which, int nrhs, const int32_t* vars, int64_t count)
+{
+    if (!h || !vars) return WT_ERR_ARG;
+    if (factor_prec == WT_PREC_F32)
+        return wt_solver_plan_tiles<float>(h->s, h->s.f32, which, nrhs, nullptr, vars, &count);
+    if (factor_prec == WT_PREC_F64)
+        return wt_solver_plan_tiles<double>(h->s, h->s.f64, which, nrhs, nullptr, vars, &count);
+    return WT_ERR_ARG;
+}
+
 int wt_solver_apply(wt_solver* h, int nrhs, const void* x, void* y, int on_device)
 {
     if (!h || !x || !y || nrhs < 1) return WT_ERR_ARG;

@@ -788,6 +788,49 @@ static int enqueue_solve(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, std::v
     return WT_OK;
 }
 
+// Tile variant of every GEMM launch of the factorization (which = 0) or of
+// the block solve for nrhs columns (which = 1).  get: vars may be null to
+// query the count; set: the count must match the plan.
+template <typename R>
+int wt_solver_plan_tiles(wt_solver_impl& s, wt_numeric<R>& num, int which, int nrhs, int32_t* vars,
+                         const int32_t* in, int64_t* count)
+{
+    int rc;
+    if (which == 0) {
+        if ((rc = factor_prepare(s, num))) return rc;
+    } else if (which == 1) {
+        if (!num.ready) return WT_ERR_STATE;
+        if (nrhs < 1 || nrhs > 64) return WT_ERR_ARG;
+        if ((rc = wt_solver_prepare_solve(s, num, nrhs))) return rc;
+    } else {
+        return WT_ERR_ARG;
+    }
+    auto& L = which == 0 ? num.fplan.launches : num.splan.launches;
+    if (in) {
+        if (*count != (int64_t)L.size()) return WT_ERR_ARG;
+        for (int64_t i = 0; i < *count; ++i)
+            if (in[i] < 0 || in[i] >= WT_GEMM_NVAR) return WT_ERR_ARG;
+        for (int64_t i = 0; i < *count; ++i) L[i].var = in[i];
+        if (which == 0) {
+            if (num.fgraph) (void)hipGraphExecDestroy(num.fgraph);
+            num.fgraph = nullptr;
+        } else {
+            if (num.sgraph) (void)hipGraphExecDestroy(num.sgraph);
+            num.sgraph = nullptr;
+            num.sgraph_nrhs = 0;
+        }
+        return WT_OK;
+    }
+    *count = (int64_t)L.size();
+    if (vars)
+        for (size_t i = 0; i < L.size(); ++i) vars[i] = L[i].var;
+    return WT_OK;
+}
+
+template int wt_solver_plan_tiles<float>(wt_solver_impl&, wt_numeric<float>&, int, int, int32_t*, const int32_t*,
+                                         int64_t*);
+template int wt_solver_plan_tiles<double>(wt_solver_impl&, wt_numeric<double>&, int, int, int32_t*, const int32_t*,
+                                          int64_t*);
 template int wt_solver_factor_run<float>(wt_solver_impl&, wt_numeric<float>&);
 template int wt_solver_factor_run<double>(wt_solver_impl&, wt_numeric<double>&);
 template int wt_solver_factor_launch<float>(wt_solver_impl&, wt_numeric<float>&);

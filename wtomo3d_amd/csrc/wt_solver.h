@@ -248,6 +248,9 @@ template <typename R>
 int wt_solver_factor_wait(wt_solver_impl& s, wt_numeric<R>& num);
 template <typename R>
 int wt_solver_autotune(wt_solver_impl& s, wt_numeric<R>& num, double* speedup);
+template <typename R>
+int wt_solver_plan_tiles(wt_solver_impl& s, wt_numeric<R>& num, int which, int nrhs, int32_t* vars,
+                         const int32_t* in, int64_t* count);
 // Refinement buffers (xb, rb, bb, partial, norms) for nrhs columns.
 int wt_solver_ir_buffers(wt_solver_impl& s, int nrhs);
 template <typename R>

@@ -109,6 +109,19 @@ class Solver:
         check(lib().wt_solver_autotune(self._h, prec, ctypes.byref(sp)), "wt_solver_autotune")
         return sp.value
 
+    def get_tiles(self, prec=32, which=0, nrhs=64):
+        """Tile variant of every GEMM launch of the factorization (which=0)
+        or of the block solve for nrhs columns (which=1)."""
+        n = ctypes.c_int64()
+        check(lib().wt_solver_get_tiles(self._h, prec, which, nrhs, None, ctypes.byref(n)), "wt_solver_get_tiles")
+        v = np.zeros(n.value, np.int32)
+        check(lib().wt_solver_get_tiles(self._h, prec, which, nrhs, _p(v), ctypes.byref(n)), "wt_solver_get_tiles")
+        return v
+
+    def set_tiles(self, vars, prec=32, which=0, nrhs=64):
+        v = np.ascontiguousarray(vars, np.int32)
+        check(lib().wt_solver_set_tiles(self._h, prec, which, nrhs, _p(v), v.size), "wt_solver_set_tiles")
+
     def release(self, prec):
         check(lib().wt_solver_release(self._h, prec), "wt_solver_release")
 
