@@ -53,7 +53,7 @@ def test_gemm_row_permuted_b(prec):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
-@pytest.mark.parametrize("var", [0, 1, 2])
+@pytest.mark.parametrize("var", [0, 1, 2, 3])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (65, 65, 8)])
 def test_gemm_tile_variants(prec, var, shape):
     """Every tile variant (the autotuner picks among them per launch) gives

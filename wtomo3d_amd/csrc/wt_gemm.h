@@ -29,11 +29,12 @@ struct wt_gemm_prob {
 // Tile shapes of the kernel for precision R: variant 0 is the square-ish
 // default, variant 1 serves problems with N <= 64 (the block solves with
 // 64 right-hand sides) without wasting half of each tile, variant 2 is a
-// 64 x 64 tile for small fronts (more blocks per problem).  Every variant
+// 64 x 64 tile for small fronts (more blocks per problem), variant 3 a
+// 128 x 64 (fp64: 64 x 128) tile between them.  Every variant
 // has BM, BN >= 64 = WT_NB, which the aliased panel products rely on (one
 // tile spans all M, or all N, of a 64-wide panel).  The K loop is the same
 // in every variant, so they give identical results.
-#define WT_GEMM_NVAR 3
+#define WT_GEMM_NVAR 4
 template <typename R, int V>
 struct wt_gemm_tile;
 template <>
@@ -48,6 +49,10 @@ template <>
 struct wt_gemm_tile<double, 1> { static constexpr int BM = 128, BN = 64, WGM = 2, WGN = 2; };
 template <>
 struct wt_gemm_tile<double, 2> { static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2; };
+template <>
+struct wt_gemm_tile<float, 3> { static constexpr int BM = 128, BN = 64, WGM = 2, WGN = 2; };
+template <>
+struct wt_gemm_tile<double, 3> { static constexpr int BM = 64, BN = 128, WGM = 2, WGN = 2; };
 
 template <typename R>
 inline int64_t wt_gemm_tiles(int64_t M, int64_t N, int var = 0)
@@ -56,6 +61,7 @@ inline int64_t wt_gemm_tiles(int64_t M, int64_t N, int var = 0)
     switch (var) {
     case 1: BM = wt_gemm_tile<R, 1>::BM; BN = wt_gemm_tile<R, 1>::BN; break;
     case 2: BM = wt_gemm_tile<R, 2>::BM; BN = wt_gemm_tile<R, 2>::BN; break;
+    case 3: BM = wt_gemm_tile<R, 3>::BM; BN = wt_gemm_tile<R, 3>::BN; break;
     default: BM = wt_gemm_tile<R, 0>::BM; BN = wt_gemm_tile<R, 0>::BN; break;
     }
     return ((M + BM - 1) / BM) * ((N + BN - 1) / BN);

@@ -254,6 +254,10 @@ int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, 
         hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                            d_tile_start, nprob, ntiles);
         break;
+    case 3:
+        hipLaunchKernelGGL((gemm_batched_kernel<R, 3>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                           d_tile_start, nprob, ntiles);
+        break;
     default:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 0>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                            d_tile_start, nprob, ntiles);
