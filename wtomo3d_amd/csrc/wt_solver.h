@@ -11,7 +11,7 @@
 
 // One launch of the batched GEMM: a contiguous range of the plan's problems.
 // Tile counts are kept for every tile variant; var is the variant used (the
-// N <= 64 rule by default, or the autotuner's measured choice).
+// default rule of end(), or the autotuner's measured choice).
 struct wt_gemm_launch_rec {
     int64_t prob_off, tile_off;
     int64_t ntiles[WT_GEMM_NVAR];
@@ -62,9 +62,17 @@ struct wt_gemm_plan {
             for (int v = 0; v < WT_GEMM_NVAR; ++v) tiles[v].pop_back();
             return -1;
         }
-        cur.var = 1;
+        // default tile: 64 x 64 unless the launch has enough large tiles to
+        // fill the GPU several times over and they pad the problems little
+        // more than 64 x 64 tiles do (measured: the large tile wins only on
+        // the largest updates; narrow, small and ragged launches run up to
+        // 2x faster on 64 x 64, which gives them 4x the blocks)
+        bool narrow = true;
         for (int64_t i = cur.prob_off; i < (int64_t)probs.size(); ++i)
-            if (probs[i].N > 64) cur.var = 0;
+            if (probs[i].N > 64) narrow = false;
+        const double a0 = (double)cur.ntiles[0] * wt_gemm_tile<R, 0>::BM * wt_gemm_tile<R, 0>::BN;
+        const double a2 = (double)cur.ntiles[2] * wt_gemm_tile<R, 2>::BM * wt_gemm_tile<R, 2>::BN;
+        cur.var = (!narrow && cur.ntiles[0] >= 1024 && a0 <= 1.15 * a2) ? 0 : 2;
         launches.push_back(cur);
         return (int)launches.size() - 1;
     }
