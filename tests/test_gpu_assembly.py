@@ -92,6 +92,29 @@ def test_gpu_values_layered_64x64x48(prec):
         rel_err_entries(a, o)[0] <= (1e-15 if prec == 64 else 4e-6)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("dims", [(6, 5, 150), (4, 3, 65), (5, 4, 128)])
+def test_gpu_values_several_z_segments(dims, prec):
+    """Columns longer than one 64-node segment (a one-node last segment, an
+    exact multiple, three segments): every segment's rows land at their CSR
+    offsets; heterogeneous random model, PML on every side."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(77 + nz)
+    ft = np.float32 if prec == 32 else np.float64
+    vp = (4500 + 2000 * rng.random((ny, nx, nz))).astype(ft)
+    vs = (vp / ft(1.8)).astype(ft)
+    rho = (2000 + 800 * rng.random((ny, nx, nz))).astype(ft)
+    meta = dict(dx=250.0, freq=0.3, pmld=500.0, pmlr=1e-3)
+    dx_, om, pmld, pmlf, ipml = ref_params(meta, prec)
+    lam, mu, rho1 = lame_ref(vp, vs, rho, prec)
+    a = W.assemble(_params(nx, ny, nz, dx_, om, pmld, pmlf, ipml), lam, mu, rho1, prec=prec)
+    rc, o = O.assemble(nx, ny, nz, dx_, om, lam, mu, rho1, pmld, pmlf, ipml, prec=prec)
+    assert rc == 0
+    # a misplaced entry would be off by O(1); rounding of the random model's
+    # cell averages stays at the assembly bar (1e-12 FP64, 3e-5 FP32)
+    assert rel_err_entries(a, o)[0] <= (1e-12 if prec == 64 else 3e-5)
+
+
 def test_gpu_c1_sampled_rows_vs_reference():
     s = load("c1_rows")
     nx = ny = nz = 24
