@@ -165,3 +165,27 @@ def test_batch_run_releases_groups_and_matches_solve():
     for k in range(len(mods)):
         assert st[k]["rel_residual"] <= 1e-11
         assert _rel(X2[k], X1[k]) <= 1e-10 and np.array_equal(X2[k], X3[k])
+
+
+def test_batch_edge_models():
+    """A 2x2x2 model (all Dirichlet rows), a column longer than one assembly
+    segment (4x3x70) and an ordinary model in one batch: each matches its
+    single-model solve."""
+    dims = [(2, 2, 2), (4, 3, 70), (9, 8, 7)]
+    freqs = [0.2, 0.35, 0.1]
+    ncols = [1, 2, 3]
+    mods = _setup(dims=dims, freqs=freqs, seed=91)
+    with _batch(mods) as b:
+        U = _fields(mods, ncols, seed=29)
+        B = b.apply(U)
+        b.factor(32)
+        X, st = b.solve(B, prec=32, tol=1e-11, max_iter=30)
+    for k, m in enumerate(mods):
+        assert st[k]["rel_residual"] <= 1e-11, k
+        with Solver(*m["d"]) as s:
+            s.set_model(m["lam"], m["mu"], m["rho"])
+            s.assemble(m["p"])
+            assert _rel(B[k], s.apply(U[k])) <= 1e-14, k
+            s.factor(64)
+            Xs, _ = s.solve(B[k], prec=64, tol=1e-13, max_iter=10)
+        assert _rel(X[k], Xs) <= 1e-9, k
