@@ -161,6 +161,9 @@ int wt_solver_autotune(wt_solver *h, int factor_prec, double *speedup);
  * assembled values); set: count must equal the plan's launch count.  No
  * counterpart in the reference. */
 int wt_solver_get_tiles(wt_solver *h, int factor_prec, int which, int nrhs, int32_t *vars, int64_t *count);
+/* Capture the factorization in a HIP graph on its first run and replay it
+ * (default off for a single handle; profiling runs eagerly). */
+int wt_solver_set_graphs(wt_solver *h, int enable);
 int wt_solver_set_tiles(wt_solver *h, int factor_prec, int which, int nrhs, const int32_t *vars, int64_t count);
 int wt_solver_release(wt_solver *h, int factor_prec);
 

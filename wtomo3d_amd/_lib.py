@@ -93,6 +93,8 @@ def _declare(L):
     L.wt_solver_get_tiles.restype = ctypes.c_int
     L.wt_solver_set_tiles.argtypes = [_vp, i32, i32, i32, _vp, i64]
     L.wt_solver_set_tiles.restype = ctypes.c_int
+    L.wt_solver_set_graphs.argtypes = [_vp, i32]
+    L.wt_solver_set_graphs.restype = ctypes.c_int
     L.wt_batch_symbolic_info.argtypes = [i32, _vp, i32, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_double)]
     L.wt_graph_host.argtypes = [i32] * 4 + [_vp, _vp]

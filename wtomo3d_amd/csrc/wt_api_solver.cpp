@@ -380,6 +380,13 @@ int wt_solver_autotune(wt_solver* h, int factor_prec, double* speedup)
     return WT_ERR_ARG;
 }
 
+int wt_solver_set_graphs(wt_solver* h, int enable)
+{
+    if (!h) return WT_ERR_ARG;
+    h->s.use_graphs = enable != 0;
+    return WT_OK;
+}
+
 int wt_solver_get_tiles(wt_solver* h, int factor_prec, int which, int nrhs, int32_t* vars, int64_t* count)
 {
     if (!h || !count) return WT_ERR_ARG;

@@ -109,6 +109,10 @@ class Solver:
         check(lib().wt_solver_autotune(self._h, prec, ctypes.byref(sp)), "wt_solver_autotune")
         return sp.value
 
+    def set_graphs(self, enable=True):
+        """Capture the factorization in a HIP graph and replay it."""
+        check(lib().wt_solver_set_graphs(self._h, int(bool(enable))), "wt_solver_set_graphs")
+
     def get_tiles(self, prec=32, which=0, nrhs=64):
         """Tile variant of every GEMM launch of the factorization (which=0)
         or of the block solve for nrhs columns (which=1)."""
