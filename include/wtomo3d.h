@@ -318,9 +318,9 @@ int wt_gemm_host(int prec, int M, int N, int K, const void *A, int64_t lda, cons
                  void *C, int64_t ldc, int mode, const int32_t *bperm, int reps, double *ms);
 /* The same with the tile variant chosen by the caller: 0 (128 x 128 fp32,
  * 128 x 64 fp64), 1 (256 x 64 fp32), 2 (64 x 64), 3 (128 x 64 fp32,
- * 64 x 128 fp64), or -1 for the default rule (0 when N > 64 and the
- * problem has at least 1024 tiles of variant 0, else 2).  No counterpart in
- * the reference. */
+ * 64 x 128 fp64), or -1 for the default rule (fp32: 0 when N > 64 and the
+ * problem has at least 1024 tiles of variant 0, else 2; fp64: 0).  No
+ * counterpart in the reference. */
 int wt_gemm_host_var(int prec, int var, int M, int N, int K, const void *A, int64_t lda, const void *B,
                      int64_t ldb, void *C, int64_t ldc, int mode, const int32_t *bperm, int reps, double *ms);
 
