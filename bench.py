@@ -290,6 +290,20 @@ def main():
     all_gemm = {"achieved": g_all_tf, "frac": g_all_tf / PEAK_FP32_TFLOPS, "ms_per_step": g_all_ms / a.steps,
                 "launches_per_step": (g_fac["launches"] + g_nar["launches"]) / a.steps}
     traffic = gemm_traffic()
+    # block solve (all right-hand sides at once): factor entries read once per
+    # solve, 8 flops per entry and right-hand side; time = the solve GEMMs,
+    # the super-panel triangular solves and the gathers / permutations
+    n_solves = max(sum(it32), 1)
+    sv_ms = (prof["gemm_solve"]["ms"] + prof["panel_trsm"]["ms"] + prof["reorder"]["ms"]) / n_solves
+    f_bytes = info["factor_entries"] * 8.0
+    sv_flops = 8.0 * nrhs * info["factor_entries"]
+    sv_gbs = f_bytes / (sv_ms * 1e-3) / 1e9 if sv_ms > 0 else 0.0
+    sv_tf = sv_flops / (sv_ms * 1e-3) / 1e12 if sv_ms > 0 else 0.0
+    t_roof = max(f_bytes / (PEAK_HBM_GBS * 1e9), sv_flops / (PEAK_FP32_TFLOPS * 1e12))
+    block_solve = {"ms_per_solve": sv_ms, "factor_bytes": f_bytes, "gbs": sv_gbs, "frac_hbm": sv_gbs / PEAK_HBM_GBS,
+                   "tflops": sv_tf, "flop_per_byte": sv_flops / f_bytes,
+                   "frac_min_roof": (t_roof * 1e3) / sv_ms if sv_ms > 0 else 0.0,
+                   "solves_per_step": n_solves / a.steps}
     asm = prof["assemble"]
     asm_gbs = asm["bytes"] / (asm["ms"] * 1e-3) / 1e9 if asm["ms"] > 0 else 0.0
     progress("CPU baseline")
@@ -321,6 +335,7 @@ def main():
         "fp64": fp64,
         "assembly": {"gbs": asm_gbs, "frac_hbm": asm_gbs / PEAK_HBM_GBS, "ms": asm["ms"] / max(asm["launches"], 1)},
         "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()},
+        "block_solve": block_solve,
         "symbolic_s": t_symbolic,
         "autotune": tune,
         "p50_latency_ms": serve["p50_latency_ms"] if serve else None,
