@@ -412,6 +412,154 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
     }
 }
 
+// The same super-panel solves for launches with few fronts (the top of the
+// tree), where the sequential tile walk is the whole cost.  One 512-thread
+// block per (front entry, group of CG columns): wave q owns the q-th 64-row
+// tile of the super-panel (lane = row) and keeps its rows of the CG columns
+// in registers.  At step t the wave owning tile t applies the tile inverse
+// and publishes x_t through LDS; after one barrier the waves of the other
+// tiles subtract their block L(q, t) x_t (U(q, t) going up), whose factor
+// entries they loaded during the previous step.  The inverse of the next
+// tile is staged in LDS by the whole block one step ahead, loaded into
+// registers two steps ahead; the forward solve stores it with its columns in
+// the pivot order (column pr[k] holds column k), so that the owner applies
+// L_t^{-1} P_t to its rows as they stand.  A step's critical path is the
+// owner's 64 x CG products, one barrier and the next owner's 64 x CG
+// products.
+// acc += a x and acc -= a x as chains of fused multiply-adds, so that no
+// product is held apart from its accumulator
+__device__ __forceinline__ void cmac(cpx<float>& acc, cpx<float> a, cpx<float> x)
+{
+    acc.re = fmaf(a.re, x.re, acc.re);
+    acc.re = fmaf(-a.im, x.im, acc.re);
+    acc.im = fmaf(a.re, x.im, acc.im);
+    acc.im = fmaf(a.im, x.re, acc.im);
+}
+__device__ __forceinline__ void cmsub(cpx<float>& acc, cpx<float> a, cpx<float> x)
+{
+    acc.re = fmaf(-a.re, x.re, acc.re);
+    acc.re = fmaf(a.im, x.im, acc.re);
+    acc.im = fmaf(-a.re, x.im, acc.im);
+    acc.im = fmaf(-a.im, x.re, acc.im);
+}
+
+template <int CG>
+__device__ __forceinline__ cpx<float> lane_value(const cpx<float> (&v)[CG], int c, int k)
+{
+    return cpx<float>(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[c].re), k)),
+                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[c].im), k)));
+}
+
+template <int CG, bool FWD>
+__global__ __launch_bounds__(512) void sp_wave_kernel(const wt_dfront* __restrict__ fr,
+                                                      const int32_t* __restrict__ ent,
+                                                      const cpx<float>* __restrict__ factor,
+                                                      const cpx<float>* __restrict__ tiles,
+                                                      const int32_t* __restrict__ perm, cpx<float>* __restrict__ y,
+                                                      int64_t n, int nrhs)
+{
+    typedef cpx<float> C;
+    constexpr int NB = WT_NB, NW = WT_NB2 / WT_NB, NTH = 64 * NW;
+    constexpr int IPT = NB * NB / NTH;  // inverse entries staged per thread
+    static_assert(NW == 8 && IPT * NTH == NB * NB, "one wave per tile of a super-panel");
+    __shared__ C Is[2][NB * NB];  // [column][row]
+    __shared__ C Xs[2][NB][CG];
+    const int32_t* e = ent + 3 * blockIdx.y;
+    const wt_dfront F = fr[e[0]];
+    const int64_t K0 = e[1], K1 = e[2];
+    const int c0 = blockIdx.x * CG;
+    const int nc = nrhs - c0 < CG ? nrhs - c0 : CG;
+    const int64_t ld = F.p + F.u;
+    const C* L = factor + F.Loff;
+    C* yb = y + F.pstart + (int64_t)c0 * n;
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    const int nt = (int)((K1 - K0 + NB - 1) / NB);
+    const int64_t r0 = K0 + (int64_t)q * NB;
+    const int wq = q < nt ? (int)(K1 - r0 < NB ? K1 - r0 : NB) : 0;
+    const bool act = lane < wq;
+    const int64_t row = r0 + lane;
+    auto width = [&](int s) { return (int)(K1 - K0 - (int64_t)s * NB < NB ? K1 - K0 - (int64_t)s * NB : NB); };
+
+    C iv[IPT];
+    int id[IPT];  // LDS slot of iv
+    auto load_inv = [&](int s) {
+        const int64_t cb = K0 + (int64_t)s * NB;
+        const int ws = width(s);
+        const C* src = tiles + F.tileoff + (cb / NB) * 2 * NB * NB + (FWD ? 0 : NB * NB);
+        // every load in bounds (clamped), the entries past the tile width zeroed
+#pragma unroll
+        for (int m = 0; m < IPT; ++m) {
+            const int j = tid + m * NTH, k = j / NB, i = j % NB;
+            const bool in = k < ws && i < ws;
+            const C v = src[j];
+            iv[m] = in ? v : C(0.f, 0.f);
+            if (FWD) {
+                const int pk = perm[F.pivoff + cb + (k < ws ? k : ws - 1)];
+                id[m] = (in ? pk : k) * NB + i;
+            } else {
+                id[m] = j;
+            }
+        }
+    };
+    auto store_inv = [&](int buf) {
+#pragma unroll
+        for (int m = 0; m < IPT; ++m) Is[buf][id[m]] = iv[m];
+    };
+    C a[NB];  // this wave's factor block for the next update
+    auto fetch = [&](int s) {
+        const int64_t cb = K0 + (int64_t)s * NB;
+        const int ws = width(s);
+        const C* src = L + cb * ld + (act ? row : r0);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const C v = src[(int64_t)(k < ws ? k : ws - 1) * ld];
+            a[k] = (act && k < ws) ? v : C(0.f, 0.f);
+        }
+    };
+    C acc[CG];
+#pragma unroll
+    for (int c = 0; c < CG; ++c) acc[c] = (act && c < nc) ? yb[(int64_t)c * n + row] : C(0.f, 0.f);
+
+    auto tile_at = [&](int st) { return FWD ? st : nt - 1 - st; };
+    load_inv(tile_at(0));
+    store_inv(0);
+    if (nt > 1) load_inv(tile_at(1));
+    __syncthreads();
+    for (int st = 0; st < nt; ++st) {
+        const int t = tile_at(st), buf = st & 1;
+        if (st + 1 < nt) {
+            store_inv(buf ^ 1);
+            if (st + 2 < nt) load_inv(tile_at(st + 2));
+        }
+        if (q == t) {
+            C x[CG];
+#pragma unroll
+            for (int c = 0; c < CG; ++c) x[c] = C(0.f, 0.f);
+            const C* I = Is[buf];
+#pragma unroll 4
+            for (int k = 0; k < NB; ++k) {
+                const C lk = I[k * NB + lane];
+#pragma unroll
+                for (int c = 0; c < CG; ++c) cmac(x[c], lk, lane_value<CG>(acc, c, k));
+            }
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                Xs[buf][lane][c] = x[c];
+                if (act && c < nc) yb[(int64_t)c * n + row] = x[c];
+            }
+        }
+        __syncthreads();
+        if (FWD ? (q > t && q < nt) : (q < t)) {
+            fetch(t);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+#pragma unroll
+                for (int c = 0; c < CG; ++c) cmsub(acc[c], a[k], Xs[buf][k][c]);
+            }
+        }
+    }
+}
+
 // ---- solve-side gathers / scatters (block of nrhs columns) ----
 
 // y (elimination order, ld = n) += children's update rows that land on the
@@ -579,6 +727,30 @@ int wt_sp_solve(int fwd, const wt_dfront* fr, const int32_t* ent, int nent, cons
     // with few fronts (the top of the tree) use 4-column groups so that more
     // blocks share the sequential tile walk
     const bool wide = (int64_t)nent * ((nrhs + 15) / 16) >= 1024;
+    if constexpr (sizeof(R) == 4)
+    if (!wide) {
+        // one wave per tile; fewer columns per block where fewer fronts share the launch
+        const int cg = nent <= 2 ? 1 : nent <= 8 ? 2 : 4;
+        const dim3 g2((nrhs + cg - 1) / cg, nent);
+        if (nent > 65535) return WT_ERR_ARG;
+        const cpx<float>* fa = (const cpx<float>*)factor;
+        const cpx<float>* ta = (const cpx<float>*)tiles;
+        cpx<float>* ya = (cpx<float>*)y;
+#define WT_SPW(CG, FW) \
+    hipLaunchKernelGGL((sp_wave_kernel<CG, FW>), g2, dim3(512), 0, s, fr, ent, fa, ta, perm, ya, n, nrhs)
+        if (fwd) {
+            if (cg == 1) WT_SPW(1, true);
+            else if (cg == 2) WT_SPW(2, true);
+            else WT_SPW(4, true);
+        } else {
+            if (cg == 1) WT_SPW(1, false);
+            else if (cg == 2) WT_SPW(2, false);
+            else WT_SPW(4, false);
+        }
+#undef WT_SPW
+        WT_HIP(hipGetLastError());
+        return WT_OK;
+    }
     const int cg = wide ? 16 : 4;
     dim3 grid(nent, (nrhs + cg - 1) / cg);
     if (fwd) {
