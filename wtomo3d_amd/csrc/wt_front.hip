@@ -418,14 +418,15 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
 // tile of the super-panel (lane = row) and keeps its rows of the CG columns
 // in registers.  At step t the wave owning tile t applies the tile inverse
 // and publishes x_t through LDS; after one barrier the waves of the other
-// tiles subtract their block L(q, t) x_t (U(q, t) going up), whose factor
-// entries they loaded during the previous step.  The inverse of the next
-// tile is staged in LDS by the whole block one step ahead, loaded into
-// registers two steps ahead; the forward solve stores it with its columns in
-// the pivot order (column pr[k] holds column k), so that the owner applies
-// L_t^{-1} P_t to its rows as they stand.  A step's critical path is the
-// owner's 64 x CG products, one barrier and the next owner's 64 x CG
-// products.
+// tiles subtract their block L(q, t) x_t (U(q, t) going up), all 64 factor
+// entries of a lane loaded at once.  The inverse of the next tile is staged
+// in LDS by the whole block one step ahead, loaded into registers two steps
+// ahead; the forward solve stores it with its columns in the pivot order
+// (column pr[k] holds column k), so that the owner applies L_t^{-1} P_t to
+// its rows as they stand.  A step's critical path is the owner's 64 x CG
+// products, one barrier, one factor load and the next owner's 64 x CG
+// products.  (Holding the next step's factor block in registers across the
+// barrier needs 128 more registers per lane than 512-thread blocks leave.)
 // acc += a x and acc -= a x as chains of fused multiply-adds, so that no
 // product is held apart from its accumulator
 __device__ __forceinline__ void cmac(cpx<float>& acc, cpx<float> a, cpx<float> x)
