@@ -221,3 +221,28 @@ def test_all_pml_uniform_model(prec):
     err_oracle = np.max(np.abs(Xo - Xt)) / np.max(np.abs(Xt))
     assert np.max(np.abs(X - Xt)) / np.max(np.abs(Xt)) <= 10 * err_oracle + 1e-12
     s.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_singular_operator_reports_zero_pivot(prec):
+    """A CSR row of zeros makes the operator exactly singular: every update
+    that row receives is zero, so its diagonal tile meets an exactly zero
+    pivot and the factorization returns WT_ERR_PIVOT (7).  The all-zero
+    operator does the same from the first tile on, and the handle factors
+    normally again once good values are set."""
+    nx, ny, nz = 10, 9, 11
+    s, A, vals = _setup(nx, ny, nz, 64, seed=6)
+    ir = A.indptr
+    r = 3 * (5 + 4 * nz + 3 * nx * nz) + 1  # an interior unknown
+    v = vals.copy()
+    v[ir[r]:ir[r + 1]] = 0
+    s.set_values(v)
+    assert W.lib().wt_solver_factor(s._h, prec) == 7
+    s.set_values(np.zeros_like(vals))
+    assert W.lib().wt_solver_factor(s._h, prec) == 7
+    s.set_values(vals)
+    s.factor(prec)
+    B, Xt = rhs_block(A, 2, 31)
+    X, st = s.solve(B, prec=prec, tol=1e-10, max_iter=30)
+    assert st["rel_residual"] <= 1e-10
+    s.close()

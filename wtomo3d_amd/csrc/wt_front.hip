@@ -253,6 +253,160 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restr
     store_inv(X[NX - 1], Ui);
 }
 
+// acc += a x and acc -= a x as chains of fused multiply-adds, so that no
+// product is held apart from its accumulator
+__device__ __forceinline__ void cmac(cpx<float>& acc, cpx<float> a, cpx<float> x)
+{
+    acc.re = fmaf(a.re, x.re, acc.re);
+    acc.re = fmaf(-a.im, x.im, acc.re);
+    acc.im = fmaf(a.re, x.im, acc.im);
+    acc.im = fmaf(a.im, x.re, acc.im);
+}
+__device__ __forceinline__ void cmsub(cpx<float>& acc, cpx<float> a, cpx<float> x)
+{
+    acc.re = fmaf(-a.re, x.re, acc.re);
+    acc.re = fmaf(a.im, x.im, acc.re);
+    acc.im = fmaf(-a.re, x.im, acc.im);
+    acc.im = fmaf(-a.im, x.re, acc.im);
+}
+
+// The same tile LU and inverses in FP32 with the tile in registers: one
+// 128-thread block per tile.  Wave 0 holds row `lane` of the tile in 64
+// complex registers and eliminates without moving rows: at step j the pivot
+// is the not-yet-chosen row with the largest |re|+|im| in column j (the
+// lowest row on ties), its row is read across lanes with v_readlane, and the
+// other open rows take their multiplier and update in place.  The row chosen
+// at step j becomes position j, so every lane ends up holding its own row of
+// the packed L\U and stores it straight to its position; there are no block
+// barriers inside the elimination.  Both inverses then run as the same
+// unit-lower forward substitution with lane = column, fully unrolled over
+// registers with the matrix entries as LDS broadcasts: wave 0 inverts L,
+// wave 1 inverts U~^T where U = D U~ (D the pivots), which yields the rows of
+// U~^{-1} and so U^{-1} = U~^{-1} D^{-1} with coalesced stores; L^{-1} goes out
+// through an LDS transpose.  Entries outside the w x w tile are stored as 0.
+__device__ __forceinline__ float wave_max(float v)
+{
+    // max within rows of 16 by DPP, then across the four rows
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false)));
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+__device__ __forceinline__ cpx<float> read_lane(cpx<float> v, int l)
+{
+    return cpx<float>(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.re), l)),
+                      __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.im), l)));
+}
+
+__global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __restrict__ fr,
+                                                             const int32_t* __restrict__ list, int64_t kb,
+                                                             cpx<float>* __restrict__ factor,
+                                                             cpx<float>* __restrict__ tiles,
+                                                             int32_t* __restrict__ perm, int* __restrict__ status)
+{
+    typedef cpx<float> C;
+    constexpr int NB = WT_NB;
+    __shared__ C M[2][NB][NB + 1];  // [0]: L[i][k] at [k][i], later L^{-1} [i][c]; [1]: U~^T[i][k] at [k][i]
+    __shared__ float Dre[NB], Dim[NB];  // 1 / pivot
+    const wt_dfront F = fr[list[blockIdx.x]];
+    if (kb >= F.p) return;
+    const int w = (int)(F.p - kb < NB ? F.p - kb : NB);
+    const int64_t ld = F.p + F.u;
+    C* base = factor + F.Loff + kb * ld + kb;
+    const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+    if (wq == 0) {
+        C a[NB];
+        {
+            const C* src = base + (lane < w ? lane : 0);
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const C v = src[(int64_t)(c < w ? c : w - 1) * ld];
+                a[c] = (lane < w && c < w) ? v : C(0.f, 0.f);
+            }
+        }
+        bool open = lane < w;  // not yet chosen as a pivot row
+        int pos = lane;        // position of this row once chosen
+        C dme(1.f, 0.f);       // 1 / own pivot
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < w) {
+                const float m = open ? fabsf(a[j].re) + fabsf(a[j].im) : -1.f;
+                const float mx = wave_max(m);
+                uint64_t hit = __ballot(open && m == mx);
+                if (hit == 0) hit = __ballot(open);  // NaN column: first open row
+                const int pv = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(hit));
+                C d = read_lane(a[j], pv);
+                if (d.re == 0.f && d.im == 0.f) {
+                    if (lane == 0) atomicOr(status, 1);
+                    d = C(1e-30f, 0.f);
+                }
+                const C dinv = cdiv(C(1.f, 0.f), d);
+                const bool elim = open && lane != pv;
+                if (lane == pv) {
+                    open = false;
+                    pos = j;
+                    dme = dinv;
+                }
+                if (lane == 0) { Dre[j] = dinv.re; Dim[j] = dinv.im; }
+                const C l = a[j] * dinv;
+                if (elim) a[j] = l;
+#pragma unroll
+                for (int c = j + 1; c < NB; ++c) {
+                    const C pc = read_lane(a[c], pv);
+                    if (elim) cmsub(a[c], l, pc);
+                }
+            } else if (lane == 0) {
+                Dre[j] = 0.f;
+                Dim[j] = 0.f;
+            }
+        }
+        if (lane < w) perm[F.pivoff + kb + pos] = lane;
+        // packed L\U to its positions; L below the diagonal and U~^T = (D^{-1} U)^T to LDS
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (lane < w && c < w) base[(int64_t)c * ld + pos] = a[c];
+            M[0][c][pos] = c < pos ? a[c] : C(0.f, 0.f);
+            M[1][pos][c] = c > pos ? dme * a[c] : C(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    // x = column `lane` of the inverse of the unit-lower M[wq]
+    const C (*Mw)[NB + 1] = M[wq];
+    C x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = C(i == lane ? 1.f : 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < NB - 1; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < NB; ++i) cmsub(x[i], Mw[k][i], x[k]);
+    }
+    C* dst = tiles + F.tileoff + (kb / NB) * 2 * NB * NB;
+    if (wq == 1) {
+        // x[i] = U~^{-1}[lane][i]; U^{-1}[lane][i] = x[i] / pivot_i, column-major at i * NB + lane
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const C v = x[i] * C(Dre[i], Dim[i]);
+            dst[NB * NB + i * NB + lane] = (lane < w && i < w) ? v : C(0.f, 0.f);
+        }
+    }
+    __syncthreads();  // wave 1 is done with M[1]; wave 0 has read all of M[0]
+    if (wq == 0) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) M[0][i][lane] = x[i];
+    }
+    __syncthreads();
+    // L^{-1} column-major: both waves store half of the columns
+#pragma unroll 4
+    for (int cc = wq; cc < NB; cc += 2)
+        dst[cc * NB + lane] = (lane < w && cc < w) ? M[0][lane][cc] : C(0.f, 0.f);
+}
+
 // ---- super-panel triangular solves of the block solve ----
 //
 // One launch covers one super-panel (<= WT_NB2 pivots) of every listed front
@@ -427,23 +581,6 @@ __global__ __launch_bounds__(256) void sp_bwd_kernel(const wt_dfront* __restrict
 // products, one barrier, one factor load and the next owner's 64 x CG
 // products.  (Holding the next step's factor block in registers across the
 // barrier needs 128 more registers per lane than 512-thread blocks leave.)
-// acc += a x and acc -= a x as chains of fused multiply-adds, so that no
-// product is held apart from its accumulator
-__device__ __forceinline__ void cmac(cpx<float>& acc, cpx<float> a, cpx<float> x)
-{
-    acc.re = fmaf(a.re, x.re, acc.re);
-    acc.re = fmaf(-a.im, x.im, acc.re);
-    acc.im = fmaf(a.re, x.im, acc.im);
-    acc.im = fmaf(a.im, x.re, acc.im);
-}
-__device__ __forceinline__ void cmsub(cpx<float>& acc, cpx<float> a, cpx<float> x)
-{
-    acc.re = fmaf(-a.re, x.re, acc.re);
-    acc.re = fmaf(a.im, x.im, acc.re);
-    acc.im = fmaf(-a.re, x.im, acc.im);
-    acc.im = fmaf(-a.im, x.re, acc.im);
-}
-
 template <int CG>
 __device__ __forceinline__ cpx<float> lane_value(const cpx<float> (&v)[CG], int c, int k)
 {
@@ -686,8 +823,12 @@ int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t k
                   int32_t* perm, int* status, hipStream_t s)
 {
     if (nlist <= 0) return WT_OK;
-    hipLaunchKernelGGL(diag_tile_kernel<R>, dim3(nlist), dim3(256), 0, s, fr, list, kb, factor, tiles, perm,
-                       status);
+    if constexpr (sizeof(R) == 4)
+        hipLaunchKernelGGL(diag_tile_wave_kernel, dim3(nlist), dim3(128), 0, s, fr, list, kb, factor, tiles, perm,
+                           status);
+    else
+        hipLaunchKernelGGL(diag_tile_kernel<R>, dim3(nlist), dim3(256), 0, s, fr, list, kb, factor, tiles, perm,
+                           status);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
