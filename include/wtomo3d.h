@@ -309,7 +309,8 @@ void strumpack_serial_finter(int *job, int *nrows, int *nnz, int *irptr, int *jc
 int strumpack_parallel_finter(void);
 
 /* Batched complex GEMM of the front kernels on one problem, host arrays
- * (column-major): C = A B (mode 1), C -= A B (0), C += A B (2); optional
+ * (column-major): C = A B (mode 1), C -= A B (0), C += A B (2), C = -A B
+ * (3; other modes are WT_ERR_ARG); optional
  * row permutation of B (bperm[k] = source row).  Runs `reps` times (C is
  * re-uploaded each time) and reports the last run's kernel time in *ms.
  * No counterpart in the reference: a test and measurement entry for the

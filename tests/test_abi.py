@@ -38,3 +38,12 @@ def test_closed_form_nnz_from_library():
     L = _lib.lib()
     assert L.wt_graph_nnz(96, 96, 64) == 9 * 286 * 286 * 190
     assert L.wt_graph_nnz(0, 4, 4) == 0
+
+
+def test_gemm_entry_rejects_unknown_modes_before_any_device_call():
+    import ctypes
+    L = _lib.lib()
+    ms = ctypes.c_double()
+    for mode in (-1, 4, 7):
+        rc = L.wt_gemm_host_var(32, -1, 4, 4, 4, None, 4, None, 4, None, 4, mode, None, 1, ctypes.byref(ms))
+        assert rc == 1  # WT_ERR_ARG

@@ -23,7 +23,7 @@ def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0, 
     if perm:
         Bm = Bm[bp, :]
     Cm = C[:, :M].T.astype(np.complex128)
-    ref = {0: Cm - Am @ Bm, 1: Am @ Bm, 2: Cm + Am @ Bm}[mode]
+    ref = {0: Cm - Am @ Bm, 1: Am @ Bm, 2: Cm + Am @ Bm, 3: -(Am @ Bm)}[mode]
     Cout = C.copy()
     ms = ctypes.c_double()
     rc = W.lib().wt_gemm_host_var(prec, var, M, N, K, P(A), lda, P(B), ldb, P(Cout), ldc, mode,
@@ -39,7 +39,7 @@ def _run(prec, M, N, K, mode, lda=None, ldb=None, ldc=None, perm=False, seed=0, 
 @pytest.mark.parametrize("prec", [32, 64])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (33, 500, 17),
                                    (600, 64, 200), (513, 37, 70)])
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_gemm_modes_shapes(prec, shape, mode):
     M, N, K = shape
     err = _run(prec, M, N, K, mode, lda=M + 3, ldb=K + 1, ldc=M + 5)

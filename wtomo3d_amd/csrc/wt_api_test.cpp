@@ -59,6 +59,7 @@ extern "C" int wt_gemm_host_var(int prec, int var, int M, int N, int K, const vo
                                 int64_t ldb, void* C, int64_t ldc, int mode, const int32_t* bperm, int reps,
                                 double* ms)
 {
+    if (mode < WT_GEMM_SUB || mode > WT_GEMM_NSET) return WT_ERR_ARG;
     if (prec == WT_PREC_F32) return run_gemm<float>(var, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
     if (prec == WT_PREC_F64) return run_gemm<double>(var, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
     return WT_ERR_ARG;

@@ -30,7 +30,7 @@ int wt_scatter_rows(const int32_t* rowptr, const cpx<double>* vals, const int64_
                     int64_t n, cpx<R>* arena, hipStream_t s);
 template <typename R>
 int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
-                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, hipStream_t s);
+                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, int part, hipStream_t s);
 template <typename R>
 int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t kb, cpx<R>* factor, cpx<R>* tiles,
                   int32_t* perm, int* status, hipStream_t s);

@@ -76,14 +76,19 @@ __global__ __launch_bounds__(256) void scatter_rows_kernel(const int32_t* __rest
 constexpr int EA_COLS = 8;  // CB columns per extend-add block
 
 // Adds the contribution blocks of the listed children into their parents.
-// At most one child per parent is listed, so the adds do not race.
+// At most one child per parent is listed, so the adds do not race.  The
+// child-to-parent map ascends, so the child's indices below q (the first
+// one past the parent's pivots) land on the parent's pivot rows/columns:
+// part 1 adds every entry with a row or column below q (into the parent's
+// L and U blocks), part 2 the rest (into the parent's contribution block),
+// part 0 everything.
 template <typename R>
 __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __restrict__ fr,
                                                          const int32_t* __restrict__ kids,
                                                          const int32_t* __restrict__ relmap,
                                                          const cpx<R>* __restrict__ cb_child,
                                                          cpx<R>* __restrict__ factor,
-                                                         cpx<R>* __restrict__ cb_parent)
+                                                         cpx<R>* __restrict__ cb_parent, int part)
 {
     const wt_dfront C = fr[kids[blockIdx.x]];
     const int64_t uc = C.u;
@@ -91,15 +96,26 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __rest
     if (j0 >= uc) return;
     const wt_dfront P = fr[C.parent];
     const int32_t* map = relmap + C.moff;
+    const int64_t pp = P.p, ld = P.p + P.u, pu = P.u;
+    int64_t q = 0;
+    if (part) {
+        int64_t hi = uc;
+        while (q < hi) {
+            const int64_t mid = (q + hi) >> 1;
+            if (map[mid] < pp) q = mid + 1;
+            else hi = mid;
+        }
+    }
+    const int64_t j1 = j0 + EA_COLS < uc ? j0 + EA_COLS : uc;
+    if (part == 2 && j1 <= q) return;
     const cpx<R>* cb = cb_child + C.CBoff;
     cpx<R>* Lp = factor + P.Loff;
     cpx<R>* Up = factor + P.Uoff;
     cpx<R>* Cp = cb_parent + P.CBoff;
-    const int64_t pp = P.p, ld = P.p + P.u, pu = P.u;
-    const int64_t j1 = j0 + EA_COLS < uc ? j0 + EA_COLS : uc;
-    for (int64_t j = j0; j < j1; ++j) {
+    for (int64_t j = (part == 2 && j0 < q) ? q : j0; j < j1; ++j) {
         const int64_t s = map[j];
-        for (int64_t i = threadIdx.x; i < uc; i += blockDim.x) {
+        const int64_t i0 = part == 2 ? q : 0, i1 = (part == 1 && j >= q) ? q : uc;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             const int64_t r = map[i];
             const cpx<R> v = cb[j * uc + i];
             cpx<R>* t;
@@ -354,13 +370,11 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
                     dme = dinv;
                 }
                 if (lane == 0) { Dre[j] = dinv.re; Dim[j] = dinv.im; }
-                const C l = a[j] * dinv;
+                // rows not eliminated here take l = 0 and keep their values
+                const C l = elim ? a[j] * dinv : C(0.f, 0.f);
                 if (elim) a[j] = l;
 #pragma unroll
-                for (int c = j + 1; c < NB; ++c) {
-                    const C pc = read_lane(a[c], pv);
-                    if (elim) cmsub(a[c], l, pc);
-                }
+                for (int c = j + 1; c < NB; ++c) cmsub(a[c], l, read_lane(a[c], pv));
             } else if (lane == 0) {
                 Dre[j] = 0.f;
                 Dim[j] = 0.f;
@@ -809,11 +823,13 @@ int wt_scatter_rows(const int32_t* rowptr, const cpx<double>* vals, const int64_
 
 template <typename R>
 int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
-                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, hipStream_t s)
+                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, int part, hipStream_t s)
 {
     if (nkids <= 0 || max_u <= 0) return WT_OK;
+    if (part < 0 || part > 2) return WT_ERR_ARG;
     dim3 grid(nkids, (unsigned)((max_u + EA_COLS - 1) / EA_COLS));
-    hipLaunchKernelGGL(extend_add_kernel<R>, grid, dim3(256), 0, s, fr, kids, relmap, cb_child, factor, cb_parent);
+    hipLaunchKernelGGL(extend_add_kernel<R>, grid, dim3(256), 0, s, fr, kids, relmap, cb_child, factor, cb_parent,
+                       part);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
@@ -917,7 +933,7 @@ template int wt_sp_solve<double>(int, const wt_dfront*, const int32_t*, int, con
 #define WT_INST(R)                                                                                              \
     template int wt_fill_zero<R>(cpx<R>*, int64_t, hipStream_t);                                                \
     template int wt_extend_add<R>(const wt_dfront*, const int32_t*, int, int64_t, const int32_t*, const cpx<R>*, \
-                                  cpx<R>*, cpx<R>*, hipStream_t);                                               \
+                                  cpx<R>*, cpx<R>*, int, hipStream_t);                                          \
     template int wt_diag_tiles<R>(const wt_dfront*, const int32_t*, int, int64_t, cpx<R>*, cpx<R>*, int32_t*,   \
                                   int*, hipStream_t);                                                           \
     template int wt_solve_gather<R>(const wt_dfront*, const int32_t*, int, const int32_t*, const cpx<R>*,       \

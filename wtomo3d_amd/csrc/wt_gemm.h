@@ -11,6 +11,7 @@ enum wt_gemm_mode : int32_t {
     WT_GEMM_SET = 1,  // C  = A B   (C may alias B when one tile spans all M,
                       //             or alias A when one tile spans all N)
     WT_GEMM_ADD = 2,  // C += A B
+    WT_GEMM_NSET = 3, // C = -A B
 };
 
 // One problem of a batch.  Column-major complex operands.

@@ -227,6 +227,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kerne
                     gv2<R>* pc = gC + (int64_t)n * P.ldc + m;
                     if (mode == WT_GEMM_SET) {
                         *pc = v2<R>{re, im};
+                    } else if (mode == WT_GEMM_NSET) {
+                        *pc = v2<R>{-re, -im};
                     } else {
                         const v2<R> c = *pc;
                         if (mode == WT_GEMM_SUB) *pc = v2<R>{c.x - re, c.y - im};

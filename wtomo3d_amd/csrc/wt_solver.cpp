@@ -291,16 +291,23 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
     const int64_t NB2 = WT_NB2;
     num.fplan.release();
     num.fsteps.clear();
+    // The contribution blocks need no clearing: each front's CB is first
+    // written by its own CB = -L21 U12 product, and its children's
+    // contributions are split around it (into the pivot blocks before the
+    // front is factored, into the CB after the product).
+    auto extend_add = [&](int l, int part) {
+        if (l + 1 >= nl) return;
+        for (int k = 0; k < 2; ++k)
+            if (s.lev_kids_cnt[k][l + 1] > 0) {
+                wt_step t{WT_STEP_EXTADD, s.lev_kids_cnt[k][l + 1], s.lev_kids_maxu[k][l + 1],
+                          s.lev_kids_off[k][l + 1], l & 1, -1};
+                t.part = part;
+                num.fsteps.push_back(t);
+            }
+    };
     for (int l = nl - 1; l >= 0; --l) {
         const int par = l & 1;
-        int64_t cbl = 0;
-        for (int f : S.levels[l]) cbl = std::max(cbl, S.fronts[f].CBoff + S.fronts[f].u * S.fronts[f].u);
-        if (cbl > 0) num.fsteps.push_back({WT_STEP_ZERO, 0, cbl, 0, par, -1});
-        if (l + 1 < nl)
-            for (int k = 0; k < 2; ++k)
-                if (s.lev_kids_cnt[k][l + 1] > 0)
-                    num.fsteps.push_back({WT_STEP_EXTADD, s.lev_kids_cnt[k][l + 1], s.lev_kids_maxu[k][l + 1],
-                                          s.lev_kids_off[k][l + 1], par, -1});
+        extend_add(l, 1);
         const std::vector<int64_t>& ps = s.lev_sorted_p[l];
         const int32_t* lst = s.hlists.data() + s.lev_sorted_off[l];
         for (int64_t kb = 0; kb < ps[0]; kb += nb) {
@@ -360,17 +367,18 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
             g = num.fplan.end();
             if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
         }
-        // contribution blocks: CB -= L21 U12
+        // contribution blocks: CB = -L21 U12, then the children's part
         num.fplan.begin();
         for (int f : S.levels[l]) {
             const wt_front& F = S.fronts[f];
             if (F.u == 0) continue;
             const int64_t p = F.p, u = F.u, ld = p + u;
             num.fplan.add(num.factor + F.Loff + p, ld, num.factor + F.Uoff, p, num.cb[par] + F.CBoff, u, u, u, p,
-                          WT_GEMM_SUB);
+                          WT_GEMM_NSET);
         }
         int g = num.fplan.end();
         if (g >= 0) num.fsteps.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});
+        extend_add(l, 2);
     }
     return num.fplan.upload();
 }
@@ -426,7 +434,7 @@ static int enqueue_factor(wt_solver_impl& s, wt_numeric<R>& num, std::vector<hip
         case WT_STEP_EXTADD:
             rc = P.run(st, WT_PROF_EXTADD, 0, 0, [&] {
                 return wt_extend_add<R>(s.fronts, s.lists + t.list_off, t.count, t.a, s.relmap, num.cb[t.buf ^ 1],
-                                        num.factor, num.cb[t.buf], st);
+                                        num.factor, num.cb[t.buf], t.part, st);
             });
             break;
         case WT_STEP_DIAG:

@@ -99,6 +99,7 @@ struct wt_step {
     int64_t list_off;    // offset into the device list array
     int32_t buf;         // arena parity (ZERO: which arena; EXTADD: parent parity)
     int32_t gemm;        // GEMM launch index
+    int32_t part = 0;    // EXTADD: 1 = into the parents' pivot blocks, 2 = into their CBs
 };
 
 template <typename R>
