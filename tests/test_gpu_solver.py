@@ -246,3 +246,58 @@ def test_singular_operator_reports_zero_pivot(prec):
     X, st = s.solve(B, prec=prec, tol=1e-10, max_iter=30)
     assert st["rel_residual"] <= 1e-10
     s.close()
+
+
+def test_fp32_factor_as_accurate_as_dense_partial_pivoting():
+    """The complex64 multifrontal factor (pivoting inside 64-row tiles, 3M
+    products, tile inverses) against a dense complex64 LU with partial
+    pivoting over whole columns (LAPACK through scipy) of the same
+    row-equilibrated operator: the first solve's residual, measured in the
+    equilibrated rows (powers of two, as the solver scales them), is of the
+    same size.  An all-PML 10^3 model, where the unscaled row norms span six
+    orders of magnitude."""
+    import scipy.linalg as sla
+    n = 10
+    prm, lam, mu, rho, ir, jc, vals = problem(n, n, n, dx=200.0, freq=0.3, seed=3, perturb=0.05, pmld=2000.0)
+    A = SO.csr(ir, jc, vals)
+    Ad = A.toarray()
+    m = np.maximum(np.abs(Ad.real), np.abs(Ad.imag)).max(axis=1)
+    _, ex = np.frexp(m)
+    D = np.ldexp(1.0, 1 - ex)
+    DA = Ad * D[:, None]
+    rng = np.random.default_rng(17)
+    B = rng.standard_normal((A.shape[0], 4)) + 1j * rng.standard_normal((A.shape[0], 4))
+    lu = sla.lu_factor(DA.astype(np.complex64), check_finite=False)
+    Xc = sla.lu_solve(lu, (D[:, None] * B).astype(np.complex64), check_finite=False).astype(np.complex128)
+
+    def scaled_res(X):
+        R = D[:, None] * (B - Ad @ X)
+        return np.max(np.linalg.norm(R, axis=0) / np.linalg.norm(D[:, None] * B, axis=0))
+
+    s = Solver(n, n, n)
+    s.set_model(lam, mu, rho)
+    s.assemble(W._lib.WtGridParams(n, n, n, prm["ipml"], 1, 1, prm["dx"], prm["pmld"], prm["pmlf"], prm["omega"],
+                                   0.0))
+    s.factor(32)
+    X, st = s.solve(B, prec=32, tol=0.0, max_iter=1)
+    s.close()
+    r_dense, r_gpu = scaled_res(Xc), scaled_res(X)
+    assert r_dense < 1e-4
+    assert r_gpu <= 10 * r_dense, (r_gpu, r_dense)
+
+
+@pytest.mark.parametrize("nrhs", [64, 40, 33, 32])
+def test_wide_block_residual_matches_numpy(nrhs):
+    """Blocks of more than 32 columns take the node-wise residual kernel,
+    32 and fewer the row-wise one: the FP64 residual the refinement reports
+    (and stops on) agrees with numpy's on the returned solution, and the
+    block solves each column as it would alone."""
+    s, A, vals = _setup(6, 5, 7, 64, seed=8)
+    B, Xt = rhs_block(A, nrhs, 41)
+    s.factor(32)
+    X, st = s.solve(B, prec=32, tol=0.0, max_iter=2)
+    r_np = SO.rel_residual(A, X, B)
+    assert abs(st["rel_residual"] - r_np) <= 1e-6 * r_np + 1e-15
+    x3, _ = s.solve(B[:, 3], prec=32, tol=0.0, max_iter=2)
+    assert np.array_equal(x3, X[:, 3])
+    s.close()

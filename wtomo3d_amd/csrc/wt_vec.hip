@@ -73,6 +73,17 @@ __global__ __launch_bounds__(256) void from_elim_kernel(const cpx<R>* __restrict
 // Per-block partial sums of |r|^2 per column go to `partial`.
 constexpr int SP_ROWS = 4;  // rows per block (one per wave)
 
+// acc += sg * a x with the rounding fixed by explicit fused multiply-adds,
+// so that every residual kernel forms each term identically.
+template <typename Ra>
+__device__ __forceinline__ void res_term(cpx<double>& acc, const cpx<Ra> a, const cpx<double> xv, double sg)
+{
+    const double tr = fma((double)a.re, xv.re, -((double)a.im * xv.im));
+    const double ti = fma((double)a.re, xv.im, (double)a.im * xv.re);
+    acc.re += sg * tr;
+    acc.im += sg * ti;
+}
+
 // r = b - A x (or A x when b is null) for a row-major block of nrhs <= CW
 // columns: each wave handles 64 / CW rows, CW lanes per row (one per
 // column), so few right-hand sides still fill the wave.  Every row of the
@@ -102,10 +113,7 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
     if (row < n && c < nrhs) {
         cpx<double> acc = b ? b[row * nrhs + c] : cpx<double>(0.0, 0.0);
         const double sg = b ? -1.0 : 1.0;
-        auto term = [&](const cpx<Ra> a, const cpx<double> xv) {
-            acc.re += sg * ((double)a.re * xv.re - (double)a.im * xv.im);
-            acc.im += sg * ((double)a.re * xv.im + (double)a.im * xv.re);
-        };
+        auto term = [&](const cpx<Ra> a, const cpx<double> xv) { res_term(acc, a, xv, sg); };
         const int32_t e0 = rowptr[row], e1 = rowptr[row + 1];
         if ((e1 - e0) % 3 == 0) {
             for (int32_t e = e0; e < e1; e += 3) {
@@ -128,6 +136,73 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
         double t = 0.0;
         for (int w = 0; w < SP_ROWS; ++w)
             for (int k = 0; k < RPW; ++k) t += s_part[w * 64 + k * CW + threadIdx.x];
+        partial[blk * nrhs + threadIdx.x] = t;
+    }
+}
+
+// The same residual for blocks of more than 32 columns, one wave per node
+// (its three rows), lane = column.  The three rows of a node have the same
+// columns, and each neighbour node's x rows (three 64-wide segments) are
+// loaded once for the node instead of once per row, a third of the x
+// traffic of the row kernel; each row adds its terms in the same order.
+// The solver's CSR gives the three rows of a node the same columns; a node
+// whose rows do not share one triplet structure is done row by row.
+template <typename Ra>
+__global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __restrict__ rowptr,
+                                                            const int32_t* __restrict__ colidx,
+                                                            const cpx<Ra>* __restrict__ vals,
+                                                            const cpx<double>* __restrict__ x,
+                                                            const cpx<double>* __restrict__ b,
+                                                            cpx<double>* __restrict__ r,
+                                                            double* __restrict__ partial, int64_t nnode, int nrhs)
+{
+    __shared__ double s_part[4 * 64];
+    const int wave = threadIdx.x >> 6, c = threadIdx.x & 63;
+    int64_t blk = blockIdx.x;
+    {
+        const int64_t nb = gridDim.x, q = nb / 8, rr = nb % 8, xx = blk % 8, i = blk / 8;
+        blk = xx * q + (xx < rr ? xx : rr) + i;
+    }
+    const int64_t node = blk * 4 + wave;
+    double nrm = 0.0;
+    if (node < nnode && c < nrhs) {
+        const int64_t row0 = 3 * node;
+        const double sg = b ? -1.0 : 1.0;
+        cpx<double> acc[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] = b ? b[(row0 + k) * nrhs + c] : cpx<double>(0.0, 0.0);
+        auto term = [&](cpx<double>& a_, const cpx<Ra> a, const cpx<double> xv) { res_term(a_, a, xv, sg); };
+        const int32_t e0 = rowptr[row0], e1 = rowptr[row0 + 1], e2 = rowptr[row0 + 2], e3 = rowptr[row0 + 3];
+        const int32_t len = e1 - e0;
+        if (len == e2 - e1 && len == e3 - e2 && len % 3 == 0) {
+            for (int32_t t = 0; t < len; t += 3) {
+                const cpx<double>* xp = x + (int64_t)colidx[e0 + t] * nrhs + c;
+                const cpx<double> x0 = xp[0], x1 = xp[nrhs], x2 = xp[2 * nrhs];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int32_t e = e0 + k * len + t;
+                    term(acc[k], vals[e], x0);
+                    term(acc[k], vals[e + 1], x1);
+                    term(acc[k], vals[e + 2], x2);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                for (int32_t e = rowptr[row0 + k]; e < rowptr[row0 + k + 1]; ++e)
+                    term(acc[k], vals[e], x[(int64_t)colidx[e] * nrhs + c]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            r[(row0 + k) * nrhs + c] = acc[k];
+            nrm += acc[k].re * acc[k].re + acc[k].im * acc[k].im;
+        }
+    }
+    s_part[threadIdx.x] = nrm;
+    __syncthreads();
+    if ((int)threadIdx.x < nrhs) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += s_part[w * 64 + threadIdx.x];
         partial[blk * nrhs + threadIdx.x] = t;
     }
 }
@@ -305,6 +380,15 @@ int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* val
                 hipStream_t s)
 {
     if (nrhs < 1 || nrhs > 64) return WT_ERR_ARG;  // one wave per row covers up to 64 columns
+    if (nrhs > 32 && n % 3 == 0) {
+        const int64_t nb = (n / 3 + 3) / 4;  // <= wt_residual_blocks(n)
+        hipLaunchKernelGGL((residual_node_kernel<Ra>), dim3((unsigned)nb), dim3(256), 0, s, rowptr, colidx, vals, x,
+                           b, r, partial, n / 3, nrhs);
+        WT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
+        WT_HIP(hipGetLastError());
+        return WT_OK;
+    }
     int cw = 1;
     while (cw < nrhs) cw <<= 1;
     const int64_t rows_per_blk = SP_ROWS * (64 / cw);
