@@ -7,24 +7,25 @@
 // kernel reproduces SREL3D (srgraphs.f:638-734) followed by SPARSE_ADJ2CRS
 // (sparse.f:13-49) from closed forms.
 //
-// Design: one wave of 64 lanes per (x, y) column segment of 64 z-nodes; the
-// 3x3 column neighbourhood of lambda, mu, rho is staged in LDS; each lane owns
-// one node, forms the octant-cell moduli and stretch factors in registers and
-// walks the 27 neighbour blocks with a wave-uniform label, so the whole wave
-// evaluates one formula at a time (no divergence).  The interior stencil is
-// the stretched form with r1 = 1, r2 = 0 (identical values; the reference
-// notes the identity at fdfd3d.f:56-57).  CSR offsets follow from closed
-// forms, so no atomics or scans.
+// Design: a block of 256 threads owns a chunk of 16 z-nodes of one (x, y)
+// column.  The clamped 3x3 column neighbourhood of lambda, mu, rho for the
+// chunk is staged in LDS.  A node's output is 3 rows of up to 81 entries
+// (neighbours ascending in y, x, z), and the rows of the chunk's nodes follow
+// each other in the CSR arrays, so the chunk's output is one contiguous span.
+// The block builds that span in LDS -- 16 lanes per neighbour label, each
+// lane one node, the label uniform over the wave's turn so only its formula
+// runs -- and writes it front to back, consecutive threads on consecutive
+// entries.  The interior stencil is the stretched form with r1 = 1, r2 = 0
+// (identical values; the reference notes the identity at fdfd3d.f:56-57).
+// CSR offsets follow from closed forms, so no atomics or scans.
 //
-// The kernel is bound by its stores.  A node's output is 3 rows of up to 81
-// entries, and the rows of consecutive z-nodes are 3 rows apart, so one lane
-// per node storing its own entries touches 64 cache lines per instruction
-// with 48-byte pieces.  The labels of each neighbour column (oy, ox) are
-// therefore staged in LDS and stored as the rows' contiguous 3*czn-entry
-// runs (144 bytes in FP64), consecutive lanes on consecutive entries.  The
-// run length is what sets the store rate: longer runs need more labels staged
-// per node than LDS holds for 64 FP64 nodes.
+// Earlier layout, for the record: one wave per 64-node segment storing each
+// neighbour column's 144-byte row runs (FP64) took 1.05 ms at 96x96x64; the
+// contiguous span takes 0.96 ms.  What is left is occupancy: the FP64 label
+// evaluation needs 256 VGPRs and the span 68 KB of LDS, two blocks per CU.
 #include "wt_common.h"
+
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -297,14 +298,18 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
     }
 }
 
-// LDS tile: 3 (x) * 3 (y) columns of ASM_WAVE + 2 z-nodes.
-constexpr int TZ = ASM_WAVE + 2;
+// Values kernel geometry: a block of ASM_THREADS threads owns one chunk of CH
+// z-nodes of an (x, y) column.
+constexpr int CH = 16;
+constexpr int ASM_THREADS = 256;
+constexpr int TZC = CH + 2;  // tile z-slots of one chunk (one node either side)
 
-// Output staging: the three labels of one (oy, ox) neighbour column for every
-// node of the segment, [node][row][oz][component].
-constexpr int STG = 27;
+// Entries before 0-based z-node m of an (x, y) column's rows, in units of
+// 9 cxy: a node has 3 rows of 3 cxy czn entries, czn = 2 at both ends of the
+// column and 3 inside.
+__device__ inline int col_units(int m, int nz) { return 3 * m - (m >= 1) - (m >= nz); }
 
-// Block id -> (z segment, x, y), with consecutive ids of one XCD (blocks go
+// Block id -> (z chunk, x, y), with consecutive ids of one XCD (blocks go
 // round-robin to the 8 XCDs) mapped to consecutive columns, so the blocks
 // sharing an L2 share input columns and write adjacent output.
 __device__ inline void segment_of_block(int64_t nblk, int nzs, int nx, int& zs, int& ix, int& iy)
@@ -318,227 +323,221 @@ __device__ inline void segment_of_block(int64_t nblk, int nzs, int nx, int& zs, 
     iy = (int)(c / nx) + 1;
 }
 
-template <typename R>
-__global__ __launch_bounds__(ASM_WAVE) void assemble_values_kernel(
-    wt_grid_params p, const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ mu,
-    const R* __restrict__ rho, cpx<R>* __restrict__ out, int* __restrict__ err, int64_t nblk)
+// Region class (fdfd3d.f:50-115) and per-node coefficients of node (ix, iy,
+// iz) whose tile z-slot is tz: octant-cell moduli (setcoefs3d.f:108-174,
+// 224-225), mass weights (calcwts.f:28-71) and PML stretch factors
+// (pml3d.f:207-295).  Returns 0 interior, 1 PML, 2 Dirichlet.
+template <typename R, int TZN>
+__device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz, int tz,
+                                 const cpx<R> (*s_lam)[TZN], const cpx<R> (*s_mu)[TZN],
+                                 const R (*s_rho)[TZN], int* err, bool report, NodeCoef<R>& c,
+                                 bool& topfree)
 {
-    // The lambda/mu tiles are dead once the node coefficients are formed; the
-    // output staging reuses their space.
-    constexpr int IN_BYTES = 2 * 9 * TZ * (int)sizeof(cpx<R>);
-    constexpr int STG_BYTES = ASM_WAVE * STG * (int)sizeof(cpx<R>);
-    __shared__ __align__(16) char s_raw[IN_BYTES > STG_BYTES ? IN_BYTES : STG_BYTES];
-    __shared__ R s_rho[9][TZ];
-    cpx<R>(*s_lam)[TZ] = (cpx<R>(*)[TZ])s_raw;
-    cpx<R>(*s_mu)[TZ] = (cpx<R>(*)[TZ])(s_raw + 9 * TZ * sizeof(cpx<R>));
-    cpx<R>* stage = (cpx<R>*)s_raw;
-    const int nx = p.nx, ny = p.ny, nz = p.nz;
-    int zs, ix, iy;
-    segment_of_block(nblk, (nz + ASM_WAVE - 1) / ASM_WAVE, nx, zs, ix, iy);
-    const int z0 = zs * ASM_WAVE;  // 0-based first z of this segment
-    const int lane = threadIdx.x;
-
-    // Stage the clamped 3x3 column neighbourhood, z in [z0-1, z0+ASM_WAVE]
-    // (tile slot zz holds 1-based z = z0 + zz).  Each column is a contiguous
-    // run of the z-fastest model arrays; every load is issued before the
-    // first LDS write, so the wave waits for memory once.
-    {
-        auto colbase = [&](int col) {
-            int xg = ix + (col / 3) - 1, yg = iy + (col % 3) - 1;
-            xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
-            yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
-            return (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz;
-        };
-        auto zclamp = [&](int zz) {
-            int zg = z0 + zz;
-            return zg < 1 ? 1 : (zg > nz ? nz : zg);
-        };
-        cpx<R> la[9], mv[9];
-        R rh[9];
-        const int zg = zclamp(lane);
-#pragma unroll
-        for (int col = 0; col < 9; ++col) {
-            const int64_t g = colbase(col) + zg - 1;
-            la[col] = lam[g];
-            mv[col] = mu[g];
-            rh[col] = rho[g];
-        }
-#pragma unroll
-        for (int col = 0; col < 9; ++col) {
-            s_lam[col][lane] = la[col];
-            s_mu[col][lane] = mv[col];
-            s_rho[col][lane] = rh[col];
-        }
-        // the two slots past the wave width
-        if (lane < TZ - ASM_WAVE) {
-            const int zz = ASM_WAVE + lane, zh = zclamp(zz);
-#pragma unroll
-            for (int col = 0; col < 9; ++col) {
-                const int64_t g = colbase(col) + zh - 1;
-                la[col] = lam[g];
-                mv[col] = mu[g];
-                rh[col] = rho[g];
-            }
-#pragma unroll
-            for (int col = 0; col < 9; ++col) {
-                s_lam[col][zz] = la[col];
-                s_mu[col][zz] = mv[col];
-                s_rho[col][zz] = rh[col];
-            }
-        }
-    }
-    __syncthreads();
-
-    // Lanes past the end of the column evaluate the last node again and
-    // write nothing (every lane takes part in the staged output).
-    const bool live = z0 + lane + 1 <= nz;
-    const int iz = live ? z0 + lane + 1 : nz;
-    const int ip = p.ipml;
-    // Region class, fdfd3d.f:50-115.
+    const int nx = p.nx, ny = p.ny, nz = p.nz, ip = p.ipml;
     const bool inx = ix > ip && ix <= nx - ip, iny = iy > ip && iy <= ny - ip;
-    int mode = 1, izp = iz;  // 0 interior, 1 PML, 2 Dirichlet
+    int mode = 1, izp = iz;
     if (inx && iny && iz > ip && iz <= nz - ip) mode = 0;
     else if (iz == nz || ix == 1 || ix == nx || iy == 1 || iy == ny) mode = 2;
     else if (iz <= ip) {
         if (iz == 1 && !p.fst) mode = 2;
-        else if (iz == 1 && !p.usemin) { if (live) atomicOr(err, 2); }
+        else if (iz == 1 && !p.usemin) { if (report) atomicOr(err, 2); }
         else if (!p.fst) mode = 1;
         else if (inx && iny) mode = 0;
         else { mode = 1; izp = ip + 1; }
     }
-    const bool topfree = p.fst && iz == 1;
-    const int tz = iz - z0;  // tile z-slot of this node
+    topfree = p.fst && iz == 1;
 
-    NodeCoef<R> c;
-    cpx<R> cen_mc;
-    {
-        // setcoefs3d.f:108-174, tile column index = 3*(x offset) + (y offset)
-        auto cell = [&](cpx<R> (*a)[TZ], cpx<R>* o) {
-            cpx<R> km[3][4];
+    auto cell = [&](const cpx<R> (*a)[TZN], cpx<R>* o) {
+        cpx<R> km[3][4];
 #pragma unroll
-            for (int kz = 0; kz < 3; ++kz) {
-                int z = tz + kz - 1;
-                km[kz][0] = (((a[0][z] + a[3][z]) + a[1][z]) + a[4][z]) / R(4);
-                km[kz][1] = (((a[2][z] + a[5][z]) + a[1][z]) + a[4][z]) / R(4);
-                km[kz][2] = (((a[6][z] + a[3][z]) + a[7][z]) + a[4][z]) / R(4);
-                km[kz][3] = (((a[8][z] + a[5][z]) + a[7][z]) + a[4][z]) / R(4);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                o[1 + q] = (km[0][q] + km[1][q]) / R(2);
-                o[5 + q] = (km[2][q] + km[1][q]) / R(2);
-            }
-        };
-        cell(s_mu, c.u);
-        cell(s_lam, c.d);
-        if (topfree) {
-#pragma unroll
-            for (int q = 1; q <= 4; ++q) { c.u[q] = cpx<R>(0, 0); c.d[q] = cpx<R>(0, 0); }
+        for (int kz = 0; kz < 3; ++kz) {
+            int z = tz + kz - 1;
+            km[kz][0] = (((a[0][z] + a[3][z]) + a[1][z]) + a[4][z]) / R(4);
+            km[kz][1] = (((a[2][z] + a[5][z]) + a[1][z]) + a[4][z]) / R(4);
+            km[kz][2] = (((a[6][z] + a[3][z]) + a[7][z]) + a[4][z]) / R(4);
+            km[kz][3] = (((a[8][z] + a[5][z]) + a[7][z]) + a[4][z]) / R(4);
         }
-        // setcoefs3d.f:224-225, calcwts.f:28-71
-        cpx<R> lc = s_lam[4][tz], mcn = s_mu[4][tz];
-        R rc = s_rho[4][tz];
-        R vp = csqrt_re((lc + R(2) * mcn) / rc);
-        R vs = csqrt_re(mcn / rc);
-        double x = (double)(vp / vs);
-        double w1 = 0.2909366e+00 + x * (-0.1792402e-01 + x * (0.7329184e-02 + x * -0.1055034e-02));
-        double w2 = 0.0, w3 = 0.0;
-        if (w1 > 1.0) w1 = 1.0;
-        else w2 = 0.6367377e+00 + x * (-0.7982022e-02 + x * (0.3385164e-02 + x * -0.5065399e-03));
-        if ((w1 + w2) > 1.0) w2 = 0.0;
-        else w3 = 0.7244875e-01 + x * (0.2569959e-01 + x * (-0.1060011e-01 + x * 0.1540534e-02));
-        if ((w1 + w2 + w3) > 1.0) w3 = 1.0 - (w1 + w2);
-        double w4 = 1.0 - (w1 + w2 + w3);
-        const R wm1 = (R)w1, wm2 = (R)w2, wm3 = (R)w3, wm4 = (R)w4;
-
-        const cpx<R> om(p.omega_re, p.omega_im);
-        const cpx<R> eyeom(om.im, -om.re);
-        const cpx<R> osq = om * om;
-        c.wm2o6 = (osq.re * wm2) / R(6);
-        c.wm3o12 = (osq.re * wm3) / R(12);
-        c.wm4o8 = (osq.re * wm4) / R(8);
-        cpx<R> mcen = wm1 * osq;
-        c.mc = cpx<R>(mcen.re * rc, mcen.im * rc);
-        (void)cen_mc;
-
-        const R dx = (R)p.dx, dx2 = dx + dx, dxx = dx * dx, dxx4 = (dxx + dxx) + (dxx + dxx);
-        cpx<R> r1x(1, 0), r1y(1, 0), r1z(1, 0), r2x(0, 0), r2y(0, 0), r2z(0, 0);
-        if (mode == 1) {
-            const R pmld = (R)p.pmld, pmlf = (R)p.pmlf;
-            bool ok = pml_axis(ix, nx, ip, pmld, dx, pmlf, vp, eyeom, r1x, r2x);
-            ok = pml_axis(iy, ny, ip, pmld, dx, pmlf, vp, eyeom, r1y, r2y) && ok;
-            ok = pml_axis(izp, nz, ip, pmld, dx, pmlf, vp, eyeom, r1z, r2z) && ok;
-            if (!ok && live) atomicOr(err, 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            o[1 + q] = (km[0][q] + km[1][q]) / R(2);
+            o[5 + q] = (km[2][q] + km[1][q]) / R(2);
         }
-        c.rx1 = (r2x / dx2) / R(4);
-        c.ry1 = (r2y / dx2) / R(4);
-        c.rz1 = (r2z / dx2) / R(4);
-        c.rx2 = (((r1x * r1x) / dxx) / R(4)) / R(2);
-        c.ry2 = (((r1y * r1y) / dxx) / R(4)) / R(2);
-        c.rz2 = (((r1z * r1z) / dxx) / R(4)) / R(2);
-        c.rxy = ((r1x * r1y) / dxx4) / R(4);
-        c.rxz = ((r1x * r1z) / dxx4) / R(4);
-        c.ryz = ((r1y * r1z) / dxx4) / R(4);
+    };
+    cell(s_mu, c.u);
+    cell(s_lam, c.d);
+    if (topfree) {
+#pragma unroll
+        for (int q = 1; q <= 4; ++q) { c.u[q] = cpx<R>(0, 0); c.d[q] = cpx<R>(0, 0); }
     }
+    cpx<R> lc = s_lam[4][tz], mcn = s_mu[4][tz];
+    R rc = s_rho[4][tz];
+    R vp = csqrt_re((lc + R(2) * mcn) / rc);
+    R vs = csqrt_re(mcn / rc);
+    double x = (double)(vp / vs);
+    double w1 = 0.2909366e+00 + x * (-0.1792402e-01 + x * (0.7329184e-02 + x * -0.1055034e-02));
+    double w2 = 0.0, w3 = 0.0;
+    if (w1 > 1.0) w1 = 1.0;
+    else w2 = 0.6367377e+00 + x * (-0.7982022e-02 + x * (0.3385164e-02 + x * -0.5065399e-03));
+    if ((w1 + w2) > 1.0) w2 = 0.0;
+    else w3 = 0.7244875e-01 + x * (0.2569959e-01 + x * (-0.1060011e-01 + x * 0.1540534e-02));
+    if ((w1 + w2 + w3) > 1.0) w3 = 1.0 - (w1 + w2);
+    double w4 = 1.0 - (w1 + w2 + w3);
+    const R wm1 = (R)w1, wm2 = (R)w2, wm3 = (R)w3, wm4 = (R)w4;
 
-    // Row layout: a node has 3 rows of 3*cxn*cyn*czn entries, its neighbours
-    // ascending (y, then x, then z).  The nodes of a column segment follow
-    // each other in the CSR arrays, and in each row the neighbour column
-    // (oy, ox) owns one run of 3*czn entries.  Each column's three labels are
-    // staged in LDS and written as those runs: a store instruction covers two
-    // nodes (54 lanes, lane = node parity h, row r, entry e), so consecutive
-    // lanes store consecutive entries.
-    //
-    // With czn(iz) = 3 - [iz = 1] - [iz = nz], the first entry of node n of
-    // the segment (iz = z0 + 1 + n) is 9 cxy (3 n - [z0 = 0, n >= 1]) and a
-    // row holds 3 cxy czn entries, so entry (r, e) of the run of column rank
-    // j = ry cxn + rx lies at 9 cxy (3 n - [z0 = 0, n >= 1]) + czn K + e with
-    // K = 3 cxy r + 3 j.
+    const cpx<R> om(p.omega_re, p.omega_im);
+    const cpx<R> eyeom(om.im, -om.re);
+    const cpx<R> osq = om * om;
+    c.wm2o6 = (osq.re * wm2) / R(6);
+    c.wm3o12 = (osq.re * wm3) / R(12);
+    c.wm4o8 = (osq.re * wm4) / R(8);
+    cpx<R> mcen = wm1 * osq;
+    c.mc = cpx<R>(mcen.re * rc, mcen.im * rc);
+
+    const R dx = (R)p.dx, dx2 = dx + dx, dxx = dx * dx, dxx4 = (dxx + dxx) + (dxx + dxx);
+    cpx<R> r1x(1, 0), r1y(1, 0), r1z(1, 0), r2x(0, 0), r2y(0, 0), r2z(0, 0);
+    if (mode == 1) {
+        const R pmld = (R)p.pmld, pmlf = (R)p.pmlf;
+        bool ok = pml_axis(ix, nx, ip, pmld, dx, pmlf, vp, eyeom, r1x, r2x);
+        ok = pml_axis(iy, ny, ip, pmld, dx, pmlf, vp, eyeom, r1y, r2y) && ok;
+        ok = pml_axis(izp, nz, ip, pmld, dx, pmlf, vp, eyeom, r1z, r2z) && ok;
+        if (!ok && report) atomicOr(err, 1);
+    }
+    c.rx1 = (r2x / dx2) / R(4);
+    c.ry1 = (r2y / dx2) / R(4);
+    c.rz1 = (r2z / dx2) / R(4);
+    c.rx2 = (((r1x * r1x) / dxx) / R(4)) / R(2);
+    c.ry2 = (((r1y * r1y) / dxx) / R(4)) / R(2);
+    c.rz2 = (((r1z * r1z) / dxx) / R(4)) / R(2);
+    c.rxy = ((r1x * r1y) / dxx4) / R(4);
+    c.rxz = ((r1x * r1z) / dxx4) / R(4);
+    c.ryz = ((r1y * r1z) / dxx4) / R(4);
+    return mode;
+}
+
+// Marks every coefficient as possibly changed here (an empty asm statement).
+template <typename R>
+__device__ __forceinline__ void launder(NodeCoef<R>& c)
+{
+    R* f = reinterpret_cast<R*>(&c);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(NodeCoef<R>) / sizeof(R)); ++i) asm volatile("" : "+v"(f[i]));
+}
+
+// Output rows of a chunk of CH z-nodes are one contiguous span of the CSR
+// values (node after node, 3 rows each, neighbours ascending in y, x, z), so
+// the block builds the span in LDS and writes it front to back with
+// consecutive threads on consecutive entries: every store instruction covers
+// whole cache lines and the block's output is one sequential sweep.
+//
+// To build the span, thread t evaluates node n = t mod CH of the chunk for
+// the neighbour labels k = g, g + 16 (k < 27) of its label group
+// g = 4 (t / 64) + (t mod 64) / 16: each wave evaluates four labels at once
+// (the label is uniform over 16 lanes).  Each thread forms its node's
+// coefficients itself.  The label evaluation is a small fraction of the
+// time; the stores set it.
+template <typename R>
+__global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
+    wt_grid_params p, const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ mu,
+    const R* __restrict__ rho, cpx<R>* __restrict__ out, int* __restrict__ err, int64_t nblk)
+{
+    // element type moved as one vector: 16 bytes (FP64), 8 bytes (FP32)
+    using V = typename std::conditional<sizeof(R) == 8, double2, float2>::type;
+    constexpr int STG_N = CH * 3 * 81;
+    __shared__ __align__(16) V stage[STG_N];
+    __shared__ __align__(16) cpx<R> s_lam[9][TZC];
+    __shared__ __align__(16) cpx<R> s_mu[9][TZC];
+    __shared__ R s_rho[9][TZC];
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    int zs, ix, iy;
+    segment_of_block(nblk, (nz + CH - 1) / CH, nx, zs, ix, iy);
+    const int t = threadIdx.x;
+    const int n = t & (CH - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave of the block
+    const int lg = (t & 63) >> 4;                             // label group in the wave
+    const int g = 4 * wv + lg;
     const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), cxy = cxn * cyn;
-    const int zlast = nz - z0 < ASM_WAVE ? nz - z0 : ASM_WAVE;  // live nodes
-    cpx<R>* const seg = out + node_offset(z0 + 1, ix, iy, nx, ny, nz);
-    const int fh = lane / STG, frem = lane - fh * STG, fr = frem / 9, fe = frem - fr * 9;
-    const bool dir = mode == 2;
-    __syncthreads();  // lambda/mu tiles dead from here on
-
-#pragma unroll
-    for (int col = 0; col < 9; ++col) {
-        const int oy = col / 3 - 1, ox = col % 3 - 1;
-        if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
-#pragma unroll
-        for (int oz = -1; oz <= 1; ++oz) {
-            const int k = 3 * col + oz + 1;
-            // density the mass lumping uses at this neighbour (clamped gather,
-            // top layer zeroed at the free surface, setcoefs3d.f:187-191)
-            R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
-            if (topfree && oz == -1) ro = R(0);
-            cpx<R> v[9];
-            label_block<R>(k, c, ro, v);
-            if (dir) {
-#pragma unroll
-                for (int e = 0; e < 9; ++e)
-                    v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
-            }
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int e = 0; e < 3; ++e) stage[lane * STG + r * 9 + (oz + 1) * 3 + e] = v[3 * r + e];
+    V* const col = (V*)(out + node_offset(1, ix, iy, nx, ny, nz));
+    {
+        const int m0 = zs * CH, m1 = m0 + CH < nz ? m0 + CH : nz;  // chunk = 0-based z-nodes [m0, m1)
+        // Tile slot zz holds 1-based z = m0 + zz, clamped to the grid, of the
+        // clamped 3x3 column neighbourhood (column index 3 (x offset) + y offset).
+        if (t < 9 * TZC) {
+            const int cc = t / TZC, zz = t - cc * TZC;
+            int xg = ix + cc / 3 - 1, yg = iy + cc % 3 - 1, zg = m0 + zz;
+            xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
+            yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
+            zg = zg < 1 ? 1 : (zg > nz ? nz : zg);
+            const int64_t gi = (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz + zg - 1;
+            s_lam[cc][zz] = lam[gi];
+            s_mu[cc][zz] = mu[gi];
+            s_rho[cc][zz] = rho[gi];
         }
         __syncthreads();
-        const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
-        const int K = 3 * cxy * fr + 3 * (ry * cxn + rx);
-        if (fh < 2) {
-#pragma unroll 4
-            for (int n = fh; n < zlast; n += 2) {
-                const int izn = z0 + 1 + n;
-                const int czn = 1 + (izn > 1) + (izn < nz);
-                if (fe < 3 * czn) {
-                    const int dst = 9 * cxy * (3 * n - (z0 == 0 && n > 0 ? 1 : 0)) + czn * K + fe;
-                    seg[dst] = stage[n * STG + frem + (izn == 1 ? 3 : 0)];
+        const int m = m0 + n;
+        const int iz = m + 1, tz = n + 1;
+        NodeCoef<R> c;
+        bool topfree = false;
+        int mode = 0;
+        if (m < m1) mode = node_setup<R, TZC>(p, ix, iy, iz, tz, s_lam, s_mu, s_rho, err, g == 0, c, topfree);
+        const int czn = 1 + (iz > 1) + (iz < nz);
+        const int rl = 3 * cxy * czn;
+        const int base = 9 * cxy * (col_units(m, nz) - col_units(m0, nz));
+        // The wave's four label groups take turns, so the label is uniform
+        // (a scalar) inside label_block and only its own formula runs.
+#pragma unroll 1
+        for (int kk = 4 * wv; kk < 27; kk += 16) {
+#pragma unroll 1
+            for (int j = 0; j < 4; ++j) {
+                const int k = kk + j;
+                if (k >= 27) break;
+                const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
+                if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
+                if (lg != j || m >= m1 || iz + oz < 1 || iz + oz > nz) continue;
+                // density the mass lumping uses at this neighbour (clamped
+                // gather, top layer zeroed at the free surface,
+                // setcoefs3d.f:187-191)
+                R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
+                if (topfree && oz == -1) ro = R(0);
+                // Keep the compiler from hoisting every label's subexpressions
+                // out of the loop (all live at once, they spill): the node
+                // coefficients count as redefined on each pass.
+                launder(c);
+                cpx<R> v[9];
+                label_block<R>(k, c, ro, v);
+                if (mode == 2) {
+#pragma unroll
+                    for (int e = 0; e < 9; ++e)
+                        v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
                 }
+                const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
+                const int rz = oz > -1 ? (iz > 1) + (oz > 0) : 0;
+                const int pos = base + ((ry * cxn + rx) * czn + rz) * 3;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        V w;
+                        w.x = v[3 * r + e].re;
+                        w.y = v[3 * r + e].im;
+                        stage[pos + r * rl + e] = w;
+                    }
             }
         }
         __syncthreads();
+        // Span out: every LDS read is issued before the first store, so the
+        // wave waits on LDS once per chunk.
+        const int len = 9 * cxy * (col_units(m1, nz) - col_units(m0, nz));
+        V* const dst = col + 9 * cxy * col_units(m0, nz);
+        constexpr int NS = (STG_N + ASM_THREADS - 1) / ASM_THREADS;
+        V w[NS];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = t + u * ASM_THREADS;
+            w[u] = stage[i < STG_N ? i : STG_N - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u)
+            if (t + u * ASM_THREADS < len) dst[t + u * ASM_THREADS] = w[u];
     }
 }
 
@@ -604,9 +603,9 @@ int wt_launch_values(const wt_grid_params& p, const void* d_lam, const void* d_m
 {
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return WT_ERR_ARG;
     if (!p.usemin && p.fst) return WT_ERR_UNSUPPORTED;
-    const int64_t nblk = (int64_t)((p.nz + ASM_WAVE - 1) / ASM_WAVE) * p.nx * p.ny;
+    const int64_t nblk = (int64_t)((p.nz + CH - 1) / CH) * p.nx * p.ny;
     if (nblk > 0x7fffffff) return WT_ERR_ARG;
-    hipLaunchKernelGGL(assemble_values_kernel<R>, dim3((unsigned)nblk), dim3(ASM_WAVE), 0, stream, p,
+    hipLaunchKernelGGL(assemble_values_kernel<R>, dim3((unsigned)nblk), dim3(ASM_THREADS), 0, stream, p,
                        (const cpx<R>*)d_lam, (const cpx<R>*)d_mu, (const R*)d_rho, (cpx<R>*)d_vals, d_err,
                        nblk);
     WT_HIP(hipGetLastError());
