@@ -7,22 +7,27 @@
 // kernel reproduces SREL3D (srgraphs.f:638-734) followed by SPARSE_ADJ2CRS
 // (sparse.f:13-49) from closed forms.
 //
-// Design: a block of 256 threads owns a chunk of 16 z-nodes of one (x, y)
-// column.  The clamped 3x3 column neighbourhood of lambda, mu, rho for the
-// chunk is staged in LDS.  A node's output is 3 rows of up to 81 entries
-// (neighbours ascending in y, x, z), and the rows of the chunk's nodes follow
-// each other in the CSR arrays, so the chunk's output is one contiguous span.
-// The block builds that span in LDS -- 16 lanes per neighbour label, each
-// lane one node, the label uniform over the wave's turn so only its formula
-// runs -- and writes it front to back, consecutive threads on consecutive
-// entries.  The interior stencil is the stretched form with r1 = 1, r2 = 0
-// (identical values; the reference notes the identity at fdfd3d.f:56-57).
-// CSR offsets follow from closed forms, so no atomics or scans.
+// Design: a block of 256 threads owns a segment of 64 z-nodes of one (x, y)
+// column.  The clamped 3x3 column neighbourhood of lambda, mu, rho is staged
+// in LDS, the octant-cell moduli are formed once per z-interface into an LDS
+// table, and lane l of every wave forms node l's mass weights and PML
+// stretch factors in registers.  A node's output is 3 rows of up to 81
+// entries (neighbours ascending in y, x, z), and the rows of consecutive
+// nodes follow each other in the CSR arrays, so a chunk of 16 nodes is one
+// contiguous span.  The block builds each chunk's span in LDS -- wave w
+// takes the neighbour labels k = w, w + 4, ..., the label uniform over the
+// wave so only its formula runs -- and writes it front to back, consecutive
+// threads on consecutive entries, while the next chunk is evaluated.  The
+// interior stencil is the stretched form with r1 = 1, r2 = 0 (identical
+// values; the reference notes the identity at fdfd3d.f:56-57).  CSR offsets
+// follow from closed forms, so no atomics or scans.
 //
-// Earlier layout, for the record: one wave per 64-node segment storing each
-// neighbour column's 144-byte row runs (FP64) took 1.05 ms at 96x96x64; the
-// contiguous span takes 0.96 ms.  What is left is occupancy: the FP64 label
-// evaluation needs 256 VGPRs and the span 68 KB of LDS, two blocks per CU.
+// Measured at 96x96x64 in FP64: one wave per segment storing each neighbour
+// column's 144-byte row runs took 1.05 ms; the contiguous spans with the
+// per-interface cell table take 0.69 ms; the span stores alone (no values
+// computed) take 0.37 ms.  The rest is the per-node divisions and square
+// roots and the label evaluation at 16 of 64 lanes, with occupancy capped at
+// two blocks per CU by the FP64 registers (254) and the 76 KB of LDS.
 #include "wt_common.h"
 
 #include <type_traits>
@@ -53,11 +58,17 @@ __host__ __device__ inline int64_t node_offset(int iz, int ix, int iy, int nx, i
 
 template <typename R>
 struct NodeCoef {
-    cpx<R> u[9], d[9];                   // octant-cell mu and lambda, index 1..8
     cpx<R> rx2, ry2, rz2, rxy, rxz, ryz;  // r1-derived scalings
     cpx<R> rx1, ry1, rz1;                // r2 / (2 dx) / 4
     R wm2o6, wm3o12, wm4o8;
     cpx<R> mc;                           // centre mass term (complex)
+    // Octant-cell mu and lambda, index 1..8, held in LDS: cells 1..4 lie
+    // above the node (cup: mu 1..4, then lambda 1..4), cells 5..8 below it
+    // (cdn: mu 5..8, then lambda 5..8).
+    const cpx<R>* cup;
+    const cpx<R>* cdn;
+    __device__ cpx<R> U(int i) const { return i <= 4 ? cup[i - 1] : cdn[i - 5]; }
+    __device__ cpx<R> D(int i) const { return i <= 4 ? cup[3 + i] : cdn[i - 1]; }
 };
 
 template <typename R>
@@ -97,15 +108,14 @@ __device__ inline bool pml_axis(int i, int n, int ipml, R pmld, R dx, R pmlf, R 
 template <typename R>
 __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, cpx<R> v[9])
 {
-    const cpx<R>* u = c.u;
-    auto lmu = [&](int i) { return c.d[i] - u[i]; };
-    auto lpu = [&](int i) { return c.d[i] + u[i]; };
-    auto lp2u = [&](int i) { return (c.d[i] + u[i]) + u[i]; };
+    auto lmu = [&](int i) { return c.D(i) - c.U(i); };
+    auto lpu = [&](int i) { return c.D(i) + c.U(i); };
+    auto lp2u = [&](int i) { return (c.D(i) + c.U(i)) + c.U(i); };
     const cpx<R> rx2 = c.rx2, ry2 = c.ry2, rz2 = c.rz2, rxy = c.rxy, rxz = c.rxz, ryz = c.ryz;
     const cpx<R> rx1 = c.rx1, ry1 = c.ry1, rz1 = c.rz1;
     // Diagonal of the 8 corner blocks: q1*L + qs*U +- ra*L +- rb*U + mass.
     auto corner = [&](cpx<R> q1, cpx<R> qs, int sa, cpx<R> ra, int sb, cpx<R> rb, int cell, cpx<R> mf) {
-        cpx<R> L = lp2u(cell), U = u[cell];
+        cpx<R> L = lp2u(cell), U = c.U(cell);
         cpx<R> s = q1 * L + qs * U;
         cpx<R> ta = ra * L, tb = rb * U;
         s = s + (sa > 0 ? ta : -ta);
@@ -131,7 +141,7 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
             corner(rz2, rxpy, -1, rz1, -1, r2xpy, 1, m4), t0, t0, t1, t1, t2, t2);
         break;
     case 1:  // a2m
-        t0 = -(rz2 * (u[1] + u[5])) + m3; t1 = -(rz2 * (lp2u(1) + lp2u(5))) + m3;
+        t0 = -(rz2 * (c.U(1) + c.U(5))) + m3; t1 = -(rz2 * (lp2u(1) + lp2u(5))) + m3;
         t2 = rxy * (lpu(1) + lpu(5)); t3 = -(rxz * (lmu(1) - lmu(5))); t4 = -(ryz * (lmu(1) - lmu(5)));
         put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
         break;
@@ -141,19 +151,19 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
             corner(rz2, rxpy, +1, rz1, -1, r2xmy, 5, m4), t0, t0, t1, t1, t2, t2);
         break;
     case 3:  // a4m
-        t0 = -(rx2 * (lp2u(1) + lp2u(3))) + m3; t1 = -(rx2 * (u[1] + u[3])) + m3;
+        t0 = -(rx2 * (lp2u(1) + lp2u(3))) + m3; t1 = -(rx2 * (c.U(1) + c.U(3))) + m3;
         t2 = rxy * (lmu(1) - lmu(3)); t3 = rxz * (lmu(1) - lmu(3)); t4 = ryz * (lpu(1) + lpu(3));
         put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
         break;
     case 4:  // a5m
-        t0 = ry2 * (((u[1] + u[7]) + u[5]) + u[3]) + m2;
+        t0 = ry2 * (((c.U(1) + c.U(7)) + c.U(5)) + c.U(3)) + m2;
         t2 = rxy * (((lmu(1) - lmu(7)) + lmu(5)) - lmu(3));
         t3 = -(rxz * (((lpu(1) + lpu(7)) - lpu(5)) - lpu(3)));
         t4 = -(ryz * (((lmu(1) - lmu(7)) - lmu(5)) + lmu(3)));
         put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
         break;
     case 5:  // a6m
-        t0 = -(rx2 * (lp2u(7) + lp2u(5))) + m3; t1 = -(rx2 * (u[7] + u[5])) + m3;
+        t0 = -(rx2 * (lp2u(7) + lp2u(5))) + m3; t1 = -(rx2 * (c.U(7) + c.U(5))) + m3;
         t2 = rxy * (lmu(5) - lmu(7)); t3 = rxz * (lmu(7) - lmu(5)); t4 = -(ryz * (lpu(7) + lpu(5)));
         put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
         break;
@@ -163,7 +173,7 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
             corner(rz2, rxpy, -1, rz1, +1, r2xpy, 3, m4), t0, t0, t1, t1, t2, t2);
         break;
     case 7:  // a8m
-        t0 = -(rz2 * (u[7] + u[3])) + m3; t1 = -(rz2 * (lp2u(7) + lp2u(3))) + m3;
+        t0 = -(rz2 * (c.U(7) + c.U(3))) + m3; t1 = -(rz2 * (lp2u(7) + lp2u(3))) + m3;
         t2 = -(rxy * (lpu(7) + lpu(3))); t3 = -(rxz * (lmu(7) - lmu(3))); t4 = -(ryz * (lmu(3) - lmu(7)));
         put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
         break;
@@ -174,25 +184,25 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
         break;
     // ---------------- oy = 0 ----------------
     case 9:  // a1n
-        t0 = -(ry2 * (u[1] + u[2])) + m3;
+        t0 = -(ry2 * (c.U(1) + c.U(2))) + m3;
         t2 = rxy * (lmu(2) - lmu(1)); t3 = rxz * (lpu(1) + lpu(2)); t4 = ryz * (lmu(1) - lmu(2));
         put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
         break;
     case 10:  // a2n
         t0 = rx2 * (((lp2u(1) + lp2u(6)) + lp2u(2)) + lp2u(5)) + m2;
-        t1 = rx2 * (((u[1] + u[6]) + u[2]) + u[5]) + m2;
+        t1 = rx2 * (((c.U(1) + c.U(6)) + c.U(2)) + c.U(5)) + m2;
         t2 = rxy * (((lmu(2) - lmu(5)) + lmu(6)) - lmu(1));
         t3 = -(rxz * (((lmu(2) - lmu(5)) + lmu(1)) - lmu(6)));
         t4 = ryz * (((lpu(2) + lpu(5)) - lpu(1)) - lpu(6));
         put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
         break;
     case 11:  // a3n
-        t0 = -(ry2 * (u[5] + u[6])) + m3;
+        t0 = -(ry2 * (c.U(5) + c.U(6))) + m3;
         t2 = -(rxy * (lmu(5) - lmu(6))); t3 = -(rxz * (lpu(5) + lpu(6))); t4 = -(ryz * (lmu(5) - lmu(6)));
         put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
         break;
     case 12:  // a4n
-        t0 = rz2 * (((u[1] + u[4]) + u[2]) + u[3]) + m2;
+        t0 = rz2 * (((c.U(1) + c.U(4)) + c.U(2)) + c.U(3)) + m2;
         t1 = rz2 * (((lp2u(1) + lp2u(4)) + lp2u(2)) + lp2u(3)) + m2;
         t2 = -(rxy * (((lpu(1) + lpu(4)) - lpu(2)) - lpu(3)));
         t3 = rxz * (((lmu(1) - lmu(4)) + lmu(2)) - lmu(3));
@@ -200,20 +210,20 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
         put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
         break;
     case 13: {  // a5n, the node itself
-        cpx<R> ls = lp2u(1), us = u[1];
+        cpx<R> ls = lp2u(1), us = c.U(1);
 #pragma unroll
-        for (int i = 2; i <= 8; ++i) { ls = ls + lp2u(i); us = us + u[i]; }
+        for (int i = 2; i <= 8; ++i) { ls = ls + lp2u(i); us = us + c.U(i); }
         cpx<R> ex = ((((((lp2u(1) - lp2u(8)) + lp2u(2)) - lp2u(7)) + lp2u(5)) - lp2u(4)) + lp2u(6)) - lp2u(3);
-        cpx<R> fa = ((u[1] - u[8]) + u[3]) - u[6];
-        cpx<R> fb = ((u[7] - u[2]) + u[5]) - u[4];
+        cpx<R> fa = ((c.U(1) - c.U(8)) + c.U(3)) - c.U(6);
+        cpx<R> fb = ((c.U(7) - c.U(2)) + c.U(5)) - c.U(4);
         t0 = ((((-(rx2 * ls) - rypz * us) + rx1 * ex) + r2ypz * fa) + r2ymz * fb) + c.mc;
         cpx<R> ey = ((((((lp2u(1) - lp2u(2)) + lp2u(5)) - lp2u(6)) + lp2u(7)) - lp2u(8)) + lp2u(3)) - lp2u(4);
-        fa = ((u[1] - u[8]) + u[2]) - u[7];
-        fb = ((u[6] - u[3]) + u[5]) - u[4];
+        fa = ((c.U(1) - c.U(8)) + c.U(2)) - c.U(7);
+        fb = ((c.U(6) - c.U(3)) + c.U(5)) - c.U(4);
         t1 = ((((-(ry2 * ls) - rxpz * us) + ry1 * ey) + r2xpz * fa) + r2xmz * fb) + c.mc;
         cpx<R> ez = ((((((lp2u(1) - lp2u(8)) + lp2u(2)) - lp2u(7)) + lp2u(4)) - lp2u(5)) + lp2u(3)) - lp2u(6);
-        fa = ((u[1] - u[8]) + u[5]) - u[4];
-        fb = ((u[2] - u[7]) + u[6]) - u[3];
+        fa = ((c.U(1) - c.U(8)) + c.U(5)) - c.U(4);
+        fb = ((c.U(2) - c.U(7)) + c.U(6)) - c.U(3);
         t2 = ((((-(rz2 * ls) - rxpy * us) + rz1 * ez) + r2xpy * fa) + r2xmy * fb) + c.mc;
         t3 = rxy * (((((((lpu(2) - lpu(1)) + lpu(7)) - lpu(8)) + lpu(3)) - lpu(4)) + lpu(6)) - lpu(5));
         t4 = rxz * (((((((lpu(4) - lpu(1)) + lpu(5)) - lpu(8)) + lpu(3)) - lpu(2)) + lpu(6)) - lpu(7));
@@ -222,7 +232,7 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
         break;
     }
     case 14:  // a6n
-        t0 = rz2 * (((u[8] + u[5]) + u[7]) + u[6]) + m2;
+        t0 = rz2 * (((c.U(8) + c.U(5)) + c.U(7)) + c.U(6)) + m2;
         t1 = rz2 * (((lp2u(8) + lp2u(5)) + lp2u(7)) + lp2u(6)) + m2;
         t2 = -(rxy * (((lpu(8) + lpu(5)) - lpu(7)) - lpu(6)));
         t3 = rxz * (((lmu(8) - lmu(5)) + lmu(7)) - lmu(6));
@@ -230,20 +240,20 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
         put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
         break;
     case 15:  // a7n
-        t0 = -(ry2 * (u[4] + u[3])) + m3;
+        t0 = -(ry2 * (c.U(4) + c.U(3))) + m3;
         t2 = -(rxy * (lmu(4) - lmu(3))); t3 = -(rxz * (lpu(4) + lpu(3))); t4 = -(ryz * (lmu(4) - lmu(3)));
         put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
         break;
     case 16:  // a8n
         t0 = rx2 * (((lp2u(8) + lp2u(3)) + lp2u(7)) + lp2u(4)) + m2;
-        t1 = rx2 * (((u[8] + u[3]) + u[7]) + u[4]) + m2;
+        t1 = rx2 * (((c.U(8) + c.U(3)) + c.U(7)) + c.U(4)) + m2;
         t2 = rxy * (((lmu(7) - lmu(4)) + lmu(3)) - lmu(8));
         t3 = -(rxz * (((lmu(7) - lmu(4)) + lmu(8)) - lmu(3)));
         t4 = ryz * (((lpu(7) + lpu(4)) - lpu(8)) - lpu(3));
         put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
         break;
     case 17:  // a9n
-        t0 = -(ry2 * (u[8] + u[7])) + m3;
+        t0 = -(ry2 * (c.U(8) + c.U(7))) + m3;
         t2 = rxy * (lmu(7) - lmu(8)); t3 = rxz * (lpu(8) + lpu(7)); t4 = ryz * (lmu(8) - lmu(7));
         put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
         break;
@@ -254,7 +264,7 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
             corner(rz2, rxpy, -1, rz1, -1, r2xmy, 2, m4), t0, t0, t1, t1, t2, t2);
         break;
     case 19:  // a2p
-        t0 = -(rz2 * (u[2] + u[6])) + m3; t1 = -(rz2 * (lp2u(2) + lp2u(6))) + m3;
+        t0 = -(rz2 * (c.U(2) + c.U(6))) + m3; t1 = -(rz2 * (lp2u(2) + lp2u(6))) + m3;
         t2 = -(rxy * (lpu(2) + lpu(6))); t3 = -(rxz * (lmu(2) - lmu(6))); t4 = -(ryz * (lmu(6) - lmu(2)));
         put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
         break;
@@ -264,19 +274,19 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
             corner(rz2, rxpy, +1, rz1, -1, r2xpy, 6, m4), t0, t0, t1, t1, t2, t2);
         break;
     case 21:  // a4p
-        t0 = -(rx2 * (lp2u(2) + lp2u(4))) + m3; t1 = -(rx2 * (u[2] + u[4])) + m3;
+        t0 = -(rx2 * (lp2u(2) + lp2u(4))) + m3; t1 = -(rx2 * (c.U(2) + c.U(4))) + m3;
         t2 = rxy * (lmu(4) - lmu(2)); t3 = rxz * (lmu(2) - lmu(4)); t4 = -(ryz * (lpu(2) + lpu(4)));
         put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
         break;
     case 22:  // a5p
-        t0 = ry2 * (((u[8] + u[2]) + u[4]) + u[6]) + m2;
+        t0 = ry2 * (((c.U(8) + c.U(2)) + c.U(4)) + c.U(6)) + m2;
         t2 = rxy * (((lmu(8) - lmu(2)) + lmu(4)) - lmu(6));
         t3 = -(rxz * (((lpu(8) + lpu(2)) - lpu(4)) - lpu(6)));
         t4 = -(ryz * (((lmu(8) - lmu(2)) - lmu(4)) + lmu(6)));
         put(t0, t0, t0, t2, -t2, t3, t3, t4, -t4);
         break;
     case 23:  // a6p
-        t0 = -(rx2 * (lp2u(8) + lp2u(6))) + m3; t1 = -(rx2 * (u[8] + u[6])) + m3;
+        t0 = -(rx2 * (lp2u(8) + lp2u(6))) + m3; t1 = -(rx2 * (c.U(8) + c.U(6))) + m3;
         t2 = rxy * (lmu(8) - lmu(6)); t3 = rxz * (lmu(8) - lmu(6)); t4 = ryz * (lpu(8) + lpu(6));
         put(t0, t0, t1, t2, -t2, t3, -t3, t4, t4);
         break;
@@ -286,7 +296,7 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
             corner(rz2, rxpy, -1, rz1, +1, r2xmy, 4, m4), t0, t0, t1, t1, t2, t2);
         break;
     case 25:  // a8p
-        t0 = -(rz2 * (u[8] + u[4])) + m3; t1 = -(rz2 * (lp2u(8) + lp2u(4))) + m3;
+        t0 = -(rz2 * (c.U(8) + c.U(4))) + m3; t1 = -(rz2 * (lp2u(8) + lp2u(4))) + m3;
         t2 = rxy * (lpu(8) + lpu(4)); t3 = -(rxz * (lmu(8) - lmu(4))); t4 = -(ryz * (lmu(8) - lmu(4)));
         put(t0, t0, t1, t2, t2, t3, -t3, t4, -t4);
         break;
@@ -302,7 +312,7 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
 // z-nodes of an (x, y) column.
 constexpr int CH = 16;
 constexpr int ASM_THREADS = 256;
-constexpr int TZC = CH + 2;  // tile z-slots of one chunk (one node either side)
+constexpr int TZ = ASM_WAVE + 2;  // tile z-slots of a segment (one node either side)
 
 // Entries before 0-based z-node m of an (x, y) column's rows, in units of
 // 9 cxy: a node has 3 rows of 3 cxy czn entries, czn = 2 at both ends of the
@@ -324,14 +334,15 @@ __device__ inline void segment_of_block(int64_t nblk, int nzs, int nx, int& zs, 
 }
 
 // Region class (fdfd3d.f:50-115) and per-node coefficients of node (ix, iy,
-// iz) whose tile z-slot is tz: octant-cell moduli (setcoefs3d.f:108-174,
-// 224-225), mass weights (calcwts.f:28-71) and PML stretch factors
-// (pml3d.f:207-295).  Returns 0 interior, 1 PML, 2 Dirichlet.
-template <typename R, int TZN>
+// iz) whose tile z-slot is tz: mass weights (calcwts.f:28-71, with the
+// moduli of setcoefs3d.f:224-225) and PML stretch factors (pml3d.f:207-295);
+// the octant cells come from the segment's cell table s_cell.  Returns 0
+// interior, 1 PML, 2 Dirichlet.
+template <typename R, int TZN, int CW>
 __device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz, int tz,
                                  const cpx<R> (*s_lam)[TZN], const cpx<R> (*s_mu)[TZN],
-                                 const R (*s_rho)[TZN], int* err, bool report, NodeCoef<R>& c,
-                                 bool& topfree)
+                                 const R (*s_rho)[TZN], const cpx<R> (*s_cell)[CW], int* err, bool report,
+                                 NodeCoef<R>& c, bool& topfree)
 {
     const int nx = p.nx, ny = p.ny, nz = p.nz, ip = p.ipml;
     const bool inx = ix > ip && ix <= nx - ip, iny = iy > ip && iy <= ny - ip;
@@ -347,28 +358,8 @@ __device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz
     }
     topfree = p.fst && iz == 1;
 
-    auto cell = [&](const cpx<R> (*a)[TZN], cpx<R>* o) {
-        cpx<R> km[3][4];
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) {
-            int z = tz + kz - 1;
-            km[kz][0] = (((a[0][z] + a[3][z]) + a[1][z]) + a[4][z]) / R(4);
-            km[kz][1] = (((a[2][z] + a[5][z]) + a[1][z]) + a[4][z]) / R(4);
-            km[kz][2] = (((a[6][z] + a[3][z]) + a[7][z]) + a[4][z]) / R(4);
-            km[kz][3] = (((a[8][z] + a[5][z]) + a[7][z]) + a[4][z]) / R(4);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            o[1 + q] = (km[0][q] + km[1][q]) / R(2);
-            o[5 + q] = (km[2][q] + km[1][q]) / R(2);
-        }
-    };
-    cell(s_mu, c.u);
-    cell(s_lam, c.d);
-    if (topfree) {
-#pragma unroll
-        for (int q = 1; q <= 4; ++q) { c.u[q] = cpx<R>(0, 0); c.d[q] = cpx<R>(0, 0); }
-    }
+    c.cup = s_cell[tz - 1];
+    c.cdn = s_cell[tz];
     cpx<R> lc = s_lam[4][tz], mcn = s_mu[4][tz];
     R rc = s_rho[4][tz];
     R vp = csqrt_re((lc + R(2) * mcn) / rc);
@@ -420,21 +411,22 @@ __device__ __forceinline__ void launder(NodeCoef<R>& c)
 {
     R* f = reinterpret_cast<R*>(&c);
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(NodeCoef<R>) / sizeof(R)); ++i) asm volatile("" : "+v"(f[i]));
+    for (int i = 0; i < (int)(__builtin_offsetof(NodeCoef<R>, cup) / sizeof(R)); ++i) asm volatile("" : "+v"(f[i]));
 }
 
-// Output rows of a chunk of CH z-nodes are one contiguous span of the CSR
-// values (node after node, 3 rows each, neighbours ascending in y, x, z), so
-// the block builds the span in LDS and writes it front to back with
-// consecutive threads on consecutive entries: every store instruction covers
-// whole cache lines and the block's output is one sequential sweep.
+// The rows of a chunk of CH z-nodes are one contiguous span of the CSR values
+// (node after node, 3 rows each, neighbours ascending in y, x, z), so the
+// block builds the span in LDS and writes it front to back with consecutive
+// threads on consecutive entries: every store instruction covers whole cache
+// lines and the block's output is one sequential sweep.
 //
-// To build the span, thread t evaluates node n = t mod CH of the chunk for
-// the neighbour labels k = g, g + 16 (k < 27) of its label group
-// g = 4 (t / 64) + (t mod 64) / 16: each wave evaluates four labels at once
-// (the label is uniform over 16 lanes).  Each thread forms its node's
-// coefficients itself.  The label evaluation is a small fraction of the
-// time; the stores set it.
+// A block owns a segment of ASM_WAVE z-nodes of one (x, y) column; lane l of
+// every wave holds node l of the segment and forms its coefficients once
+// (the divisions and square roots there cost more than all 27 labels).  The
+// segment is then written chunk by chunk: wave w evaluates the labels
+// k = w, w + 4, ... (uniform over the wave, so only that label's formula
+// runs) for the chunk's lanes, and the span goes out while the next chunk is
+// evaluated.
 template <typename R>
 __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
     wt_grid_params p, const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ mu,
@@ -443,101 +435,134 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
     // element type moved as one vector: 16 bytes (FP64), 8 bytes (FP32)
     using V = typename std::conditional<sizeof(R) == 8, double2, float2>::type;
     constexpr int STG_N = CH * 3 * 81;
-    __shared__ __align__(16) V stage[STG_N];
-    __shared__ __align__(16) cpx<R> s_lam[9][TZC];
-    __shared__ __align__(16) cpx<R> s_mu[9][TZC];
-    __shared__ R s_rho[9][TZC];
+    constexpr int IN_BYTES = 2 * 9 * TZ * (int)sizeof(cpx<R>);
+    constexpr int STG_BYTES = STG_N * (int)sizeof(V);
+    // The lambda/mu tiles are dead once the node coefficients are formed; the
+    // span reuses their space.
+    __shared__ __align__(16) char s_raw[IN_BYTES > STG_BYTES ? IN_BYTES : STG_BYTES];
+    __shared__ R s_rho[9][TZ];
+    // Octant cells between tile z-slots j and j + 1: mu of the four (x, y)
+    // quadrants, then lambda; one pad entry keeps the lanes' rows apart in
+    // the LDS banks.
+    constexpr int CW = 9;
+    __shared__ __align__(16) cpx<R> s_cell[TZ - 1][CW];
+    cpx<R>(*s_lam)[TZ] = (cpx<R>(*)[TZ])s_raw;
+    cpx<R>(*s_mu)[TZ] = (cpx<R>(*)[TZ])(s_raw + 9 * TZ * sizeof(cpx<R>));
+    V* const stage = (V*)s_raw;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     int zs, ix, iy;
-    segment_of_block(nblk, (nz + CH - 1) / CH, nx, zs, ix, iy);
+    segment_of_block(nblk, (nz + ASM_WAVE - 1) / ASM_WAVE, nx, zs, ix, iy);
     const int t = threadIdx.x;
-    const int n = t & (CH - 1);
+    const int n = t & (ASM_WAVE - 1);                        // node of the segment
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave of the block
-    const int lg = (t & 63) >> 4;                             // label group in the wave
-    const int g = 4 * wv + lg;
+    const int z0 = zs * ASM_WAVE;                            // 0-based first z of the segment
+    const int zend = nz < z0 + ASM_WAVE ? nz : z0 + ASM_WAVE;
+
+    // Tile slot zz holds 1-based z = z0 + zz, clamped to the grid, of the
+    // clamped 3x3 column neighbourhood (column index 3 (x offset) + y offset).
+    for (int i = t; i < 9 * TZ; i += ASM_THREADS) {
+        const int cc = i / TZ, zz = i - cc * TZ;
+        int xg = ix + cc / 3 - 1, yg = iy + cc % 3 - 1, zg = z0 + zz;
+        xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
+        yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
+        zg = zg < 1 ? 1 : (zg > nz ? nz : zg);
+        const int64_t gi = (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz + zg - 1;
+        s_lam[cc][zz] = lam[gi];
+        s_mu[cc][zz] = mu[gi];
+        s_rho[cc][zz] = rho[gi];
+    }
+    __syncthreads();
+    // Octant-cell moduli (setcoefs3d.f:108-174): the mean of a quadrant's four
+    // columns at two z-levels.  A node's cells 1..4 are the interface above
+    // it and 5..8 the one below, so each interface is formed once (the two
+    // nodes sharing it add the same two terms, in either order).  At the
+    // free surface the cells above the top node are zero.
+    for (int i = t; i < (TZ - 1) * 4; i += ASM_THREADS) {
+        const int j = i >> 2, q = i & 3;
+        const int c0 = (q & 1) ? ((q & 2) ? 8 : 2) : ((q & 2) ? 6 : 0);
+        const int c1 = (q & 1) ? 5 : 3, c2 = (q & 2) ? 7 : 1;
+        auto km = [&](const cpx<R>(*a)[TZ], int z) {
+            return (((a[c0][z] + a[c1][z]) + a[c2][z]) + a[4][z]) / R(4);
+        };
+        cpx<R> uu = (km(s_mu, j) + km(s_mu, j + 1)) / R(2);
+        cpx<R> dd = (km(s_lam, j) + km(s_lam, j + 1)) / R(2);
+        if (j == 0 && z0 == 0 && p.fst) { uu = cpx<R>(0, 0); dd = cpx<R>(0, 0); }
+        s_cell[j][q] = uu;
+        s_cell[j][4 + q] = dd;
+    }
+    __syncthreads();
+
+    const int m = z0 + n;  // 0-based z of this lane's node
+    const bool live = m < zend;
+    const int iz = live ? m + 1 : zend, tz = iz - z0;
+    NodeCoef<R> c;
+    bool topfree;
+    const int mode = node_setup<R, TZ, CW>(p, ix, iy, iz, tz, s_lam, s_mu, s_rho, s_cell, err, live && wv == 0, c, topfree);
     const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), cxy = cxn * cyn;
+    const int czn = 1 + (iz > 1) + (iz < nz);
+    const int rl = 3 * cxy * czn;
     V* const col = (V*)(out + node_offset(1, ix, iy, nx, ny, nz));
-    {
-        const int m0 = zs * CH, m1 = m0 + CH < nz ? m0 + CH : nz;  // chunk = 0-based z-nodes [m0, m1)
-        // Tile slot zz holds 1-based z = m0 + zz, clamped to the grid, of the
-        // clamped 3x3 column neighbourhood (column index 3 (x offset) + y offset).
-        if (t < 9 * TZC) {
-            const int cc = t / TZC, zz = t - cc * TZC;
-            int xg = ix + cc / 3 - 1, yg = iy + cc % 3 - 1, zg = m0 + zz;
-            xg = xg < 1 ? 1 : (xg > nx ? nx : xg);
-            yg = yg < 1 ? 1 : (yg > ny ? ny : yg);
-            zg = zg < 1 ? 1 : (zg > nz ? nz : zg);
-            const int64_t gi = (int64_t)(xg - 1) * nz + (int64_t)(yg - 1) * nx * nz + zg - 1;
-            s_lam[cc][zz] = lam[gi];
-            s_mu[cc][zz] = mu[gi];
-            s_rho[cc][zz] = rho[gi];
-        }
-        __syncthreads();
-        const int m = m0 + n;
-        const int iz = m + 1, tz = n + 1;
-        NodeCoef<R> c;
-        bool topfree = false;
-        int mode = 0;
-        if (m < m1) mode = node_setup<R, TZC>(p, ix, iy, iz, tz, s_lam, s_mu, s_rho, err, g == 0, c, topfree);
-        const int czn = 1 + (iz > 1) + (iz < nz);
-        const int rl = 3 * cxy * czn;
+    __syncthreads();  // lambda/mu tiles dead from here on
+
+#pragma unroll 1
+    for (int m0 = z0; m0 < zend; m0 += CH) {
+        const int m1 = m0 + CH < zend ? m0 + CH : zend;  // chunk = 0-based z-nodes [m0, m1)
+        const bool mine = live && m >= m0 && m < m1;
         const int base = 9 * cxy * (col_units(m, nz) - col_units(m0, nz));
-        // The wave's four label groups take turns, so the label is uniform
-        // (a scalar) inside label_block and only its own formula runs.
 #pragma unroll 1
-        for (int kk = 4 * wv; kk < 27; kk += 16) {
-#pragma unroll 1
-            for (int j = 0; j < 4; ++j) {
-                const int k = kk + j;
-                if (k >= 27) break;
-                const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
-                if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
-                if (lg != j || m >= m1 || iz + oz < 1 || iz + oz > nz) continue;
-                // density the mass lumping uses at this neighbour (clamped
-                // gather, top layer zeroed at the free surface,
-                // setcoefs3d.f:187-191)
-                R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
-                if (topfree && oz == -1) ro = R(0);
-                // Keep the compiler from hoisting every label's subexpressions
-                // out of the loop (all live at once, they spill): the node
-                // coefficients count as redefined on each pass.
-                launder(c);
-                cpx<R> v[9];
-                label_block<R>(k, c, ro, v);
-                if (mode == 2) {
+        for (int k = wv; k < 27; k += 4) {
+            const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
+            if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
+            if (!mine || iz + oz < 1 || iz + oz > nz) continue;
+            // density the mass lumping uses at this neighbour (clamped gather,
+            // top layer zeroed at the free surface, setcoefs3d.f:187-191)
+            R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
+            if (topfree && oz == -1) ro = R(0);
+            // Keep the compiler from hoisting every label's subexpressions out
+            // of the loop (all live at once, they spill): the node
+            // coefficients count as redefined on each pass.
+            launder(c);
+            cpx<R> v[9];
+            label_block<R>(k, c, ro, v);
+            if (mode == 2) {
 #pragma unroll
-                    for (int e = 0; e < 9; ++e)
-                        v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
-                }
-                const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
-                const int rz = oz > -1 ? (iz > 1) + (oz > 0) : 0;
-                const int pos = base + ((ry * cxn + rx) * czn + rz) * 3;
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int e = 0; e < 3; ++e) {
-                        V w;
-                        w.x = v[3 * r + e].re;
-                        w.y = v[3 * r + e].im;
-                        stage[pos + r * rl + e] = w;
-                    }
+                for (int e = 0; e < 9; ++e)
+                    v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
             }
+            const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
+            const int rz = oz > -1 ? (iz > 1) + (oz > 0) : 0;
+            const int pos = base + ((ry * cxn + rx) * czn + rz) * 3;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    V w;
+                    w.x = v[3 * r + e].re;
+                    w.y = v[3 * r + e].im;
+                    stage[pos + r * rl + e] = w;
+                }
         }
         __syncthreads();
-        // Span out: every LDS read is issued before the first store, so the
-        // wave waits on LDS once per chunk.
+        // Span out.  The LDS reads of a half are issued before its first
+        // store, so a wave waits on LDS twice per chunk; the stores drain
+        // while the next chunk is evaluated.
         const int len = 9 * cxy * (col_units(m1, nz) - col_units(m0, nz));
         V* const dst = col + 9 * cxy * col_units(m0, nz);
         constexpr int NS = (STG_N + ASM_THREADS - 1) / ASM_THREADS;
-        V w[NS];
+        constexpr int NH = 4;
 #pragma unroll
-        for (int u = 0; u < NS; ++u) {
-            const int i = t + u * ASM_THREADS;
-            w[u] = stage[i < STG_N ? i : STG_N - 1];
+        for (int h = 0; h < NS; h += NH) {
+            V w[NH];
+#pragma unroll
+            for (int u = 0; u < NH; ++u) {
+                const int i = t + (h + u) * ASM_THREADS;
+                w[u] = stage[i < STG_N ? i : STG_N - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < NH; ++u)
+                if (t + (h + u) * ASM_THREADS < len) dst[t + (h + u) * ASM_THREADS] = w[u];
         }
-#pragma unroll
-        for (int u = 0; u < NS; ++u)
-            if (t + u * ASM_THREADS < len) dst[t + u * ASM_THREADS] = w[u];
+        __syncthreads();  // span read before the next chunk overwrites it
     }
 }
 
@@ -603,7 +628,7 @@ int wt_launch_values(const wt_grid_params& p, const void* d_lam, const void* d_m
 {
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return WT_ERR_ARG;
     if (!p.usemin && p.fst) return WT_ERR_UNSUPPORTED;
-    const int64_t nblk = (int64_t)((p.nz + CH - 1) / CH) * p.nx * p.ny;
+    const int64_t nblk = (int64_t)((p.nz + ASM_WAVE - 1) / ASM_WAVE) * p.nx * p.ny;
     if (nblk > 0x7fffffff) return WT_ERR_ARG;
     hipLaunchKernelGGL(assemble_values_kernel<R>, dim3((unsigned)nblk), dim3(ASM_THREADS), 0, stream, p,
                        (const cpx<R>*)d_lam, (const cpx<R>*)d_mu, (const R*)d_rho, (cpx<R>*)d_vals, d_err,
