@@ -14,9 +14,9 @@
 // stretch factors in registers.  A node's output is 3 rows of up to 81
 // entries (neighbours ascending in y, x, z), and the rows of consecutive
 // nodes follow each other in the CSR arrays, so a chunk of 16 nodes is one
-// contiguous span.  The block builds each chunk's span in LDS -- wave w
-// takes the neighbour labels k = w, w + 4, ..., the label uniform over the
-// wave so only its formula runs -- and writes it front to back, consecutive
+// contiguous span.  The block builds each chunk's span in LDS -- each wave
+// takes seven of the 27 neighbour labels, the label uniform over the wave so
+// only its formula runs -- and writes it front to back, consecutive
 // threads on consecutive entries, while the next chunk is evaluated.  The
 // interior stencil is the stretched form with r1 = 1, r2 = 0 (identical
 // values; the reference notes the identity at fdfd3d.f:56-57).  CSR offsets
@@ -414,6 +414,18 @@ __device__ __forceinline__ void launder(NodeCoef<R>& c)
     for (int i = 0; i < (int)(__builtin_offsetof(NodeCoef<R>, cup) / sizeof(R)); ++i) asm volatile("" : "+v"(f[i]));
 }
 
+// Neighbour labels of each wave of a block, 5 bits per turn (31 = none):
+// two corner labels (three nonzero offsets, the costliest formulas), three
+// edge labels and two face labels each, the centre on the fourth wave, so
+// the four waves reach the chunk's barrier together.
+constexpr uint64_t lbl7(int a, int b, int c, int d, int e, int f, int g)
+{
+    return (uint64_t)a | (uint64_t)b << 5 | (uint64_t)c << 10 | (uint64_t)d << 15 | (uint64_t)e << 20 |
+           (uint64_t)f << 25 | (uint64_t)g << 30;
+}
+__constant__ const uint64_t kWaveLabels[4] = {lbl7(0, 2, 1, 3, 5, 4, 10), lbl7(6, 8, 7, 9, 11, 12, 14),
+                                              lbl7(18, 20, 15, 17, 19, 16, 22), lbl7(24, 26, 21, 23, 25, 13, 31)};
+
 // The rows of a chunk of CH z-nodes are one contiguous span of the CSR values
 // (node after node, 3 rows each, neighbours ascending in y, x, z), so the
 // block builds the span in LDS and writes it front to back with consecutive
@@ -422,9 +434,9 @@ __device__ __forceinline__ void launder(NodeCoef<R>& c)
 //
 // A block owns a segment of ASM_WAVE z-nodes of one (x, y) column; lane l of
 // every wave holds node l of the segment and forms its coefficients once
-// (the divisions and square roots there cost more than all 27 labels).  The
-// segment is then written chunk by chunk: wave w evaluates the labels
-// k = w, w + 4, ... (uniform over the wave, so only that label's formula
+// for all chunks (their divisions and square roots are not repeated).  The
+// segment is then written chunk by chunk: each wave evaluates its seven
+// labels of kWaveLabels (uniform over the wave, so only that label's formula
 // runs) for the chunk's lanes, and the span goes out while the next chunk is
 // evaluated.
 template <typename R>
@@ -510,7 +522,9 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
         const bool mine = live && m >= m0 && m < m1;
         const int base = 9 * cxy * (col_units(m, nz) - col_units(m0, nz));
 #pragma unroll 1
-        for (int k = wv; k < 27; k += 4) {
+        for (int turn = 0; turn < 7; ++turn) {
+            const int k = (int)((kWaveLabels[wv] >> (5 * turn)) & 31);
+            if (k > 26) break;
             const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
             if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
             if (!mine || iz + oz < 1 || iz + oz > nz) continue;
