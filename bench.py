@@ -106,6 +106,41 @@ def cpu_baseline(nx, ny, nz, nrhs, budget):
                       % (len(sample), sample[:3], t, rate / 1e9, total, t_tree)}
 
 
+def reference_assembly(nx, ny, nz, dx, freq, vp, vs, rho, gpu_ms):
+    """The reference's own assembly (SREL3D + SPARSE_ADJ2CRS, then ASMBLE with
+    its SETCOEFS3D / FDFD3D / PML3D physics), compiled from its Fortran
+    sources into oracle/_ref/libwtref32.so as shipped (single precision,
+    one thread), timed on the host on the same model; None when that build
+    is absent."""
+    import ctypes
+    so = os.path.join(ROOT, "oracle", "_ref", "libwtref32.so")
+    if not os.path.exists(so):
+        return None
+    L = ctypes.CDLL(so)
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    F = lambda v: ctypes.byref(ctypes.c_float(v))
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    nv, nzr, ierr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(-1)
+    t0 = time.perf_counter()
+    L.wtref_graph_size(I(nx), I(ny), I(nz), ctypes.byref(nv), ctypes.byref(nzr))
+    ir = np.zeros(nv.value + 1, np.int32)
+    jc = np.zeros(nzr.value, np.int32)
+    L.wtref_graph(I(nx), I(ny), I(nz), P(ir), P(jc))
+    t_graph = time.perf_counter() - t0
+    vals = np.zeros(jc.size, np.complex64)
+    v32 = [np.ascontiguousarray(x, np.float32) for x in (vp, vs, rho)]
+    t0 = time.perf_counter()
+    L.wtref_assemble(I(nx), I(ny), I(nz), F(dx), F(freq), F(999.999), I(1), I(1), F(10 * dx), F(1e-3),
+                     P(v32[0]), P(v32[1]), P(v32[2]), I(jc.size), P(ir), P(jc), P(vals), ctypes.byref(ierr))
+    t_asm = time.perf_counter() - t0
+    if ierr.value != 0:
+        return None
+    return {"s": t_asm, "graph_s": t_graph, "unit": "s per assembly", "cores": 1, "kind": "reference",
+            "gpu_ms": gpu_ms, "gpu_speedup": t_asm / (gpu_ms * 1e-3) if gpu_ms else None,
+            "sample": "ASMBLE of the %dx%dx%d model in complex64 (the reference as shipped, %d values), one "
+                      "thread; the GPU figure is the FP64 assembly" % (nx, ny, nz, jc.size)}
+
+
 GEMM_KERNEL = "gemm_batched_kernel<float, 0>"
 
 
@@ -308,6 +343,10 @@ def main():
     asm_gbs = asm["bytes"] / (asm["ms"] * 1e-3) / 1e9 if asm["ms"] > 0 else 0.0
     progress("CPU baseline")
     cpu = None if a.no_cpu else cpu_baseline(nx, ny, nz, nrhs, a.cpu_budget)
+    if cpu is not None:
+        progress("CPU reference assembly")
+        cpu["assembly_reference"] = reference_assembly(nx, ny, nz, dx, freq, vp, vs, rho,
+                                                       asm["ms"] / max(asm["launches"], 1))
     progress("serving sample")
     serve = None if a.no_serve else serving_sample(a.serve_rate, a.serve_seconds)
     value = nrhs / t32
