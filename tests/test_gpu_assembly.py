@@ -93,10 +93,11 @@ def test_gpu_values_layered_64x64x48(prec):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
-@pytest.mark.parametrize("dims", [(6, 5, 150), (4, 3, 65), (5, 4, 128)])
+@pytest.mark.parametrize("dims", [(6, 5, 150), (4, 3, 65), (5, 4, 128), (5, 3, 16), (7, 6, 17), (4, 6, 49)])
 def test_gpu_values_several_z_segments(dims, prec):
     """Columns longer than one 64-node segment (a one-node last segment, an
-    exact multiple, three segments): every segment's rows land at their CSR
+    exact multiple, three segments) and columns ending at, one past and one
+    past three 16-node output chunks: every chunk's rows land at their CSR
     offsets; heterogeneous random model, PML on every side."""
     nx, ny, nz = dims
     rng = np.random.default_rng(77 + nz)
