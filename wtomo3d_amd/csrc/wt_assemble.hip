@@ -65,6 +65,7 @@ struct NodeCoef {
     // Octant-cell mu and lambda, index 1..8, held in LDS: cells 1..4 lie
     // above the node (cup: mu 1..4, then lambda 1..4), cells 5..8 below it
     // (cdn: mu 5..8, then lambda 5..8).
+    // (The numeric members come first: launder() walks them as an R array.)
     const cpx<R>* cup;
     const cpx<R>* cdn;
     __device__ cpx<R> U(int i) const { return i <= 4 ? cup[i - 1] : cdn[i - 5]; }
@@ -308,8 +309,8 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
     }
 }
 
-// Values kernel geometry: a block of ASM_THREADS threads owns one chunk of CH
-// z-nodes of an (x, y) column.
+// Values kernel geometry: a block of ASM_THREADS threads owns a segment of
+// ASM_WAVE z-nodes of an (x, y) column and writes it in chunks of CH nodes.
 constexpr int CH = 16;
 constexpr int ASM_THREADS = 256;
 constexpr int TZ = ASM_WAVE + 2;  // tile z-slots of a segment (one node either side)
@@ -319,7 +320,7 @@ constexpr int TZ = ASM_WAVE + 2;  // tile z-slots of a segment (one node either 
 // column and 3 inside.
 __device__ inline int col_units(int m, int nz) { return 3 * m - (m >= 1) - (m >= nz); }
 
-// Block id -> (z chunk, x, y), with consecutive ids of one XCD (blocks go
+// Block id -> (z segment, x, y), with consecutive ids of one XCD (blocks go
 // round-robin to the 8 XCDs) mapped to consecutive columns, so the blocks
 // sharing an L2 share input columns and write adjacent output.
 __device__ inline void segment_of_block(int64_t nblk, int nzs, int nx, int& zs, int& ix, int& iy)
