@@ -184,7 +184,7 @@ def plane_wave_block(model, seed_offset=0):
                                    model["seed"] + seed_offset)
 
 
-def benchmark(nmodels=256, seed=1004, group_bytes=100e9, reps=2, autotune_models=48, progress=None):
+def benchmark(nmodels=256, seed=1004, group_bytes=160e9, reps=2, autotune_models=48, progress=None):
     """The batched-small-models workload: ``nmodels`` (256) models, every side uniform in
     16-40 nodes, a seeded layered medium each (dx = 250 m, 5-node PML), a
     frequency uniform in 0.05-0.5 Hz and 1-16 plane-wave right-hand sides
@@ -290,7 +290,7 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="run the batched small-model workload once and print JSON")
     ap.add_argument("--models", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1004)
-    ap.add_argument("--group-gb", type=float, default=100.0)
+    ap.add_argument("--group-gb", type=float, default=160.0)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--autotune-models", type=int, default=48)
     a = ap.parse_args()
