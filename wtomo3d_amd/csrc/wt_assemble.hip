@@ -309,6 +309,66 @@ __device__ __forceinline__ void label_block(int k, const NodeCoef<R>& c, R ro, c
     }
 }
 
+// The eight corner labels (all three offsets nonzero) share one formula that
+// differs only in the octant cell and in signs, so one pass can evaluate a
+// different corner on every 16 lanes.  kCornerK[j] is the label of corner j;
+// kCorner[j] packs its cell (bits 0-3) and, set for +, the signs of the
+// xy, xz, yz off-diagonal terms (bits 4-6) and, per diagonal entry x, y, z,
+// of the ra and rb terms and of the second r1 term inside rb (bits 7-15).
+// The table was read off label_block's corner cases; corner_block computes
+// exactly what those cases compute, operation for operation.
+__constant__ const int kCornerK[8] = {0, 2, 6, 8, 18, 20, 24, 26};
+__constant__ const uint32_t kCorner[8] = {0x9271, 0x2015, 0xcac3, 0x78a7, 0x1522, 0xa746, 0x4d94, 0xfff8};
+
+template <typename R, typename F, typename E>
+__device__ __forceinline__ void corner_block(uint32_t prm, const cpx<R>* cup, const cpx<R>* cdn, F get, R wm4o8,
+                                             R ro, E emit)
+{
+    // get(i): the node's r-coefficient i (0..8 = rx2, ry2, rz2, rxy, rxz, ryz,
+    // rx1, ry1, rz1), fetched where it is used; emit(e, value) takes entry e
+    // of the row-major 3x3 block as soon as it is formed
+    const int cell = (int)(prm & 15);
+    auto pos = [&](int b) { return ((prm >> (4 + b)) & 1) != 0; };
+    const cpx<R> U = cell <= 4 ? cup[cell - 1] : cdn[cell - 5];
+    const cpx<R> D = cell <= 4 ? cup[3 + cell] : cdn[cell - 1];
+    const cpx<R> L = (D + U) + U, P = D + U;
+    const cpx<R> m4(wm4o8 * ro, R(0));
+    auto corner = [&](cpx<R> q1, cpx<R> qs, bool sa, cpx<R> ra, bool sb, cpx<R> rb) {
+        cpx<R> s = q1 * L + qs * U;
+        cpx<R> ta = ra * L, tb = rb * U;
+        s = s + (sa ? ta : -ta);
+        s = s + (sb ? tb : -tb);
+        return s + m4;
+    };
+    {
+        const cpx<R> rx2 = get(0), ry2 = get(1), rz2 = get(2), ry1 = get(7), rz1 = get(8);
+        const cpx<R> rbx = pos(5) ? ry1 + rz1 : ry1 - rz1;
+        emit(0, corner(rx2, ry2 + rz2, pos(3), get(6), pos(4), rbx));
+    }
+    {
+        const cpx<R> rx2 = get(0), ry2 = get(1), rz2 = get(2), rx1 = get(6), rz1 = get(8);
+        const cpx<R> rby = pos(8) ? rx1 + rz1 : rx1 - rz1;
+        emit(4, corner(ry2, rx2 + rz2, pos(6), get(7), pos(7), rby));
+    }
+    {
+        const cpx<R> rx2 = get(0), ry2 = get(1), rz2 = get(2), rx1 = get(6), ry1 = get(7);
+        const cpx<R> rbz = pos(11) ? rx1 + ry1 : rx1 - ry1;
+        emit(8, corner(rz2, rx2 + ry2, pos(9), get(8), pos(10), rbz));
+    }
+    cpx<R> t0 = get(3) * P;
+    if (!pos(0)) t0 = -t0;
+    emit(1, t0);
+    emit(3, t0);
+    cpx<R> t1 = get(4) * P;
+    if (!pos(1)) t1 = -t1;
+    emit(2, t1);
+    emit(6, t1);
+    cpx<R> t2 = get(5) * P;
+    if (!pos(2)) t2 = -t2;
+    emit(5, t2);
+    emit(7, t2);
+}
+
 // Values kernel geometry: a block of ASM_THREADS threads owns a segment of
 // ASM_WAVE z-nodes of an (x, y) column and writes it in chunks of CH nodes.
 constexpr int CH = 16;
@@ -415,17 +475,18 @@ __device__ __forceinline__ void launder(NodeCoef<R>& c)
     for (int i = 0; i < (int)(__builtin_offsetof(NodeCoef<R>, cup) / sizeof(R)); ++i) asm volatile("" : "+v"(f[i]));
 }
 
-// Neighbour labels of each wave of a block, 5 bits per turn (31 = none):
-// two corner labels (three nonzero offsets, the costliest formulas), three
-// edge labels and two face labels each, the centre on the fourth wave, so
-// the four waves reach the chunk's barrier together.
+// Neighbour labels of each wave of a block, 5 bits per turn (31 = none; 28
+// and 29 = corners 0-3 and 4-7 of kCornerK, one per 16 lanes): the corners in
+// two four-wide passes, three edge labels (two nonzero offsets) and one to
+// three face labels each, the centre on the third wave, so the four waves
+// reach the chunk's barrier together.
 constexpr uint64_t lbl7(int a, int b, int c, int d, int e, int f, int g)
 {
     return (uint64_t)a | (uint64_t)b << 5 | (uint64_t)c << 10 | (uint64_t)d << 15 | (uint64_t)e << 20 |
            (uint64_t)f << 25 | (uint64_t)g << 30;
 }
-__constant__ const uint64_t kWaveLabels[4] = {lbl7(0, 2, 1, 3, 5, 4, 10), lbl7(6, 8, 7, 9, 11, 12, 14),
-                                              lbl7(18, 20, 15, 17, 19, 16, 22), lbl7(24, 26, 21, 23, 25, 13, 31)};
+__constant__ const uint64_t kWaveLabels[4] = {lbl7(28, 1, 3, 4, 5, 31, 31), lbl7(29, 19, 21, 22, 23, 31, 31),
+                                              lbl7(7, 9, 10, 11, 13, 31, 31), lbl7(12, 14, 15, 16, 17, 25, 31)};
 
 // The rows of a chunk of CH z-nodes are one contiguous span of the CSR values
 // (node after node, 3 rows each, neighbours ascending in y, x, z), so the
@@ -525,7 +586,50 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
 #pragma unroll 1
         for (int turn = 0; turn < 7; ++turn) {
             const int k = (int)((kWaveLabels[wv] >> (5 * turn)) & 31);
-            if (k > 26) break;
+            if (k == 31) break;
+            // Keep the compiler from hoisting any label's subexpressions out
+            // of the loop (all live at once, they spill): the node
+            // coefficients count as redefined on each pass.
+            launder(c);
+            if (k >= 28) {
+                // Four corners at once: lanes 16q..16q+15 evaluate corner
+                // 4 (k - 28) + q for the chunk's 16 nodes, whose coefficients
+                // come from the lanes holding those nodes.
+                const int q = (t & 63) >> 4, n16 = t & 15;
+                const int j = 4 * (k - 28) + q;
+                const int kc = kCornerK[j];
+                const int oy = kc / 9 - 1, ox = (kc / 3) % 3 - 1, oz = kc % 3 - 1;
+                const int src = (m0 - z0) + n16;
+                // Every lane evaluates (its tile indices stay in range for
+                // any node of the segment) and only lanes whose neighbour
+                // exists store, so the shuffles run with the whole wave.
+                auto get = [&](int i) {
+                    const cpx<R> a = i == 0 ? c.rx2 : i == 1 ? c.ry2 : i == 2 ? c.rz2 : i == 3 ? c.rxy :
+                                     i == 4 ? c.rxz : i == 5 ? c.ryz : i == 6 ? c.rx1 : i == 7 ? c.ry1 : c.rz1;
+                    return cpx<R>(__shfl(a.re, src), __shfl(a.im, src));
+                };
+                const R wm4N = __shfl(c.wm4o8, src);
+                const int modeN = __shfl(mode, src);
+                const int mN = m0 + n16, izN = mN + 1, tzN = mN - z0 + 1;
+                R ro = s_rho[3 * (ox + 1) + (oy + 1)][tzN + oz];
+                if (p.fst && izN == 1 && oz == -1) ro = R(0);
+                const bool ok = mN < m1 && iy + oy >= 1 && iy + oy <= ny && ix + ox >= 1 && ix + ox <= nx &&
+                                izN + oz >= 1 && izN + oz <= nz;
+                const int cznN = 1 + (izN > 1) + (izN < nz), rlN = 3 * cxy * cznN;
+                const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
+                const int rz = oz > -1 ? (izN > 1) + (oz > 0) : 0;
+                const int posN = 9 * cxy * (col_units(mN, nz) - col_units(m0, nz)) + ((ry * cxn + rx) * cznN + rz) * 3;
+                auto emit = [&](int e, cpx<R> val) {
+                    if (!ok) return;
+                    if (modeN == 2) val = cpx<R>(R(0), R(0));  // a corner is never the centre
+                    V w;
+                    w.x = val.re;
+                    w.y = val.im;
+                    stage[posN + (e / 3) * rlN + (e % 3)] = w;
+                };
+                corner_block<R>(kCorner[j], s_cell[tzN - 1], s_cell[tzN], get, wm4N, ro, emit);
+                continue;
+            }
             const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
             if (iy + oy < 1 || iy + oy > ny || ix + ox < 1 || ix + ox > nx) continue;  // block-uniform
             if (!mine || iz + oz < 1 || iz + oz > nz) continue;
@@ -533,10 +637,6 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
             // top layer zeroed at the free surface, setcoefs3d.f:187-191)
             R ro = s_rho[3 * (ox + 1) + (oy + 1)][tz + oz];
             if (topfree && oz == -1) ro = R(0);
-            // Keep the compiler from hoisting every label's subexpressions out
-            // of the loop (all live at once, they spill): the node
-            // coefficients count as redefined on each pass.
-            launder(c);
             cpx<R> v[9];
             label_block<R>(k, c, ro, v);
             if (mode == 2) {
