@@ -15,8 +15,10 @@
 // entries (neighbours ascending in y, x, z), and the rows of consecutive
 // nodes follow each other in the CSR arrays, so a chunk of 16 nodes is one
 // contiguous span.  The block builds each chunk's span in LDS -- each wave
-// takes seven of the 27 neighbour labels, the label uniform over the wave so
-// only its formula runs -- and writes it front to back, consecutive
+// takes its share of the 27 neighbour labels, the label uniform over 16 lanes
+// (the eight corners four at a time, one formula with per-lane signs and
+// cells; the others one at a time, so only that label's formula runs) --
+// and writes it front to back, consecutive
 // threads on consecutive entries, while the next chunk is evaluated.  The
 // interior stencil is the stretched form with r1 = 1, r2 = 0 (identical
 // values; the reference notes the identity at fdfd3d.f:56-57).  CSR offsets
@@ -497,10 +499,9 @@ __constant__ const uint64_t kWaveLabels[4] = {lbl7(28, 1, 3, 4, 5, 31, 31), lbl7
 // A block owns a segment of ASM_WAVE z-nodes of one (x, y) column; lane l of
 // every wave holds node l of the segment and forms its coefficients once
 // for all chunks (their divisions and square roots are not repeated).  The
-// segment is then written chunk by chunk: each wave evaluates its seven
-// labels of kWaveLabels (uniform over the wave, so only that label's formula
-// runs) for the chunk's lanes, and the span goes out while the next chunk is
-// evaluated.
+// segment is then written chunk by chunk: each wave evaluates its labels of
+// kWaveLabels for the chunk's 16 nodes, and the span goes out while the next
+// chunk is evaluated.
 template <typename R>
 __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
     wt_grid_params p, const cpx<R>* __restrict__ lam, const cpx<R>* __restrict__ mu,
