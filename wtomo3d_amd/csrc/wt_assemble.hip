@@ -371,6 +371,51 @@ __device__ __forceinline__ void corner_block(uint32_t prm, const cpx<R>* cup, co
     emit(7, t2);
 }
 
+// The twelve edge labels (two nonzero offsets) fall into three families by
+// their zero offset -- z (labels 1, 7, 19, 25), x (3, 5, 21, 23), y (9, 11,
+// 15, 17) -- each with one formula that differs only in the two octant cells,
+// the signs of the xy, xz, yz terms and the order of the differences in them.
+// kEdge[j] packs cells a, b (bits 0-3, 4-7), the signs, set for + (bits
+// 8-10), and, set when a difference is lmu(a) - lmu(b), its order (bits
+// 11-13).  Read off label_block's edge cases; edge_block computes exactly
+// what those cases compute.
+__constant__ const int kEdgeK[12] = {1, 7, 19, 25, 3, 5, 21, 23, 9, 11, 15, 17};
+__constant__ const uint32_t kEdge[12] = {0x3951, 0x1837, 0x1862, 0x3948, 0x3f31, 0x3357,
+                                         0x3342, 0x3f68, 0x3721, 0x3865, 0x3834, 0x3778};
+
+// F: the family, 0 = z, 1 = x, 2 = y; get and emit as for corner_block.
+template <typename R, int F, typename G, typename E>
+__device__ __forceinline__ void edge_block(uint32_t prm, const cpx<R>* cup, const cpx<R>* cdn, G get, R wm3o12,
+                                           R ro, E emit)
+{
+    const int a = (int)(prm & 15), b = (int)((prm >> 4) & 15);
+    auto pos = [&](int bit) { return ((prm >> bit) & 1) != 0; };
+    const cpx<R> Ua = a <= 4 ? cup[a - 1] : cdn[a - 5], Ub = b <= 4 ? cup[b - 1] : cdn[b - 5];
+    const cpx<R> Da = a <= 4 ? cup[3 + a] : cdn[a - 1], Db = b <= 4 ? cup[3 + b] : cdn[b - 1];
+    const cpx<R> m3(wm3o12 * ro, R(0));
+    const cpx<R> lmuA = Da - Ua, lmuB = Db - Ub;
+    auto diff = [&](int ordbit) { return pos(ordbit) ? lmuA - lmuB : lmuB - lmuA; };
+    auto sgn = [&](int bit, cpx<R> x) { return pos(bit) ? x : -x; };
+    {
+        const cpx<R> rF2 = get(F == 0 ? 2 : (F == 1 ? 0 : 1));  // rz2, rx2, ry2
+        const cpx<R> tU = -(rF2 * (Ua + Ub)) + m3;
+        if (F == 2) {
+            emit(0, tU); emit(4, tU); emit(8, tU);
+        } else {
+            const cpx<R> tL = -(rF2 * (((Da + Ua) + Ua) + ((Db + Ub) + Ub))) + m3;
+            if (F == 0) { emit(0, tU); emit(4, tU); emit(8, tL); }
+            else { emit(0, tL); emit(4, tL); emit(8, tU); }
+        }
+    }
+    const cpx<R> S = (Da + Ua) + (Db + Ub);
+    const cpx<R> t2 = sgn(8, get(3) * (F == 0 ? S : diff(11)));
+    emit(1, t2); emit(3, F == 0 ? t2 : -t2);
+    const cpx<R> t3 = sgn(9, get(4) * (F == 2 ? S : diff(12)));
+    emit(2, t3); emit(6, F == 2 ? t3 : -t3);
+    const cpx<R> t4 = sgn(10, get(5) * (F == 1 ? S : diff(13)));
+    emit(5, t4); emit(7, F == 1 ? t4 : -t4);
+}
+
 // Values kernel geometry: a block of ASM_THREADS threads owns a segment of
 // ASM_WAVE z-nodes of an (x, y) column and writes it in chunks of CH nodes.
 constexpr int CH = 16;
@@ -477,18 +522,21 @@ __device__ __forceinline__ void launder(NodeCoef<R>& c)
     for (int i = 0; i < (int)(__builtin_offsetof(NodeCoef<R>, cup) / sizeof(R)); ++i) asm volatile("" : "+v"(f[i]));
 }
 
-// Neighbour labels of each wave of a block, 5 bits per turn (31 = none; 28
-// and 29 = corners 0-3 and 4-7 of kCornerK, one per 16 lanes): the corners in
-// two four-wide passes, three edge labels (two nonzero offsets) and one to
-// three face labels each, the centre on the third wave, so the four waves
-// reach the chunk's barrier together.
+// Neighbour labels of each wave of a block, 6 bits per turn: a label 0..26
+// runs alone (the label uniform over the wave); 32 + j runs a group of four
+// labels at once, one per 16 lanes -- j = 0, 1: corners 0-3, 4-7 of
+// kCornerK; j = 2, 3, 4: the z, x, y edge families of kEdgeK; 63 ends the
+// list.  The groups and the six face labels and the centre are spread so
+// the four waves reach the chunk's barrier together.
+constexpr int kQuad = 32, kEnd = 63;
 constexpr uint64_t lbl7(int a, int b, int c, int d, int e, int f, int g)
 {
-    return (uint64_t)a | (uint64_t)b << 5 | (uint64_t)c << 10 | (uint64_t)d << 15 | (uint64_t)e << 20 |
-           (uint64_t)f << 25 | (uint64_t)g << 30;
+    return (uint64_t)a | (uint64_t)b << 6 | (uint64_t)c << 12 | (uint64_t)d << 18 | (uint64_t)e << 24 |
+           (uint64_t)f << 30 | (uint64_t)g << 36;
 }
-__constant__ const uint64_t kWaveLabels[4] = {lbl7(28, 1, 3, 4, 5, 31, 31), lbl7(29, 19, 21, 22, 23, 31, 31),
-                                              lbl7(7, 9, 10, 11, 13, 31, 31), lbl7(12, 14, 15, 16, 17, 25, 31)};
+__constant__ const uint64_t kWaveLabels[4] = {
+    lbl7(kQuad + 0, kQuad + 2, kEnd, kEnd, kEnd, kEnd, kEnd), lbl7(kQuad + 1, kQuad + 3, kEnd, kEnd, kEnd, kEnd, kEnd),
+    lbl7(kQuad + 4, 4, 10, 12, kEnd, kEnd, kEnd), lbl7(13, 14, 16, 22, kEnd, kEnd, kEnd)};
 
 // The rows of a chunk of CH z-nodes are one contiguous span of the CSR values
 // (node after node, 3 rows each, neighbours ascending in y, x, z), so the
@@ -586,19 +634,20 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
         const int base = 9 * cxy * (col_units(m, nz) - col_units(m0, nz));
 #pragma unroll 1
         for (int turn = 0; turn < 7; ++turn) {
-            const int k = (int)((kWaveLabels[wv] >> (5 * turn)) & 31);
-            if (k == 31) break;
+            const int k = (int)((kWaveLabels[wv] >> (6 * turn)) & 63);
+            if (k == kEnd) break;
             // Keep the compiler from hoisting any label's subexpressions out
             // of the loop (all live at once, they spill): the node
             // coefficients count as redefined on each pass.
             launder(c);
-            if (k >= 28) {
-                // Four corners at once: lanes 16q..16q+15 evaluate corner
-                // 4 (k - 28) + q for the chunk's 16 nodes, whose coefficients
+            if (k >= kQuad) {
+                // Four labels at once: lanes 16q..16q+15 evaluate label q of
+                // group k - kQuad for the chunk's 16 nodes, whose coefficients
                 // come from the lanes holding those nodes.
+                const int grp = k - kQuad;
                 const int q = (t & 63) >> 4, n16 = t & 15;
-                const int j = 4 * (k - 28) + q;
-                const int kc = kCornerK[j];
+                const int j = grp < 2 ? 4 * grp + q : 4 * (grp - 2) + q;
+                const int kc = grp < 2 ? kCornerK[j] : kEdgeK[j];
                 const int oy = kc / 9 - 1, ox = (kc / 3) % 3 - 1, oz = kc % 3 - 1;
                 const int src = (m0 - z0) + n16;
                 // Every lane evaluates (its tile indices stay in range for
@@ -609,7 +658,7 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                                      i == 4 ? c.rxz : i == 5 ? c.ryz : i == 6 ? c.rx1 : i == 7 ? c.ry1 : c.rz1;
                     return cpx<R>(__shfl(a.re, src), __shfl(a.im, src));
                 };
-                const R wm4N = __shfl(c.wm4o8, src);
+                const R wmN = __shfl(grp < 2 ? c.wm4o8 : c.wm3o12, src);
                 const int modeN = __shfl(mode, src);
                 const int mN = m0 + n16, izN = mN + 1, tzN = mN - z0 + 1;
                 R ro = s_rho[3 * (ox + 1) + (oy + 1)][tzN + oz];
@@ -622,13 +671,21 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                 const int posN = 9 * cxy * (col_units(mN, nz) - col_units(m0, nz)) + ((ry * cxn + rx) * cznN + rz) * 3;
                 auto emit = [&](int e, cpx<R> val) {
                     if (!ok) return;
-                    if (modeN == 2) val = cpx<R>(R(0), R(0));  // a corner is never the centre
+                    if (modeN == 2) val = cpx<R>(R(0), R(0));  // a group label is never the centre
                     V w;
                     w.x = val.re;
                     w.y = val.im;
                     stage[posN + (e / 3) * rlN + (e % 3)] = w;
                 };
-                corner_block<R>(kCorner[j], s_cell[tzN - 1], s_cell[tzN], get, wm4N, ro, emit);
+                if (grp < 2) {
+                    corner_block<R>(kCorner[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                } else if (grp == 2) {
+                    edge_block<R, 0>(kEdge[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                } else if (grp == 3) {
+                    edge_block<R, 1>(kEdge[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                } else {
+                    edge_block<R, 2>(kEdge[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                }
                 continue;
             }
             const int oy = k / 9 - 1, ox = (k / 3) % 3 - 1, oz = k % 3 - 1;
