@@ -16,9 +16,10 @@
 // nodes follow each other in the CSR arrays, so a chunk of 16 nodes is one
 // contiguous span.  The block builds each chunk's span in LDS -- each wave
 // takes its share of the 27 neighbour labels, the label uniform over 16 lanes
-// (the eight corners four at a time, one formula with per-lane signs and
-// cells; the others one at a time, so only that label's formula runs) --
-// and writes it front to back, consecutive
+// (the eight corners and the three families of four edges four at a time, one
+// formula with per-lane cells and signs; the faces and the centre one at a
+// time, so only that label's formula runs) -- and writes it front to back,
+// consecutive
 // threads on consecutive entries, while the next chunk is evaluated.  The
 // interior stencil is the stretched form with r1 = 1, r2 = 0 (identical
 // values; the reference notes the identity at fdfd3d.f:56-57).  CSR offsets
@@ -26,10 +27,11 @@
 //
 // Measured at 96x96x64 in FP64: one wave per segment storing each neighbour
 // column's 144-byte row runs took 1.05 ms; the contiguous spans with the
-// per-interface cell table take 0.69 ms; the span stores alone (no values
-// computed) take 0.37 ms.  The rest is the per-node divisions and square
-// roots and the label evaluation at 16 of 64 lanes, with occupancy capped at
-// two blocks per CU by the FP64 registers (254) and the 76 KB of LDS.
+// per-interface cell table 0.69 ms, with the four-wide label groups 0.62 ms;
+// the span stores alone (no values computed) take 0.37 ms.  The rest is the
+// tile loads, the per-node divisions and square roots and the labels, with
+// occupancy capped at two blocks per CU by the FP64 registers (256) and the
+// 76 KB of LDS.
 #include "wt_common.h"
 
 #include <type_traits>
