@@ -547,8 +547,9 @@ __constant__ const uint64_t kWaveLabels[4] = {
 // lines and the block's output is one sequential sweep.
 //
 // A block owns a segment of ASM_WAVE z-nodes of one (x, y) column; lane l of
-// every wave holds node l of the segment and forms its coefficients once
-// for all chunks (their divisions and square roots are not repeated).  The
+// every wave holds node l of the segment.  Wave 0 forms the node
+// coefficients once for all chunks (their divisions and square roots are not
+// repeated) and the other waves take them from LDS.  The
 // segment is then written chunk by chunk: each wave evaluates its labels of
 // kWaveLabels for the chunk's 16 nodes, and the span goes out while the next
 // chunk is evaluated.
@@ -597,37 +598,62 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
         s_rho[cc][zz] = rho[gi];
     }
     __syncthreads();
-    // Octant-cell moduli (setcoefs3d.f:108-174): the mean of a quadrant's four
-    // columns at two z-levels.  A node's cells 1..4 are the interface above
-    // it and 5..8 the one below, so each interface is formed once (the two
-    // nodes sharing it add the same two terms, in either order).  At the
-    // free surface the cells above the top node are zero.
-    for (int i = t; i < (TZ - 1) * 4; i += ASM_THREADS) {
-        const int j = i >> 2, q = i & 3;
-        const int c0 = (q & 1) ? ((q & 2) ? 8 : 2) : ((q & 2) ? 6 : 0);
-        const int c1 = (q & 1) ? 5 : 3, c2 = (q & 2) ? 7 : 1;
-        auto km = [&](const cpx<R>(*a)[TZ], int z) {
-            return (((a[c0][z] + a[c1][z]) + a[c2][z]) + a[4][z]) / R(4);
-        };
-        cpx<R> uu = (km(s_mu, j) + km(s_mu, j + 1)) / R(2);
-        cpx<R> dd = (km(s_lam, j) + km(s_lam, j + 1)) / R(2);
-        if (j == 0 && z0 == 0 && p.fst) { uu = cpx<R>(0, 0); dd = cpx<R>(0, 0); }
-        s_cell[j][q] = uu;
-        s_cell[j][4 + q] = dd;
-    }
-    __syncthreads();
-
     const int m = z0 + n;  // 0-based z of this lane's node
     const bool live = m < zend;
     const int iz = live ? m + 1 : zend, tz = iz - z0;
+    // Wave 0 forms the node coefficients and leaves them in LDS past the
+    // lambda/mu tiles (space the span takes over later) while the other three
+    // waves build the octant-cell table; the other waves then read the
+    // coefficients instead of forming them again.
+    constexpr int NF = (int)(__builtin_offsetof(NodeCoef<R>, cup) / sizeof(R));
+    static_assert(IN_BYTES + (NF + 1) * ASM_WAVE * (int)sizeof(R) <= STG_BYTES, "coefficients overflow the span");
+    R* const s_coef = (R*)(s_raw + IN_BYTES);  // [NF][ASM_WAVE], then the mode word per node
+    int* const s_mode = (int*)(s_coef + NF * ASM_WAVE);
     NodeCoef<R> c;
     bool topfree;
-    const int mode = node_setup<R, TZ, CW>(p, ix, iy, iz, tz, s_lam, s_mu, s_rho, s_cell, err, live && wv == 0, c, topfree);
+    int mode;
+    if (wv == 0) {
+        mode = node_setup<R, TZ, CW>(p, ix, iy, iz, tz, s_lam, s_mu, s_rho, s_cell, err, live, c, topfree);
+        const R* f = reinterpret_cast<const R*>(&c);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) s_coef[i * ASM_WAVE + n] = f[i];
+        s_mode[n] = mode | (topfree ? 4 : 0);
+    } else {
+        // Octant-cell moduli (setcoefs3d.f:108-174): the mean of a quadrant's
+        // four columns at two z-levels.  A node's cells 1..4 are the interface
+        // above it and 5..8 the one below, so each interface is formed once
+        // (the two nodes sharing it add the same two terms, in either order).
+        // At the free surface the cells above the top node are zero.
+        for (int i = t - ASM_WAVE; i < (TZ - 1) * 4; i += ASM_THREADS - ASM_WAVE) {
+            const int j = i >> 2, q = i & 3;
+            const int c0 = (q & 1) ? ((q & 2) ? 8 : 2) : ((q & 2) ? 6 : 0);
+            const int c1 = (q & 1) ? 5 : 3, c2 = (q & 2) ? 7 : 1;
+            auto km = [&](const cpx<R>(*a)[TZ], int z) {
+                return (((a[c0][z] + a[c1][z]) + a[c2][z]) + a[4][z]) / R(4);
+            };
+            cpx<R> uu = (km(s_mu, j) + km(s_mu, j + 1)) / R(2);
+            cpx<R> dd = (km(s_lam, j) + km(s_lam, j + 1)) / R(2);
+            if (j == 0 && z0 == 0 && p.fst) { uu = cpx<R>(0, 0); dd = cpx<R>(0, 0); }
+            s_cell[j][q] = uu;
+            s_cell[j][4 + q] = dd;
+        }
+    }
+    __syncthreads();
+    if (wv != 0) {
+        R* f = reinterpret_cast<R*>(&c);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) f[i] = s_coef[i * ASM_WAVE + n];
+        const int mt = s_mode[n];
+        mode = mt & 3;
+        topfree = (mt & 4) != 0;
+        c.cup = s_cell[tz - 1];
+        c.cdn = s_cell[tz];
+    }
     const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny), cxy = cxn * cyn;
     const int czn = 1 + (iz > 1) + (iz < nz);
     const int rl = 3 * cxy * czn;
     V* const col = (V*)(out + node_offset(1, ix, iy, nx, ny, nz));
-    __syncthreads();  // lambda/mu tiles dead from here on
+    __syncthreads();  // lambda/mu tiles and coefficient copies dead from here on
 
 #pragma unroll 1
     for (int m0 = z0; m0 < zend; m0 += CH) {
