@@ -27,11 +27,11 @@
 //
 // Measured at 96x96x64 in FP64: one wave per segment storing each neighbour
 // column's 144-byte row runs took 1.05 ms; the contiguous spans with the
-// per-interface cell table 0.69 ms, with the four-wide label groups 0.62 ms;
-// the span stores alone (no values computed) take 0.37 ms.  The rest is the
-// tile loads, the per-node divisions and square roots and the labels, with
-// occupancy capped at two blocks per CU by the FP64 registers (256) and the
-// 76 KB of LDS.
+// per-interface cell table 0.69 ms, with the four-wide label groups 0.62 ms,
+// with the coefficients formed by one wave 0.61 ms; the span stores alone (no
+// values computed) take 0.37 ms.  The labels cost 0.15 ms of the rest, the
+// tile loads and barriers the remainder, with occupancy capped at two blocks
+// per CU by the FP64 registers (256) and the 76 KB of LDS.
 #include "wt_common.h"
 
 #include <type_traits>
