@@ -16,10 +16,9 @@
 // nodes follow each other in the CSR arrays, so a chunk of 16 nodes is one
 // contiguous span.  The block builds each chunk's span in LDS -- each wave
 // takes its share of the 27 neighbour labels, the label uniform over 16 lanes
-// (the eight corners and the three families of four edges four at a time, one
-// formula with per-lane cells and signs; the faces and the centre one at a
-// time, so only that label's formula runs) -- and writes it front to back,
-// consecutive
+// (the eight corners and the three families of four edges four at a time and
+// the three pairs of faces two at a time, one formula with per-lane cells and
+// signs; the centre alone) -- and writes it front to back, consecutive
 // threads on consecutive entries, while the next chunk is evaluated.  The
 // interior stencil is the stretched form with r1 = 1, r2 = 0 (identical
 // values; the reference notes the identity at fdfd3d.f:56-57).  CSR offsets
@@ -28,10 +27,11 @@
 // Measured at 96x96x64 in FP64: one wave per segment storing each neighbour
 // column's 144-byte row runs took 1.05 ms; the contiguous spans with the
 // per-interface cell table 0.69 ms, with the four-wide label groups 0.62 ms,
-// with the coefficients formed by one wave 0.61 ms; the span stores alone (no
-// values computed) take 0.37 ms.  The labels cost 0.15 ms of the rest, the
-// tile loads and barriers the remainder, with occupancy capped at two blocks
-// per CU by the FP64 registers (256) and the 76 KB of LDS.
+// with the coefficients formed by one wave 0.61 ms, with the face pairs and
+// the waves' label sets balanced by measurement 0.59 ms; the span stores
+// alone (no values computed) take 0.37 ms.  The labels cost 0.15 ms of the
+// rest, the tile loads and barriers the remainder, with occupancy capped at
+// two blocks per CU by the FP64 registers (256) and the 76 KB of LDS.
 #include "wt_common.h"
 
 #include <type_traits>
@@ -418,6 +418,65 @@ __device__ __forceinline__ void edge_block(uint32_t prm, const cpx<R>* cup, cons
     emit(5, t4); emit(7, F == 1 ? t4 : -t4);
 }
 
+// The six face labels (one nonzero offset) pair up by that offset's axis --
+// y (labels 4, 22), x (10, 16), z (12, 14) -- with one formula per pair that
+// differs only in the octant cells and their order in each four-term sum.
+// kFace[j] packs three lists of four cells, 4 bits each (list l, entry i at
+// bits 16 l + 4 i): y uses list 0 throughout; x list 0 for the diagonal, 1 for
+// the xy term, 2 for xz and yz; z list 0, and list 2 for the yz term.  Read
+// off label_block's face cases; face_block computes exactly what those cases
+// compute.
+__constant__ const int kFaceK[6] = {4, 22, 10, 16, 12, 14};
+__constant__ const uint64_t kFace[6] = {0x357135713571ull, 0x642864286428ull, 0x615216525261ull,
+                                        0x384783474738ull, 0x234132413241ull, 0x765867586758ull};
+
+// F: the axis, 0 = y, 1 = x, 2 = z; get and emit as for corner_block.
+template <typename R, int F, typename G, typename E>
+__device__ __forceinline__ void face_block(uint64_t cells, const cpx<R>* cup, const cpx<R>* cdn, G get, R wm2o6,
+                                           R ro, E emit)
+{
+    auto cell = [&](int l, int i) { return (int)((cells >> (16 * l + 4 * i)) & 15); };
+    auto Uc = [&](int x) { return x <= 4 ? cup[x - 1] : cdn[x - 5]; };
+    auto Dc = [&](int x) { return x <= 4 ? cup[3 + x] : cdn[x - 1]; };
+    auto U = [&](int l, int i) { return Uc(cell(l, i)); };
+    auto lpu = [&](int l, int i) { const int x = cell(l, i); return Dc(x) + Uc(x); };
+    auto lmu = [&](int l, int i) { const int x = cell(l, i); return Dc(x) - Uc(x); };
+    auto lp2u = [&](int l, int i) { const int x = cell(l, i); return (Dc(x) + Uc(x)) + Uc(x); };
+    const cpx<R> m2(wm2o6 * ro, R(0));
+    if (F == 0) {
+        const cpx<R> t0 = get(1) * (((U(0, 0) + U(0, 1)) + U(0, 2)) + U(0, 3)) + m2;
+        emit(0, t0); emit(4, t0); emit(8, t0);
+        const cpx<R> t2 = get(3) * (((lmu(0, 0) - lmu(0, 1)) + lmu(0, 2)) - lmu(0, 3));
+        emit(1, t2); emit(3, -t2);
+        const cpx<R> t3 = -(get(4) * (((lpu(0, 0) + lpu(0, 1)) - lpu(0, 2)) - lpu(0, 3)));
+        emit(2, t3); emit(6, t3);
+        const cpx<R> t4 = -(get(5) * (((lmu(0, 0) - lmu(0, 1)) - lmu(0, 2)) + lmu(0, 3)));
+        emit(5, t4); emit(7, -t4);
+    } else if (F == 1) {
+        const cpx<R> rx2 = get(0);
+        const cpx<R> t0 = rx2 * (((lp2u(0, 0) + lp2u(0, 1)) + lp2u(0, 2)) + lp2u(0, 3)) + m2;
+        const cpx<R> t1 = rx2 * (((U(0, 0) + U(0, 1)) + U(0, 2)) + U(0, 3)) + m2;
+        emit(0, t0); emit(4, t0); emit(8, t1);
+        const cpx<R> t2 = get(3) * (((lmu(1, 0) - lmu(1, 1)) + lmu(1, 2)) - lmu(1, 3));
+        emit(1, t2); emit(3, -t2);
+        const cpx<R> t3 = -(get(4) * (((lmu(2, 0) - lmu(2, 1)) + lmu(2, 2)) - lmu(2, 3)));
+        emit(2, t3); emit(6, -t3);
+        const cpx<R> t4 = get(5) * (((lpu(2, 0) + lpu(2, 1)) - lpu(2, 2)) - lpu(2, 3));
+        emit(5, t4); emit(7, t4);
+    } else {
+        const cpx<R> rz2 = get(2);
+        const cpx<R> t0 = rz2 * (((U(0, 0) + U(0, 1)) + U(0, 2)) + U(0, 3)) + m2;
+        const cpx<R> t1 = rz2 * (((lp2u(0, 0) + lp2u(0, 1)) + lp2u(0, 2)) + lp2u(0, 3)) + m2;
+        emit(0, t0); emit(4, t0); emit(8, t1);
+        const cpx<R> t2 = -(get(3) * (((lpu(0, 0) + lpu(0, 1)) - lpu(0, 2)) - lpu(0, 3)));
+        emit(1, t2); emit(3, t2);
+        const cpx<R> t3 = get(4) * (((lmu(0, 0) - lmu(0, 1)) + lmu(0, 2)) - lmu(0, 3));
+        emit(2, t3); emit(6, -t3);
+        const cpx<R> t4 = get(5) * (((lmu(2, 0) - lmu(2, 1)) + lmu(2, 2)) - lmu(2, 3));
+        emit(5, t4); emit(7, -t4);
+    }
+}
+
 // Values kernel geometry: a block of ASM_THREADS threads owns a segment of
 // ASM_WAVE z-nodes of an (x, y) column and writes it in chunks of CH nodes.
 constexpr int CH = 16;
@@ -525,11 +584,12 @@ __device__ __forceinline__ void launder(NodeCoef<R>& c)
 }
 
 // Neighbour labels of each wave of a block, 6 bits per turn: a label 0..26
-// runs alone (the label uniform over the wave); 32 + j runs a group of four
-// labels at once, one per 16 lanes -- j = 0, 1: corners 0-3, 4-7 of
-// kCornerK; j = 2, 3, 4: the z, x, y edge families of kEdgeK; 63 ends the
-// list.  The groups and the six face labels and the centre are spread so
-// the four waves reach the chunk's barrier together.
+// runs alone (the label uniform over the wave); kQuad + j runs a group at
+// once, one label per 16 lanes -- j = 0, 1: corners 0-3, 4-7 of kCornerK;
+// j = 2, 3, 4: the z, x, y edge families of kEdgeK; j = 5, 6, 7: the y, x, z
+// face pairs of kFace (32 lanes) -- and kEnd ends the list.  The groups and
+// the centre are spread so the four waves reach the chunk's barrier about
+// together (measured per wave).
 constexpr int kQuad = 32, kEnd = 63;
 constexpr uint64_t lbl7(int a, int b, int c, int d, int e, int f, int g)
 {
@@ -537,8 +597,8 @@ constexpr uint64_t lbl7(int a, int b, int c, int d, int e, int f, int g)
            (uint64_t)f << 30 | (uint64_t)g << 36;
 }
 __constant__ const uint64_t kWaveLabels[4] = {
-    lbl7(kQuad + 0, kQuad + 2, kEnd, kEnd, kEnd, kEnd, kEnd), lbl7(kQuad + 1, kQuad + 3, kEnd, kEnd, kEnd, kEnd, kEnd),
-    lbl7(kQuad + 4, 4, 10, 12, kEnd, kEnd, kEnd), lbl7(13, 14, 16, 22, kEnd, kEnd, kEnd)};
+    lbl7(kQuad + 6, kQuad + 4, kEnd, kEnd, kEnd, kEnd, kEnd), lbl7(13, kQuad + 0, kEnd, kEnd, kEnd, kEnd, kEnd),
+    lbl7(kQuad + 5, kQuad + 1, kEnd, kEnd, kEnd, kEnd, kEnd), lbl7(kQuad + 2, kQuad + 3, kQuad + 7, kEnd, kEnd, kEnd, kEnd)};
 
 // The rows of a chunk of CH z-nodes are one contiguous span of the CSR values
 // (node after node, 3 rows each, neighbours ascending in y, x, z), so the
@@ -674,8 +734,8 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                 // come from the lanes holding those nodes.
                 const int grp = k - kQuad;
                 const int q = (t & 63) >> 4, n16 = t & 15;
-                const int j = grp < 2 ? 4 * grp + q : 4 * (grp - 2) + q;
-                const int kc = grp < 2 ? kCornerK[j] : kEdgeK[j];
+                const int j = grp < 2 ? 4 * grp + q : (grp < 5 ? 4 * (grp - 2) + q : 2 * (grp - 5) + (q & 1));
+                const int kc = grp < 2 ? kCornerK[j] : (grp < 5 ? kEdgeK[j] : kFaceK[j]);
                 const int oy = kc / 9 - 1, ox = (kc / 3) % 3 - 1, oz = kc % 3 - 1;
                 const int src = (m0 - z0) + n16;
                 // Every lane evaluates (its tile indices stay in range for
@@ -686,13 +746,13 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                                      i == 4 ? c.rxz : i == 5 ? c.ryz : i == 6 ? c.rx1 : i == 7 ? c.ry1 : c.rz1;
                     return cpx<R>(__shfl(a.re, src), __shfl(a.im, src));
                 };
-                const R wmN = __shfl(grp < 2 ? c.wm4o8 : c.wm3o12, src);
+                const R wmN = __shfl(grp < 2 ? c.wm4o8 : (grp < 5 ? c.wm3o12 : c.wm2o6), src);
                 const int modeN = __shfl(mode, src);
                 const int mN = m0 + n16, izN = mN + 1, tzN = mN - z0 + 1;
                 R ro = s_rho[3 * (ox + 1) + (oy + 1)][tzN + oz];
                 if (p.fst && izN == 1 && oz == -1) ro = R(0);
-                const bool ok = mN < m1 && iy + oy >= 1 && iy + oy <= ny && ix + ox >= 1 && ix + ox <= nx &&
-                                izN + oz >= 1 && izN + oz <= nz;
+                const bool ok = (grp < 5 || q < 2) && mN < m1 && iy + oy >= 1 && iy + oy <= ny && ix + ox >= 1 &&
+                                ix + ox <= nx && izN + oz >= 1 && izN + oz <= nz;
                 const int cznN = 1 + (izN > 1) + (izN < nz), rlN = 3 * cxy * cznN;
                 const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
                 const int rz = oz > -1 ? (izN > 1) + (oz > 0) : 0;
@@ -711,8 +771,14 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                     edge_block<R, 0>(kEdge[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
                 } else if (grp == 3) {
                     edge_block<R, 1>(kEdge[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
-                } else {
+                } else if (grp == 4) {
                     edge_block<R, 2>(kEdge[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                } else if (grp == 5) {
+                    face_block<R, 0>(kFace[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                } else if (grp == 6) {
+                    face_block<R, 1>(kFace[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
+                } else {
+                    face_block<R, 2>(kFace[j], s_cell[tzN - 1], s_cell[tzN], get, wmN, ro, emit);
                 }
                 continue;
             }
