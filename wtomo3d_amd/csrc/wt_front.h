@@ -28,9 +28,15 @@ int wt_row_scale(const int32_t* rowptr, const cpx<double>* vals, int64_t n, doub
 template <typename R>
 int wt_scatter_rows(const int32_t* rowptr, const cpx<double>* vals, const int64_t* dst, const double* scale,
                     int64_t n, cpx<R>* arena, hipStream_t s);
+// Contribution-block columns per extend-add block.  A launch covers a list of
+// children whose column blocks are numbered consecutively: col_start[i] is
+// the first block of child i (exclusive prefix sum of ceil(u / WT_EA_COLS)
+// over the list) and nblocks the total, so no block is launched empty.
+constexpr int WT_EA_COLS = 8;
 template <typename R>
-int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
-                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, int part, hipStream_t s);
+int wt_extend_add(const wt_dfront* fr, const int32_t* kids, const int32_t* col_start, int nkids, int64_t nblocks,
+                  const int32_t* relmap, const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, int part,
+                  hipStream_t s);
 template <typename R>
 int wt_diag_tiles(const wt_dfront* fr, const int32_t* list, int nlist, int64_t kb, cpx<R>* factor, cpx<R>* tiles,
                   int32_t* perm, int* status, hipStream_t s);

@@ -73,7 +73,6 @@ __global__ __launch_bounds__(256) void scatter_rows_kernel(const int32_t* __rest
     }
 }
 
-constexpr int EA_COLS = 8;  // CB columns per extend-add block
 
 // Adds the contribution blocks of the listed children into their parents.
 // At most one child per parent is listed, so the adds do not race.  The
@@ -85,14 +84,23 @@ constexpr int EA_COLS = 8;  // CB columns per extend-add block
 template <typename R>
 __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __restrict__ fr,
                                                          const int32_t* __restrict__ kids,
+                                                         const int32_t* __restrict__ col_start, int nkids,
                                                          const int32_t* __restrict__ relmap,
                                                          const cpx<R>* __restrict__ cb_child,
                                                          cpx<R>* __restrict__ factor,
                                                          cpx<R>* __restrict__ cb_parent, int part)
 {
-    const wt_dfront C = fr[kids[blockIdx.x]];
+    // child of this block: the last i with col_start[i] <= block
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = nkids - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (col_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const wt_dfront C = fr[kids[lo]];
     const int64_t uc = C.u;
-    const int64_t j0 = (int64_t)blockIdx.y * EA_COLS;
+    const int64_t j0 = (int64_t)(b - col_start[lo]) * WT_EA_COLS;
     if (j0 >= uc) return;
     const wt_dfront P = fr[C.parent];
     const int32_t* map = relmap + C.moff;
@@ -106,7 +114,7 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __rest
             else hi = mid;
         }
     }
-    const int64_t j1 = j0 + EA_COLS < uc ? j0 + EA_COLS : uc;
+    const int64_t j1 = j0 + WT_EA_COLS < uc ? j0 + WT_EA_COLS : uc;
     if (part == 2 && j1 <= q) return;
     const cpx<R>* cb = cb_child + C.CBoff;
     cpx<R>* Lp = factor + P.Loff;
@@ -822,14 +830,14 @@ int wt_scatter_rows(const int32_t* rowptr, const cpx<double>* vals, const int64_
 }
 
 template <typename R>
-int wt_extend_add(const wt_dfront* fr, const int32_t* kids, int nkids, int64_t max_u, const int32_t* relmap,
-                  const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, int part, hipStream_t s)
+int wt_extend_add(const wt_dfront* fr, const int32_t* kids, const int32_t* col_start, int nkids, int64_t nblocks,
+                  const int32_t* relmap, const cpx<R>* cb_child, cpx<R>* factor, cpx<R>* cb_parent, int part,
+                  hipStream_t s)
 {
-    if (nkids <= 0 || max_u <= 0) return WT_OK;
-    if (part < 0 || part > 2) return WT_ERR_ARG;
-    dim3 grid(nkids, (unsigned)((max_u + EA_COLS - 1) / EA_COLS));
-    hipLaunchKernelGGL(extend_add_kernel<R>, grid, dim3(256), 0, s, fr, kids, relmap, cb_child, factor, cb_parent,
-                       part);
+    if (nkids <= 0 || nblocks <= 0) return WT_OK;
+    if (part < 0 || part > 2 || nblocks > 0x7fffffff) return WT_ERR_ARG;
+    hipLaunchKernelGGL(extend_add_kernel<R>, dim3((unsigned)nblocks), dim3(256), 0, s, fr, kids, col_start, nkids,
+                       relmap, cb_child, factor, cb_parent, part);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }
@@ -932,8 +940,8 @@ template int wt_sp_solve<double>(int, const wt_dfront*, const int32_t*, int, con
 
 #define WT_INST(R)                                                                                              \
     template int wt_fill_zero<R>(cpx<R>*, int64_t, hipStream_t);                                                \
-    template int wt_extend_add<R>(const wt_dfront*, const int32_t*, int, int64_t, const int32_t*, const cpx<R>*, \
-                                  cpx<R>*, cpx<R>*, int, hipStream_t);                                          \
+    template int wt_extend_add<R>(const wt_dfront*, const int32_t*, const int32_t*, int, int64_t, const int32_t*, \
+                                  const cpx<R>*, cpx<R>*, cpx<R>*, int, hipStream_t);                           \
     template int wt_diag_tiles<R>(const wt_dfront*, const int32_t*, int, int64_t, cpx<R>*, cpx<R>*, int32_t*,   \
                                   int*, hipStream_t);                                                           \
     template int wt_solve_gather<R>(const wt_dfront*, const int32_t*, int, const int32_t*, const cpx<R>*,       \

@@ -216,10 +216,12 @@ static int upload_symbolic(wt_solver_impl& s)
         s.lev_kids_off[k].assign(nl, 0);
         s.lev_kids_cnt[k].assign(nl, 0);
         s.lev_kids_maxu[k].assign(nl, 0);
+        s.lev_kids_blocks[k].assign(nl, 0);
     }
     s.lev_upd_cnt.assign(nl, 0);
     s.lev_upd_off.assign(nl, 0);
     s.hlists.clear();
+    s.hcols.clear();
     for (int l = 0; l < nl; ++l) {
         std::vector<int32_t> v = S.levels[l];
         std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return S.fronts[a].p > S.fronts[b].p; });
@@ -230,12 +232,18 @@ static int upload_symbolic(wt_solver_impl& s)
         }
         for (int k = 0; k < 2; ++k) {
             s.lev_kids_off[k][l] = (int64_t)s.hlists.size();
+            s.hcols.resize(s.hlists.size(), 0);
+            int64_t nb = 0;
             for (int f : S.levels[l])
                 if (S.fronts[f].parent >= 0 && S.fronts[f].slot == k && S.fronts[f].u > 0) {
                     s.hlists.push_back(f);
+                    s.hcols.push_back((int32_t)nb);
+                    nb += (S.fronts[f].u + WT_EA_COLS - 1) / WT_EA_COLS;
                     s.lev_kids_cnt[k][l]++;
                     s.lev_kids_maxu[k][l] = std::max(s.lev_kids_maxu[k][l], S.fronts[f].u);
                 }
+            if (nb > 0x7fffffff) return WT_ERR_ARG;
+            s.lev_kids_blocks[k][l] = nb;
         }
         s.lev_upd_off[l] = (int64_t)s.hlists.size();
         for (int f : S.levels[l])
@@ -244,12 +252,15 @@ static int upload_symbolic(wt_solver_impl& s)
                 s.lev_upd_cnt[l]++;
             }
     }
+    s.hcols.resize(s.hlists.size(), 0);
     if ((rc = dalloc(&s.fronts, (int64_t)df.size())) || (rc = dalloc(&s.lists, (int64_t)s.hlists.size())) ||
+        (rc = dalloc(&s.cols, (int64_t)s.hcols.size())) ||
         (rc = dalloc(&s.dst, S.nnz)) || (rc = dalloc(&s.iperm, S.n)) || (rc = dalloc(&s.upd, (int64_t)S.upd_idx.size())) ||
         (rc = dalloc(&s.relmap, (int64_t)S.relmap.size())) || (rc = dalloc(&s.d_err, 1)))
         return rc;
     WT_HIP(hipMemcpy(s.fronts, df.data(), df.size() * sizeof(wt_dfront), hipMemcpyHostToDevice));
     WT_HIP(hipMemcpy(s.lists, s.hlists.data(), s.hlists.size() * 4, hipMemcpyHostToDevice));
+    WT_HIP(hipMemcpy(s.cols, s.hcols.data(), s.hcols.size() * 4, hipMemcpyHostToDevice));
     WT_HIP(hipMemcpy(s.dst, S.dst.data(), S.nnz * 8, hipMemcpyHostToDevice));
     WT_HIP(hipMemcpy(s.iperm, S.iperm.data(), S.n * 8, hipMemcpyHostToDevice));
     if (!S.upd_idx.empty())
@@ -268,7 +279,7 @@ void wt_solver_free(wt_solver_impl& s)
     s.f32.release();
     s.f64.release();
     dfree(s.rowptr); dfree(s.colidx); dfree(s.dst); dfree(s.iperm); dfree(s.upd); dfree(s.relmap);
-    dfree(s.fronts); dfree(s.lists); dfree(s.rscale); dfree(s.lam); dfree(s.mu); dfree(s.rho); dfree(s.vals); dfree(s.d_err);
+    dfree(s.fronts); dfree(s.lists); dfree(s.cols); dfree(s.rscale); dfree(s.lam); dfree(s.mu); dfree(s.rho); dfree(s.vals); dfree(s.d_err);
     dfree(s.xb); dfree(s.rb); dfree(s.bb); dfree(s.partial); dfree(s.norms);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
@@ -299,7 +310,7 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
         if (l + 1 >= nl) return;
         for (int k = 0; k < 2; ++k)
             if (s.lev_kids_cnt[k][l + 1] > 0) {
-                wt_step t{WT_STEP_EXTADD, s.lev_kids_cnt[k][l + 1], s.lev_kids_maxu[k][l + 1],
+                wt_step t{WT_STEP_EXTADD, s.lev_kids_cnt[k][l + 1], s.lev_kids_blocks[k][l + 1],
                           s.lev_kids_off[k][l + 1], l & 1, -1};
                 t.part = part;
                 num.fsteps.push_back(t);
@@ -433,8 +444,8 @@ static int enqueue_factor(wt_solver_impl& s, wt_numeric<R>& num, std::vector<hip
             break;
         case WT_STEP_EXTADD:
             rc = P.run(st, WT_PROF_EXTADD, 0, 0, [&] {
-                return wt_extend_add<R>(s.fronts, s.lists + t.list_off, t.count, t.a, s.relmap, num.cb[t.buf ^ 1],
-                                        num.factor, num.cb[t.buf], t.part, st);
+                return wt_extend_add<R>(s.fronts, s.lists + t.list_off, s.cols + t.list_off, t.count, t.a, s.relmap,
+                                        num.cb[t.buf ^ 1], num.factor, num.cb[t.buf], t.part, st);
             });
             break;
         case WT_STEP_DIAG:

@@ -209,12 +209,17 @@ struct wt_solver_impl {
     wt_dfront* fronts = nullptr;
     int32_t* lists = nullptr;
     std::vector<int32_t> hlists;
+    // beside every child-list entry of hlists: its first extend-add block
+    // (the list's exclusive prefix sum of ceil(u / WT_EA_COLS))
+    int32_t* cols = nullptr;
+    std::vector<int32_t> hcols;
     // per level: fronts sorted by p (desc), and children by slot
     std::vector<int64_t> lev_sorted_off;
     std::vector<std::vector<int64_t>> lev_sorted_p;
     std::vector<int64_t> lev_kids_off[2];
     std::vector<int32_t> lev_kids_cnt[2];
     std::vector<int64_t> lev_kids_maxu[2];
+    std::vector<int64_t> lev_kids_blocks[2];  // extend-add blocks of the list
     std::vector<int32_t> lev_upd_cnt;
     std::vector<int64_t> lev_upd_off;
     // model and matrix
