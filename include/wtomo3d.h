@@ -19,6 +19,19 @@
 extern "C" {
 #endif
 
+/* Return codes. */
+#define WT_OK 0
+#define WT_ERR_ARG 1          /* invalid argument or size */
+#define WT_ERR_HIP 2          /* a HIP runtime call failed */
+#define WT_ERR_STATE 3        /* call out of order (no model, values, factor ...) */
+#define WT_ERR_UNSUPPORTED 4  /* a configuration the library does not build */
+#define WT_ERR_NOMEM 5        /* device allocation failed */
+#define WT_ERR_PATTERN 6      /* CSR pattern is not the 27-point elastic grid's */
+#define WT_ERR_PIVOT 7        /* zero pivot, or a non-finite residual */
+#define WT_ERR_NOCONV 8       /* refinement stopped above tol (max_iter reached or the
+                               * residual stopped falling); the solution and the
+                               * statistics are still written */
+
 /* Grid / medium / boundary parameters of one assembly.  The reference keeps
  * them in module globals: MODEL_MODULE (nx, ny, nz, dx, omega, freesurf),
  * PML_MODULE (pmld, pmlf, ipml) and INIT_MODULE (usemin), modules.F90:1-76. */
@@ -89,6 +102,8 @@ typedef struct wt_solve_stats {
     double rel_residual_first;   /* the same after the first solve */
     double abs_residual;         /* max_j ||b_j - A x_j|| */
     double ms;                   /* wall time of the call */
+    int32_t converged;           /* 1: rel_residual <= tol at exit (always 1 for tol <= 0) */
+    int32_t stalled;             /* 1: stopped because the residual did not fall */
 } wt_solve_stats;
 
 /* Ordering + symbolic analysis (the work of STRUMPACK_reorder in job 1,
@@ -171,6 +186,10 @@ int wt_solver_release(wt_solver *h, int factor_prec);
  * unknown-major: all columns of one unknown contiguous).  With the
  * complex64 factor each block of <= 64 columns is refined with FP64
  * residuals until max_j ||r_j||/||b_j|| <= tol or max_iter solves.
+ * Refinement also stops when a step does not lower the residual.  A block
+ * that ends above tol gives WT_ERR_NOCONV after every block has been solved
+ * and X and *stats written (stats->converged = 0).  tol <= 0 asks for
+ * exactly max_iter solves and never reports WT_ERR_NOCONV.
  * Replaces STRUMPACK_solve (strumpack_finter.c:102) for a whole block. */
 int wt_solver_solve(wt_solver *h, int factor_prec, int nrhs, const void *b, void *x, int on_device,
                     double tol, int max_iter, wt_solve_stats *stats);
@@ -387,7 +406,9 @@ int wt_batch_apply(wt_batch *b, int nrhs, const void *x, void *y, int on_device)
  * are row-major (total unknowns) x nrhs, nrhs <= 64; model k uses its first
  * ncol[k] columns (the others are ignored on input and zero on output).
  * Refines until every model's columns reach |b - A x| / |b| <= tol or
- * max_iter solves; stats[nmodels] per model. */
+ * max_iter solves, or no model's residual still falls; stats[nmodels] per
+ * model (converged, stalled as for wt_solver_solve).  WT_ERR_NOCONV when
+ * some model ends above tol > 0: x and stats are written for every model. */
 int wt_batch_solve(wt_batch *b, int factor_prec, int nrhs, const int32_t *ncol, const void *rhs, void *x,
                    int on_device, double tol, int max_iter, wt_solve_stats *stats);
 

@@ -101,3 +101,115 @@ def cpu_rate_sample(fronts, dtype=np.complex64, budget_s=20.0, seed=0):
         if done_t >= 0.5 * budget_s:
             break
     return done_f, done_t, sample
+
+
+# ---------------------------------------------------------------------------
+# FP64 multifrontal direct solve (the checker of full-size solves)
+
+def nd_tree(nx, ny, nz, leaf=64):
+    """The tree of ``nd_fronts`` with node sets: list of fronts in post-order,
+    each a dict with ``piv`` (the grid nodes it eliminates: the separator
+    plane, or the whole leaf box), ``upd`` (the one-node halo of its box
+    outside it) and ``kids`` (indices of its children).  Node g = iz + nz ix
+    + nz nx iy (0-based), the reference's unknown order (srgraphs.f:659-665)."""
+    out = []
+
+    def nodes(x0, x1, y0, y1, z0, z1):
+        iy, ix, iz = np.meshgrid(np.arange(y0, y1), np.arange(x0, x1), np.arange(z0, z1), indexing="ij")
+        return (iz + nz * ix + nz * nx * iy).ravel()
+
+    def rec(b):
+        x0, x1, y0, y1, z0, z1 = b
+        dims = [x1 - x0, y1 - y0, z1 - z0]
+        vol = dims[0] * dims[1] * dims[2]
+        ax = 1
+        if dims[0] > dims[ax]:
+            ax = 0
+        if dims[2] > dims[ax]:
+            ax = 2
+        kids = []
+        if vol > leaf and dims[ax] >= 3:
+            s = b[2 * ax] + dims[ax] // 2
+            c0, c1 = list(b), list(b)
+            c0[2 * ax + 1] = s
+            c1[2 * ax] = s + 1
+            kids = [rec(tuple(c0)), rec(tuple(c1))]
+            sep = list(b)
+            sep[2 * ax], sep[2 * ax + 1] = s, s + 1
+            piv = nodes(*sep)
+        else:
+            piv = nodes(*b)
+        halo = nodes(max(x0 - 1, 0), min(x1 + 1, nx), max(y0 - 1, 0), min(y1 + 1, ny), max(z0 - 1, 0),
+                     min(z1 + 1, nz))
+        upd = np.setdiff1d(halo, nodes(*b))
+        out.append(dict(piv=np.sort(piv), upd=upd, kids=kids))
+        return len(out) - 1
+
+    rec((0, nx, 0, ny, 0, nz))
+    return out
+
+
+def _unk(nodes):
+    return (3 * np.asarray(nodes, np.int64)[:, None] + np.arange(3)).ravel()
+
+
+class MultifrontalLU:
+    """LU of a sparse matrix on the 27-point elastic grid by the multifrontal
+    method over ``nd_tree``: each front assembles the original entries of its
+    pivot rows and columns and its children's contribution blocks, factors
+    its pivot block with LAPACK's partial pivoting (getrf), forms the panels
+    L21, U12 and the contribution block F22 - L21 U12.  Complex128; the
+    checker of the GPU factorization at sizes where SuperLU is too slow."""
+
+    def __init__(self, A, nx, ny, nz, leaf=64):
+        A = A.tocsr()
+        self.n = A.shape[0]
+        tree = nd_tree(nx, ny, nz, leaf)
+        At = A.T.tocsr()  # columns of the pivot unknowns as rows
+        self.fronts = []
+        cbs = {}
+        for f in tree:
+            P, U = _unk(f["piv"]), _unk(f["upd"])
+            I = np.concatenate([P, U])
+            p, m = P.size, I.size
+            pos = np.full(self.n, -1, np.int64)
+            pos[I] = np.arange(m)
+            F = np.zeros((m, m), np.complex128)
+            rows = A[P][:, I].toarray()        # pivot rows (columns in the front)
+            F[:p, :] += rows
+            cols = At[P][:, U].toarray().T     # pivot columns, update rows
+            F[p:, :p] += cols
+            for k in f["kids"]:
+                ku, cb = cbs.pop(k)
+                ix = pos[ku]
+                F[np.ix_(ix, ix)] += cb
+            lu, piv = sla.lu_factor(F[:p, :p], check_finite=False)
+            F12 = F[:p, p:]
+            for i, j in enumerate(piv):
+                if i != j:
+                    F12[[i, j]] = F12[[j, i]]
+            U12 = sla.solve_triangular(lu, F12, lower=True, unit_diagonal=True, check_finite=False)
+            L21 = sla.solve_triangular(lu, F[p:, :p].T, trans="T", lower=False, check_finite=False).T
+            if U.size:
+                cbs[len(self.fronts)] = (U, F[p:, p:] - L21 @ U12)
+            self.fronts.append((P, U, lu, piv, L21, U12))
+
+    def solve(self, B):
+        B = np.asarray(B, np.complex128)
+        one = B.ndim == 1
+        Y = B.reshape(self.n, -1).copy()
+        for P, U, lu, piv, L21, U12 in self.fronts:
+            y = Y[P]
+            for i, j in enumerate(piv):
+                if i != j:
+                    y[[i, j]] = y[[j, i]]
+            y = sla.solve_triangular(lu, y, lower=True, unit_diagonal=True, check_finite=False)
+            Y[P] = y
+            if U.size:
+                Y[U] -= L21 @ y
+        for P, U, lu, piv, L21, U12 in reversed(self.fronts):
+            y = Y[P]
+            if U.size:
+                y = y - U12 @ Y[U]
+            Y[P] = sla.solve_triangular(lu, y, lower=False, check_finite=False)
+        return Y[:, 0] if one else Y

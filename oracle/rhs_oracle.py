@@ -35,3 +35,32 @@ def nodes_to_unknowns(ub, vb, wb):
     """Per-component grids (ny, nx, nz) (z fastest) -> unknown vector
     (u, v, w per node)."""
     return np.stack([np.ravel(ub), np.ravel(vb), np.ravel(wb)], axis=1).ravel()
+
+
+def planewave_expand(nx, ny, nz, dx, U, W, kh, phi):
+    """The 3-D background field of plane waves from their depth profiles (the
+    expansion the reference's missing planewvexp.f90 would do, defined by
+    this build; interface.f90:43-47):
+
+        u_j = (U_j(z) cos phi_j, U_j(z) sin phi_j, W_j(z))
+              * exp(-i kh_j (x cos phi_j + y sin phi_j)),
+
+    x = ix dx, y = iy dx (0-based nodes), phi_j the azimuth of horizontal
+    propagation from +x towards +y, U along it, W vertical (positive down),
+    the e^{+i omega t} convention of the profiles.  U, W: (nrhs, nz) complex;
+    kh, phi: (nrhs,).  Returns row-major (n, nrhs) complex128 in the unknown
+    order (u, v, w per node; z fastest, then x, then y)."""
+    U = np.atleast_2d(np.asarray(U, np.complex128))
+    W = np.atleast_2d(np.asarray(W, np.complex128))
+    kh = np.atleast_1d(np.asarray(kh, np.float64))
+    phi = np.atleast_1d(np.asarray(phi, np.float64))
+    c, s = np.cos(phi), np.sin(phi)
+    x = np.arange(nx) * dx
+    y = np.arange(ny) * dx
+    # phase[iy, ix, j]
+    ph = np.exp(-1j * kh[None, None, :] * (x[None, :, None] * c[None, None, :] + y[:, None, None] * s[None, None, :]))
+    out = np.empty((ny, nx, nz, 3, kh.size), np.complex128)
+    out[..., 0, :] = ph[:, :, None, :] * (U.T * c)[None, None, :, :]
+    out[..., 1, :] = ph[:, :, None, :] * (U.T * s)[None, None, :, :]
+    out[..., 2, :] = ph[:, :, None, :] * W.T[None, None, :, :]
+    return out.reshape(-1, kh.size)

@@ -1,6 +1,7 @@
 """GPU plane-wave right-hand sides through the C ABI: Thomson-Haskell
 profiles (wt_haskgrn) against the reference's golden profiles and the
-oracle, the 3-D expansion (wt_planewave_layered) against its formula, and
+oracle, the 3-D expansion (wt_planewave_layered) against the oracle's and as a
+discrete solution of the device operator, and
 the scattered-field source (wt_solver_scattered_source) against the
 reference's HETFLDSRC fixture and the oracle.
 
@@ -72,6 +73,8 @@ def test_haskgrn_many_sources_vs_oracle(prec):
 
 
 def test_layered_plane_wave_expansion():
+    """The device expansion against the oracle's (oracle/rhs_oracle.py,
+    itself pinned by the convergence test of test_planewave_consistency)."""
     nx, ny, nz, dx = 7, 6, 9, 250.0
     lay = media.layer_stack(nx, ny, nz, dx, 5)
     ain = np.deg2rad([12.0, 30.0, 44.0])
@@ -79,12 +82,35 @@ def test_layered_plane_wave_expansion():
     pr = rhs.profiles(lay, ain, [0, 1, 0], 2 * np.pi * 0.25, nz, dx, 0.0, 64)
     X = rhs.background_fields(nx, ny, nz, dx, pr, phi).to_host(np.complex128, (3 * nx * ny * nz, 3))
     U, W, kh = pr.host()
-    iy, ix, iz = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
-    for j in range(3):
-        e = np.exp(-1j * kh[j] * dx * (ix * np.cos(phi[j]) + iy * np.sin(phi[j]))).ravel()
-        u = U[j][iz.ravel()] * e
-        ref = np.stack([u * np.cos(phi[j]), u * np.sin(phi[j]), W[j][iz.ravel()] * e], axis=1).ravel()
-        assert np.abs(X[:, j] - ref).max() <= 1e-13 * np.abs(ref).max()
+    ref = RO.planewave_expand(nx, ny, nz, dx, U, W, kh, phi)
+    assert np.abs(X - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("phi_deg,is_s", [(0.0, 0), (135.0, 1)])
+def test_expansion_converges_on_the_device_operator(phi_deg, is_s):
+    """The device profiles, expansion and FP64 assembly together: the
+    homogeneous half-space's Haskell field is a second-order discrete
+    solution on the interior rows (the property the oracle's expansion is
+    pinned by), falling about 4x per halving of dx."""
+    import test_planewave_consistency as PC
+    r = []
+    for dx in PC.DXS:
+        nx, ny, nz = PC.grid(dx)
+        pr = rhs.profiles(PC.LAYERS, [np.deg2rad(20.0)], [is_s], PC.OMEGA, nz, dx, 0.0, 64)
+        s = Solver(nx, ny, nz)
+        sh = (ny, nx, nz)
+        s.set_model(*media.lame(np.full(sh, PC.VP), np.full(sh, PC.VS), np.full(sh, PC.RHO)))
+        s.assemble(grid_params(nx, ny, nz, dx, PC.OMEGA, pmld=2 * dx))
+        X = rhs.background_fields(nx, ny, nz, dx, pr, [np.deg2rad(phi_deg)]).to_host(np.complex128,
+                                                                                    (3 * nx * ny * nz, 1))
+        AX = s.apply(X)[:, 0]
+        s.close()
+        _, m = PC.operator(dx)
+        r.append(float(np.linalg.norm(AX[m]) / np.linalg.norm(PC.OMEGA ** 2 * PC.RHO * X[m, 0])))
+        print("dx %.0f interior residual %.3e" % (dx, r[-1]), flush=True)
+    assert r[-1] <= 1e-3
+    for a, b in zip(r, r[1:]):
+        assert 3.0 <= a / b <= 6.0, r
 
 
 def _golden_solver():

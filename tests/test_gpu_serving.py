@@ -47,3 +47,62 @@ def test_admission_budget():
     r = _requests()[:1]
     tiny.run(r, realtime=False)
     assert r[0].error is not None  # cannot fit: rejected, not deadlocked
+
+
+def _write_project(path, nx, ny, nz, dx, freqs, vp, vs, rho, extra=""):
+    from wtomo3d_amd import io
+    with open(path + ".ini", "w") as f:
+        f.write("[model]\nnx = %d\nny = %d\nnz = %d\ndx = %g\nnom = %d\n" % (nx, ny, nz, dx, len(freqs)))
+        for i, fr in enumerate(freqs):
+            f.write("freq_%d = %g\n" % (i + 1, fr))
+        f.write("[pml]\npmld = %g\n[sources]\nns = 1\n%s" % (3 * dx, extra))
+    io.write_model_h5(path + "_model.h5", vp, vs, rho)
+
+
+def test_on_disk_project_requests(tmp_path):
+    """Requests that name an on-disk project (ini + HDF5 model, the
+    reference's READINI_FORWARD / MODELIO_READ_MODEL inputs): the model, grid,
+    PML and frequencies come from the files, the background is the model's
+    lateral average.  The total field satisfies A u = A_bg u_bg (the
+    scattered-field identity); a layered project has no scattered field."""
+    from wtomo3d_amd import io, media, rhs
+    from wtomo3d_amd.solver import Solver
+    nx, ny, nz, dx = 14, 12, 16, 250.0
+    vp1, vs1, rho1 = media.layered(nx, ny, nz, dx, seed=21)
+    d = media.gaussian_field((ny, nx, nz), dx, 1000.0, 0.05, 3)
+    het = tmp_path / "het"
+    _write_project(str(het), nx, ny, nz, dx, [0.2, 0.35], vp1 * (1 + d), vs1 * (1 + d), rho1 * (1 + 0.3 * d))
+    lay = tmp_path / "lay"
+    full = [np.broadcast_to(a[None, None, :], (ny, nx, nz)) for a in (vp1, vs1, rho1)]
+    _write_project(str(lay), nx, ny, nz, dx, [0.3], *full)
+    reqs = serving.project_requests(str(het), 5, 100) + serving.project_requests(str(lay), 3, 200, start_id=2)
+    e = serving.Engine(hbm_budget=50e9, slots=2, tol=1e-9, return_solutions=True)
+    e.run(reqs, realtime=False)
+    for r in reqs:
+        assert r.error is None, r.error
+        assert r.stats["converged"] == 1 and r.stats["rel_residual"] <= 1e-9
+        prob = e._problem(r)
+        ain, phi, is_s = e.sources(r)
+        om = 2 * np.pi * r.freq
+        pr = rhs.profiles(prob.layers, ain, is_s, om, nz, dx, 0.0, 64)
+        n = 3 * nx * ny * nz
+        ub = rhs.background_fields(nx, ny, nz, dx, pr, phi).to_host(np.complex128, (n, r.nrhs))
+        with Solver(nx, ny, nz) as s, Solver(nx, ny, nz) as sb:
+            s.set_model(*prob.model)
+            s.assemble(prob.params(r.freq))
+            sb.set_model(*prob.bg)
+            sb.assemble(prob.params(r.freq))
+            lhs, rhs_ = s.apply(r.x), sb.apply(ub)
+            src = rhs_ - s.apply(ub)
+        if r.project == str(lay):
+            assert np.abs(src).max() == 0.0  # model == background: no source
+            assert np.abs(r.x - ub).max() <= 1e-12 * np.abs(ub).max()
+        else:
+            assert np.linalg.norm(src) > 1e-3 * np.linalg.norm(rhs_)
+            dev = np.linalg.norm(lhs - rhs_, axis=0) / np.linalg.norm(src, axis=0)
+            assert dev.max() <= 1e-8, dev
+    # the project's parameters reached the assembly
+    p = io.read_ini(str(het) + ".ini")
+    assert e._problem(reqs[0]).pmld == p["pmld"] == 750.0
+    e.close()
+    assert e.used == 0

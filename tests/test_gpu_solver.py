@@ -301,3 +301,29 @@ def test_wide_block_residual_matches_numpy(nrhs):
     x3, _ = s.solve(B[:, 3], prec=32, tol=0.0, max_iter=2)
     assert np.array_equal(x3, X[:, 3])
     s.close()
+
+
+def test_refinement_reports_non_convergence():
+    """Refinement that cannot reach its tolerance ends with WT_ERR_NOCONV
+    (NotConverged here) after delivering the solution it reached and its
+    statistics; tol <= 0 runs a fixed number of solves without it."""
+    from wtomo3d_amd._lib import NotConverged
+    s, A, vals = _setup(10, 9, 11, 64, seed=5)
+    B, Xt = rhs_block(A, 3, 51)
+    s.factor(32)
+    # below the FP64 floor: the residual stops falling before max_iter
+    with pytest.raises(NotConverged) as ei:
+        s.solve(B, prec=32, tol=1e-18, max_iter=40)
+    st = ei.value.stats
+    assert st["converged"] == 0 and (st["stalled"] == 1 or st["iterations"] == 40)
+    assert st["rel_residual"] <= 1e-12
+    assert SO.rel_residual(A, ei.value.x, B) <= 1e-12
+    # max_iter reached above tol
+    with pytest.raises(NotConverged) as ei:
+        s.solve(B, prec=32, tol=1e-12, max_iter=1)
+    assert ei.value.stats["iterations"] == 1 and ei.value.stats["stalled"] == 0
+    X, st = s.solve(B, prec=32, tol=0.0, max_iter=2)
+    assert st["converged"] == 1 and st["iterations"] == 2
+    X, st = s.solve(B, prec=32, tol=1e-10, max_iter=30)
+    assert st["converged"] == 1 and st["stalled"] == 0
+    s.close()

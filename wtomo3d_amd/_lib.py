@@ -11,6 +11,29 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+# return codes of include/wtomo3d.h
+WT_OK, WT_ERR_ARG, WT_ERR_HIP, WT_ERR_STATE, WT_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+WT_ERR_NOMEM, WT_ERR_PATTERN, WT_ERR_PIVOT, WT_ERR_NOCONV = 5, 6, 7, 8
+
+
+class WtError(RuntimeError):
+    """A library call returned a WT_ERR_* code (``.code``)."""
+
+    def __init__(self, what, code):
+        super().__init__("%s failed with code %d" % (what, code))
+        self.what, self.code = what, code
+
+
+class NotConverged(WtError):
+    """Refinement ended above its tolerance (WT_ERR_NOCONV).  ``stats`` (a
+    dict, or a list of dicts for a batch) and ``x`` (the solution reached,
+    when the caller's arrays were on the host) are attached."""
+
+    def __init__(self, what, stats=None, x=None):
+        super().__init__(what, WT_ERR_NOCONV)
+        self.stats, self.x = stats, x
+
+
 class WtGridParams(ctypes.Structure):
     """Mirror of ``wt_grid_params`` (include/wtomo3d.h)."""
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
@@ -23,7 +46,8 @@ class WtSolveStats(ctypes.Structure):
     """Mirror of ``wt_solve_stats``."""
     _fields_ = [("iterations", ctypes.c_int32), ("nrhs", ctypes.c_int32),
                 ("rel_residual", ctypes.c_double), ("rel_residual_first", ctypes.c_double),
-                ("abs_residual", ctypes.c_double), ("ms", ctypes.c_double)]
+                ("abs_residual", ctypes.c_double), ("ms", ctypes.c_double),
+                ("converged", ctypes.c_int32), ("stalled", ctypes.c_int32)]
 
 
 _LIB = None
@@ -158,8 +182,10 @@ def lib():
 
 
 def check(rc, what):
+    if rc == WT_ERR_NOCONV:
+        raise NotConverged(what)
     if rc != 0:
-        raise RuntimeError("%s failed with code %d" % (what, rc))
+        raise WtError(what, rc)
 
 
 def pml_params(dx, pmld=None, pmlr=1e-3, single=False):

@@ -13,7 +13,7 @@ import time
 
 import numpy as np
 
-from ._lib import lib, check, grid_params, WtGridParams, WtSolveStats
+from ._lib import lib, check, grid_params, WtGridParams, WtSolveStats, NotConverged, WT_ERR_NOCONV
 from . import media
 
 _vp = ctypes.c_void_p
@@ -119,10 +119,13 @@ class Batch:
         B, cols, nrhs = self._pack(bs)
         X = np.zeros_like(B)
         st = (WtSolveStats * self.m)()
-        check(lib().wt_batch_solve(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st),
-              "wt_batch_solve")
+        rc = lib().wt_batch_solve(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st)
         stats = [{k: getattr(s, k) for k, _ in WtSolveStats._fields_} for s in st]
-        return self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs]), stats
+        xs = self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs])
+        if rc == WT_ERR_NOCONV:
+            raise NotConverged("wt_batch_solve", stats, xs)
+        check(rc, "wt_batch_solve")
+        return xs, stats
 
     def run(self, bs, prec=32, tol=1e-10, max_iter=30):
         """Factor and solve group after group, releasing each group's factors
@@ -130,10 +133,13 @@ class Batch:
         B, cols, nrhs = self._pack(bs)
         X = np.zeros_like(B)
         st = (WtSolveStats * self.m)()
-        check(lib().wt_batch_run(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st),
-              "wt_batch_run")
+        rc = lib().wt_batch_run(self._h, prec, nrhs, _p(cols), _p(B), _p(X), 0, float(tol), int(max_iter), st)
         stats = [{k: getattr(s, k) for k, _ in WtSolveStats._fields_} for s in st]
-        return self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs]), stats
+        xs = self._unpack(X, cols, [np.asarray(b).ndim == 1 for b in bs])
+        if rc == WT_ERR_NOCONV:
+            raise NotConverged("wt_batch_run", stats, xs)
+        check(rc, "wt_batch_run")
+        return xs, stats
 
 
 def symbolic_info(dims, leaf_nodes=64):

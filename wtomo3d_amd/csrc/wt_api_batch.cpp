@@ -138,7 +138,8 @@ int refine_group(group& g, wt_numeric<R>& num, int nrhs, const int32_t* ncol, do
         if (rc) return rc;
         num.sgraph_nrhs = nrhs;
     }
-    std::vector<double> first(nm, INFINITY), last(nm, INFINITY), absr(nm, 0.0);
+    std::vector<double> first(nm, INFINITY), last(nm, INFINITY), absr(nm, 0.0), prev(nm, INFINITY);
+    std::vector<char> stall(nm, 0);
     int it = 0;
     bool bad = false;
     while (it < max_iter) {
@@ -148,7 +149,7 @@ int refine_group(group& g, wt_numeric<R>& num, int nrhs, const int32_t* ncol, do
         WT_HIP(hipStreamSynchronize(q));
         s.prof.flush();
         ++it;
-        bool done = true;
+        bool done = true, moving = false;
         for (int k = 0; k < nm; ++k) {
             double rel = 0.0, am = 0.0;
             for (int c = 0; c < ncol[k]; ++c) {
@@ -161,10 +162,18 @@ int refine_group(group& g, wt_numeric<R>& num, int nrhs, const int32_t* ncol, do
             last[k] = rel;
             absr[k] = am;
             if (!std::isfinite(rel)) bad = true;
-            if (!(rel <= tol)) done = false;
+            if (!(rel <= tol)) {
+                done = false;
+                // a model whose residual stopped falling is not refined on
+                // its own account; the loop ends when no model is moving
+                if (rel >= prev[k]) stall[k] = 1;
+                else moving = true;
+            }
+            prev[k] = rel;
         }
-        if (done || bad) break;
+        if (done || bad || (tol > 0 && !moving)) break;
     }
+    bool conv_all = true;
     for (int k = 0; k < nm; ++k) {
         wt_solve_stats& o = st[k];
         o.iterations = it;
@@ -173,8 +182,12 @@ int refine_group(group& g, wt_numeric<R>& num, int nrhs, const int32_t* ncol, do
         o.rel_residual_first = first[k];
         o.abs_residual = absr[k];
         o.ms = 0;
+        o.converged = (tol <= 0 || last[k] <= tol) ? 1 : 0;
+        o.stalled = (o.converged || !stall[k]) ? 0 : 1;
+        if (!o.converged) conv_all = false;
     }
-    return bad ? WT_ERR_PIVOT : WT_OK;
+    if (bad) return WT_ERR_PIVOT;
+    return conv_all ? WT_OK : WT_ERR_NOCONV;
 }
 
 }  // namespace
@@ -276,7 +289,7 @@ int run_groups(wt_batch* b, int nrhs, const int32_t* ncol, const void* rhs, void
             WT_HIP(hipMemcpyAsync(g->d_ncol, ncol + g->m0, g->nm() * 4, hipMemcpyHostToDevice, s.stream));
             rc = wt_seg_mask_cols(s.bb, g->d_seg, g->d_ncol, g->nm(), s.n, nrhs, s.stream);
             if (rc == WT_OK) rc = refine_group<R>(*g, num, nrhs, ncol + g->m0, tol, max_iter, stats + g->m0);
-            if (rc == WT_OK || rc == WT_ERR_PIVOT) {
+            if (rc == WT_OK || rc == WT_ERR_PIVOT || rc == WT_ERR_NOCONV) {
                 WT_HIP(hipMemcpyAsync((cpx<double>*)x + row0 * nrhs, s.xb, s.n * nrhs * 16, outk, s.stream));
                 WT_HIP(hipStreamSynchronize(s.stream));
             }
@@ -284,7 +297,7 @@ int run_groups(wt_batch* b, int nrhs, const int32_t* ncol, const void* rhs, void
         // the pool passes to the next group
         WT_HIP(hipStreamSynchronize(s.stream));
         num.release();
-        if (rc == WT_ERR_PIVOT) result = rc;
+        if (rc == WT_ERR_PIVOT || (rc == WT_ERR_NOCONV && result != WT_ERR_PIVOT)) result = rc;
         else if (rc) return rc;
     }
     return result;
@@ -586,7 +599,7 @@ int wt_batch_solve(wt_batch* b, int factor_prec, int nrhs, const int32_t* ncol, 
         if ((rc = wt_seg_mask_cols(s.bb, g->d_seg, g->d_ncol, g->nm(), s.n, nrhs, s.stream))) return rc;
         rc = f32 ? refine_group<float>(*g, s.f32, nrhs, ncol + g->m0, tol, max_iter, stats + g->m0)
                  : refine_group<double>(*g, s.f64, nrhs, ncol + g->m0, tol, max_iter, stats + g->m0);
-        if (rc == WT_ERR_PIVOT) result = rc;
+        if (rc == WT_ERR_PIVOT || (rc == WT_ERR_NOCONV && result != WT_ERR_PIVOT)) result = rc;
         else if (rc) return rc;
         WT_HIP(hipMemcpyAsync((cpx<double>*)x + row0 * nrhs, s.xb, s.n * nrhs * 16, outk, s.stream));
         WT_HIP(hipStreamSynchronize(s.stream));

@@ -66,7 +66,10 @@ int ensure_ir(wt_solver_impl& s, int nrhs)
 
 // Refinement on a row-major block of <= 64 columns already on the device:
 // x = 0, r = b; repeat { d = A_f^{-1} r (factor precision); x += d;
-// r = b - A x (fp64) } until max_j |r_j| / |b_j| <= tol or max_iter solves.
+// r = b - A x (fp64) } until max_j |r_j| / |b_j| <= tol or max_iter solves,
+// or a step that does not lower max_j |r_j| / |b_j| (tol > 0 only).
+// Returns WT_ERR_NOCONV when it ends above tol > 0 (x and *st are written),
+// WT_ERR_PIVOT for a non-finite residual.
 template <typename R>
 int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b, cpx<double>* x, double tol,
            int max_iter, wt_solve_stats* st)
@@ -82,7 +85,8 @@ int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b
     const cpx<double>* r = b;
     std::vector<double> h(2 * nrhs);
     int it = 0;
-    double rel = INFINITY, first = INFINITY;
+    double rel = INFINITY, first = INFINITY, prev = INFINITY;
+    bool stalled = false;
     while (it < max_iter) {
         wt_prof& P = s.prof;
         const double cs = sizeof(cpx<R>), blk = (double)n * nrhs;
@@ -113,15 +117,24 @@ int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b
         if (it == 1) first = rel;
         r = s.rb;
         if (st) st->abs_residual = absmax;
-        if (rel <= tol) break;
+        if (rel <= tol || !std::isfinite(rel)) break;
+        if (tol > 0 && rel >= prev) {
+            stalled = true;
+            break;
+        }
+        prev = rel;
     }
+    const bool conv = tol <= 0 || rel <= tol;
     if (st) {
         st->iterations = it;
         st->nrhs = nrhs;
         st->rel_residual = rel;
         st->rel_residual_first = first;
+        st->converged = conv ? 1 : 0;
+        st->stalled = stalled ? 1 : 0;
     }
-    return std::isfinite(rel) ? WT_OK : WT_ERR_PIVOT;
+    if (!std::isfinite(rel)) return WT_ERR_PIVOT;
+    return conv ? WT_OK : WT_ERR_NOCONV;
 }
 
 }  // namespace
@@ -570,6 +583,8 @@ int wt_solver_solve(wt_solver* h, int factor_prec, int nrhs, const void* b, void
     wt_solve_stats agg{};
     agg.rel_residual = 0;
     agg.rel_residual_first = 0;
+    agg.converged = 1;
+    int result = WT_OK;
     auto t0 = std::chrono::steady_clock::now();
     for (int c0 = 0; c0 < nrhs; c0 += chunk) {
         const int nc = nrhs - c0 < chunk ? nrhs - c0 : chunk;
@@ -591,7 +606,10 @@ int wt_solver_solve(wt_solver* h, int factor_prec, int nrhs, const void* b, void
         wt_solve_stats cs{};
         rc = f32 ? refine<float>(s, s.f32, nc, bblk, xblk, tol, max_iter, &cs)
                  : refine<double>(s, s.f64, nc, bblk, xblk, tol, max_iter, &cs);
-        if (rc) return rc;
+        if (rc == WT_ERR_NOCONV) result = rc;  // keep going: x and stats are still delivered
+        else if (rc) return rc;
+        if (!cs.converged) agg.converged = 0;
+        if (cs.stalled) agg.stalled = 1;
         if (xblk == s.xb) {
             hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
             if (nc == nrhs) WT_HIP(hipMemcpyAsync(dstx, s.xb, s.n * nc * 16, k, s.stream));
@@ -607,7 +625,7 @@ int wt_solver_solve(wt_solver* h, int factor_prec, int nrhs, const void* b, void
     agg.nrhs = nrhs;
     agg.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (st) *st = agg;
-    return WT_OK;
+    return result;
 }
 
 // ---------------------------------------------------------------------------
@@ -686,10 +704,12 @@ void strumpack_serial_finter(int* job, int* nrows, int* nnz, int* irptr, int* jc
         for (int64_t i = 0; i < n; ++i) b[i] = cpx<double>(r32[i].re, r32[i].im);
         wt_solve_stats st{};
         int rc = wt_solver_solve(g_legacy, WT_PREC_F32, 1, b.data(), x.data(), 0, 1e-4, 10, &st);
-        if (rc != WT_OK || (st.rel_residual > 1e-4 && st.abs_residual > 1e-10)) {
+        // converged when either the relative or the absolute criterion holds
+        // (rctol, actol); a block above both is a failed solve
+        if ((rc != WT_OK && rc != WT_ERR_NOCONV) || (st.rel_residual > 1e-4 && st.abs_residual > 1e-10)) {
             printf("%s: Error in solution phase1\n", fcnm);
             *ierr = 1;
-            if (rc != WT_OK) return;
+            if (rc != WT_OK && rc != WT_ERR_NOCONV) return;
         }
         cpx<float>* s32 = (cpx<float>*)sol;
         for (int64_t i = 0; i < n; ++i) s32[i] = cpx<float>((float)x[i].re, (float)x[i].im);

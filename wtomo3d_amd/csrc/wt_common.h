@@ -4,14 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 
-#define WT_OK 0
-#define WT_ERR_ARG 1
-#define WT_ERR_HIP 2
-#define WT_ERR_STATE 3
-#define WT_ERR_UNSUPPORTED 4
-#define WT_ERR_NOMEM 5
-#define WT_ERR_PATTERN 6
-#define WT_ERR_PIVOT 7
+#include "wtomo3d.h"  // return codes WT_OK, WT_ERR_*
 
 #define WT_HIP(call)                                                              \
     do {                                                                          \
@@ -67,4 +60,3 @@ __host__ __device__ inline cpx<R> cdiv(cpx<R> a, cpx<R> b)
     return cpx<R>((a.re * t + a.im) / den, (a.im * t - a.re) / den);
 }
 
-#include "wtomo3d.h"

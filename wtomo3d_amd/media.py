@@ -32,6 +32,40 @@ def layer_stack(nx, ny, nz, dx, seed, nlayers=6):
     return dict(top=top, vp=vps, vs=vs, rho=310.0 * vps ** 0.25, vpd=vps.copy(), vsd=vs.copy())
 
 
+def background_1d(vp, vs, rho, dx, max_layers=64):
+    """Layered background of a 3-D model (arrays (ny, nx, nz), z fastest):
+    the lateral mean of vp, vs and rho at each depth, with runs of equal
+    consecutive depths merged into layers.  When that leaves more than
+    ``max_layers`` layers (the profile generator's limit), the depths are cut
+    into ``max_layers`` runs of near-equal length and each run takes its mean.
+    Layer tops lie half a cell above a layer's first node (the first at 0), so
+    no node sits on an interface.  A one-layer background gets a second,
+    identical layer below the grid (the profiles need a layer above the
+    half-space).  Returns (vp1, vs1, rho1) per depth -- piecewise constant,
+    the values of the layer holding each node -- and the layer stack in the
+    form of ``layer_stack``."""
+    def mean(v, axis=None):
+        # a run of equal values keeps its value exactly
+        v = np.asarray(v, np.float64)
+        return np.where(v.max(axis=axis) == v.min(axis=axis), v.max(axis=axis), v.mean(axis=axis))
+
+    m = [mean(np.reshape(a, (-1, np.shape(a)[-1])), axis=0) for a in (vp, vs, rho)]
+    nz = m[0].size
+    starts = [0] + [iz for iz in range(1, nz) if any(a[iz] != a[iz - 1] for a in m)]
+    if len(starts) > max_layers:
+        starts = sorted(set(int(v) for v in np.linspace(0, nz, max_layers, endpoint=False)))
+    bounds = starts + [nz]
+    per_layer = [np.array([mean(a[bounds[k]:bounds[k + 1]]) for k in range(len(starts))]) for a in m]
+    per_depth = [np.repeat(v, np.diff(bounds)) for v in per_layer]
+    top = np.array([0.0] + [(s - 0.5) * dx for s in starts[1:]])
+    if len(starts) == 1:
+        per_layer = [np.repeat(v, 2) for v in per_layer]
+        top = np.array([0.0, nz * dx])
+    vpl, vsl, rhol = per_layer
+    layers = dict(top=top, vp=vpl, vs=vsl, rho=rhol, vpd=vpl.copy(), vsd=vsl.copy())
+    return per_depth[0], per_depth[1], per_depth[2], layers
+
+
 def gaussian_field(shape, dx, corr_len, std, seed):
     """Zero-mean Gaussian random field with Gaussian covariance of the given
     correlation length, scaled to the requested standard deviation

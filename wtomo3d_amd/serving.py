@@ -9,14 +9,32 @@ idle factors are evicted least-recently-used when memory is needed.  The
 reference has no serving layer (its driver factors one frequency at a time,
 elwave.f90:147-182); this is the request-level wrapper around jobs 2 and 3.
 
+Factor cache.  A (model, frequency) key has at most one entry.  The entry is
+created when its first request is admitted, before the factorization starts,
+and its bytes are reserved then; a later request for the same key waits in
+the queue until the entry is free and then reuses its factor (the
+reference's one factorization per (model, omega), elwave.f90:171-178).  An
+entry leaves the cache exactly once -- evicted while idle, dropped after a
+failed request, dropped after its request when factors are not kept, or at
+``close()`` -- and its bytes are released then, so ``used`` always equals
+the bytes of the entries in the cache.
+
 Each request solves the reference's forward problem for plane-wave sources
-on its model: a heterogeneous model (seeded by the model key) over a layered
-background; Thomson-Haskell profiles of the background (HASKGRN) expanded to
-3-D background fields; the scattered-field source (A_bg - A) u_bg
-(HETFLDSRC); the FP32-factor / FP64-refined solve for the scattered field;
-and the total field u_h + u_bg (MAKEUH).  Sources: P (70 %) or S plane
-waves, incidence 10-35 degrees, azimuth 0-360 degrees, from the request's
-seed.
+on its model: a heterogeneous model over a layered background; Thomson-
+Haskell profiles of the background (HASKGRN) expanded to 3-D background
+fields; the scattered-field source (A_bg - A) u_bg (HETFLDSRC); the
+FP32-factor / FP64-refined solve for the scattered field; and the total field
+u_h + u_bg (MAKEUH).  Sources: P (70 %) or S plane waves, incidence 10-35
+degrees, azimuth 0-360 degrees, from the request's seed.  A request whose
+refinement does not reach the engine's tolerance is reported as an error.
+
+Models are either synthetic (seeded by the request's model key: layered
+background plus a 5 % Gaussian random field) or an on-disk project, the
+reference's ``name.ini`` + ``name_model.h5`` pair (READINI_FORWARD
+readini.f90:34-222, MODELIO_READ_MODEL modelio.f90:45-113): its grid,
+spacing, PML, free-surface and anti-alias settings come from the ini file,
+the speeds and density from the HDF5 file, and its layered background is the
+model's lateral average (``media.background_1d``).
 """
 import collections
 import ctypes
@@ -26,24 +44,61 @@ import time
 import numpy as np
 
 from . import media, rhs
-from ._lib import lib, check, WtSolveStats, grid_params
+from ._lib import lib, check, WtSolveStats, grid_params, NotConverged, WT_ERR_NOCONV
 from .solver import Solver
 
 
 class Request:
-    def __init__(self, rid, model_key, dims, dx, freq, nrhs, src_seed, arrival=0.0):
-        self.rid, self.model_key, self.dims, self.dx = rid, model_key, tuple(dims), dx
+    """One solve request.  ``project`` (a path prefix naming ``name.ini`` and
+    ``name_model.h5``) replaces the synthetic model of ``model_key``; its
+    grid and spacing are then read from the project when it is submitted."""
+
+    def __init__(self, rid, model_key, dims, dx, freq, nrhs, src_seed, arrival=0.0, project=None):
+        self.rid, self.model_key, self.dx = rid, model_key, dx
+        self.dims = tuple(dims) if dims is not None else None
         self.freq, self.nrhs, self.src_seed, self.arrival = freq, nrhs, src_seed, arrival
+        self.project = project
         self.start = self.end = None
+        self.arrival_abs = None
         self.stats = None
         self.reused = False
         self.x = None
         self.error = None
 
+    def key(self):
+        """The factor-cache key: one factorization per (model, frequency)."""
+        if self.project is not None:
+            return ("project", self.project, float(self.freq))
+        return ("model", self.model_key, self.dims, float(self.dx), float(self.freq))
+
 
 class _Entry:
-    def __init__(self, solver, nbytes):
-        self.solver, self.nbytes, self.busy, self.last = solver, nbytes, False, time.time()
+    """One cached factor: a solver handle and the bytes reserved for it."""
+
+    def __init__(self, nbytes):
+        self.solver = None
+        self.nbytes = nbytes
+        self.busy = True     # a request owns it (factoring or solving)
+        self.ready = False   # factored
+        self.last = time.time()
+
+
+class _Problem:
+    """What a request needs besides its sources: the model's (lam, mu, rho),
+    the layer stack and (lam, mu, rho) of its background, and the assembly
+    parameters at a frequency."""
+
+    def __init__(self, dims, dx, model, layers, bg, pmld, pmlr=1e-3, fst=True, usemin=True, tau=999.999):
+        self.dims, self.dx, self.model, self.layers, self.bg = dims, dx, model, layers, bg
+        self.pmld, self.pmlr, self.fst, self.usemin, self.tau = pmld, pmlr, fst, usemin, tau
+
+    def params(self, freq):
+        om = complex(2 * np.pi * freq, 0.0)
+        if self.tau != 999.999:  # anti-alias damping, elwave.f90:164
+            om += 1j / self.tau
+        nx, ny, nz = self.dims
+        return grid_params(nx, ny, nz, self.dx, om, pmld=self.pmld, pmlr=self.pmlr, fst=self.fst,
+                           usemin=self.usemin)
 
 
 class Engine:
@@ -52,28 +107,43 @@ class Engine:
         self.budget, self.slots, self.tol, self.max_iter = hbm_budget, slots, tol, max_iter
         self.keep, self.return_x, self.lookahead = keep_factors, return_solutions, lookahead
         self.used = 0
-        self.cache = collections.OrderedDict()  # (model_key, freq) -> _Entry
+        self.cache = collections.OrderedDict()  # Request.key() -> _Entry
         self.pending = collections.deque()
         self.cv = threading.Condition()
         self.closed = False
         self.done = []
-        self._models = {}
-        self._mlock = threading.Lock()
+        self.factorizations = 0
+        self._problems = {}
+        self._plock = threading.Lock()
         self._est = {}
 
-    # -- model / memory helpers --------------------------------------------
-    def _model(self, key, dims, dx):
-        """(lam, mu, rho) of the heterogeneous model, its layer stack and the
-        (lam, mu, rho) of its layered background."""
-        k = (key, dims, dx)
-        with self._mlock:
-            if k not in self._models:
-                nx, ny, nz = dims
-                vp, vs, rho = media.heterogeneous(nx, ny, nz, dx, seed=int(key))
-                vp1, vs1, rho1 = media.layered(nx, ny, nz, dx, seed=int(key))
-                bg = media.lame(*(np.broadcast_to(a[None, None, :], (ny, nx, nz)) for a in (vp1, vs1, rho1)))
-                self._models[k] = (media.lame(vp, vs, rho), media.layer_stack(nx, ny, nz, dx, seed=int(key)), bg)
-            return self._models[k]
+    # -- models ----------------------------------------------------------------
+    def _problem(self, r):
+        k = ("project", r.project) if r.project is not None else ("model", r.model_key, r.dims, r.dx)
+        with self._plock:
+            if k not in self._problems:
+                self._problems[k] = self._load_project(r.project) if r.project is not None else \
+                    self._synthetic(r.model_key, r.dims, r.dx)
+            return self._problems[k]
+
+    @staticmethod
+    def _synthetic(key, dims, dx):
+        nx, ny, nz = dims
+        vp, vs, rho = media.heterogeneous(nx, ny, nz, dx, seed=int(key))
+        vp1, vs1, rho1 = media.layered(nx, ny, nz, dx, seed=int(key))
+        bg = media.lame(*(np.broadcast_to(a[None, None, :], (ny, nx, nz)) for a in (vp1, vs1, rho1)))
+        return _Problem(dims, dx, media.lame(vp, vs, rho), media.layer_stack(nx, ny, nz, dx, seed=int(key)), bg,
+                        pmld=10 * dx)
+
+    @staticmethod
+    def _load_project(name):
+        from . import io
+        p, vp, vs, rho = io.load_project(name)
+        nx, ny, nz, dx = p["nx"], p["ny"], p["nz"], p["dx"]
+        vp1, vs1, rho1, layers = media.background_1d(vp, vs, rho, dx)
+        bg = media.lame(*(np.broadcast_to(a[None, None, :], (ny, nx, nz)) for a in (vp1, vs1, rho1)))
+        return _Problem((nx, ny, nz), dx, media.lame(vp, vs, rho), layers, bg, pmld=p["pmld"], pmlr=p["pmlr"],
+                        fst=p["freesurf"][0], usemin=p["usemin"], tau=p["tau"])
 
     def _estimate(self, dims):
         """Solver bytes plus the scattered-source scratch (the background
@@ -93,25 +163,103 @@ class Engine:
         is_s = (rng.random(r.nrhs) < 0.3).astype(np.int32)
         return ain, phi, is_s
 
+    # -- GPU work (the hooks a test can replace) --------------------------------
+    def _factor(self, r, prob):
+        """A solver handle holding the complex64 factor of the request's
+        operator (jobs 1 and 2)."""
+        s = Solver(*prob.dims)
+        try:
+            lam, mu, rho = prob.model
+            s.set_model(lam, mu, rho)
+            s.assemble(prob.params(r.freq))
+            s.factor(32)
+        except Exception:
+            s.close()
+            raise
+        return s
+
+    def _solve(self, r, s, prob, slot):
+        """Sources, scattered-field source, refined solve and total field of
+        one request on the factored handle ``s``; fills r.stats (and r.x)."""
+        L = lib()
+        n = s.n
+        nx, ny, nz = prob.dims
+        st_ = s.stream()
+        x, sf, ub, pu, pw, pk = slot(n * r.nrhs * 16, nz * r.nrhs * 16, r.nrhs * 8)
+        omega = 2 * np.pi * r.freq
+        ain, phi, is_s = self.sources(r)
+        prof = rhs.profiles(prob.layers, ain, is_s, omega, nz, prob.dx, 0.0, rhs.WT_PREC_F64, stream=st_,
+                            out=(pu, pw, pk))
+        rhs.background_fields(nx, ny, nz, prob.dx, prof, phi, stream=st_, out=ub)
+        s.scattered_source(prob.bg[0], prob.bg[1], prob.bg[2], ub, out=sf, nrhs=r.nrhs)
+        st = WtSolveStats()
+        rc = L.wt_solver_solve(s._h, 32, r.nrhs, sf.ptr, x.ptr, 1, self.tol, self.max_iter, ctypes.byref(st))
+        r.stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
+        if rc == WT_ERR_NOCONV:
+            raise NotConverged("request %d: refinement stopped at %.3e > %.1e after %d solves"
+                               % (r.rid, st.rel_residual, self.tol, st.iterations), r.stats)
+        check(rc, "wt_solver_solve")
+        rhs.total_field(x, ub, n * r.nrhs, stream=st_)
+        if self.return_x:
+            r.x = x.to_host(np.complex128, (n, r.nrhs))
+
+    def _slot(self):
+        """Per-slot device buffers that only grow: solution, source and
+        background blocks, then the two profiles and the wavenumbers.
+        Returns (get(block, prof, kh) -> buffers, release())."""
+        held = [None] * 6
+
+        def get(block, prof, kh):
+            sizes = (block, block, block, prof, prof, kh)
+            for i, nb in enumerate(sizes):
+                if held[i] is None or held[i].nbytes < nb:
+                    if held[i] is not None:
+                        held[i].free()
+                    held[i] = rhs.DeviceBuffer(nb)
+            return held
+
+        def release():
+            for b in held:
+                if b is not None:
+                    b.free()
+        return get, release
+
     # -- scheduling ----------------------------------------------------------
     def submit(self, req):
+        if req.project is not None:
+            prob = self._problem(req)
+            req.dims, req.dx = prob.dims, prob.dx
         with self.cv:
             self.pending.append(req)
             self.cv.notify_all()
 
+    def _drop(self, key, e):
+        """Remove an entry from the cache and release its bytes (once; the
+        caller holds the lock and the entry is not in use by anyone else)."""
+        if self.cache.get(key) is e:
+            del self.cache[key]
+        if e.solver is not None:
+            e.solver.close()
+            e.solver = None
+        self.used -= e.nbytes
+        e.nbytes = 0
+
     def _admit(self):
-        """Pick a pending request that can run now (caller holds the lock)."""
-        for idx in range(min(self.lookahead, len(self.pending))):
+        """Pick a pending request that can run now and the cache entry it
+        runs on (caller holds the lock)."""
+        idx = 0
+        while idx < min(self.lookahead, len(self.pending)):
             r = self.pending[idx]
-            key = (r.model_key, r.dims, r.freq)
+            key = r.key()
             e = self.cache.get(key)
             if e is not None:
-                if e.busy:
+                if e.busy:  # its factor is being built or used: wait for it
+                    idx += 1
                     continue
                 e.busy = True
                 self.cache.move_to_end(key)
                 del self.pending[idx]
-                return r, e, 0
+                return r, e
             need = self._estimate(r.dims)
             if need > self.budget:
                 # can never fit: fail it instead of waiting forever
@@ -120,26 +268,26 @@ class Engine:
                 r.start = r.end = time.perf_counter()
                 self.done.append(r)
                 self.cv.notify_all()
-                return self._admit()
+                continue
             while self.used + need > self.budget:
                 victim = next((k for k, v in self.cache.items() if not v.busy), None)
                 if victim is None:
                     break
-                v = self.cache.pop(victim)
-                v.solver.close()
-                self.used -= v.nbytes
+                self._drop(victim, self.cache[victim])
             if self.used + need <= self.budget:
+                e = _Entry(need)
+                self.cache[key] = e
                 self.used += need
                 del self.pending[idx]
-                return r, None, need
-        return None, None, 0
+                return r, e
+            idx += 1
+        return None, None
 
-    def _worker(self, slot_bufs):
-        L = lib()
+    def _worker(self, slot):
         while True:
             with self.cv:
                 while True:
-                    r, e, need = self._admit()
+                    r, e = self._admit()
                     if r is not None or (self.closed and not self.pending):
                         break
                     self.cv.wait()
@@ -147,79 +295,33 @@ class Engine:
                     return
             r.start = time.perf_counter()
             try:
-                (lam, mu, rho), layers, bg = self._model(r.model_key, r.dims, r.dx)
-                if e is None:
-                    s = Solver(*r.dims)
-                    s.set_model(lam, mu, rho)
-                    s.assemble(grid_params(r.dims[0], r.dims[1], r.dims[2], r.dx, 2 * np.pi * r.freq,
-                                           pmld=10 * r.dx))
-                    s.factor(32)
-                    e = _Entry(s, need)
-                    e.busy = True
+                prob = self._problem(r)
+                if not e.ready:
+                    e.solver = self._factor(r, prob)
+                    e.ready = True
+                    self.factorizations += 1
                 else:
                     r.reused = True
-                s = e.solver
-                n = s.n
-                st_ = s.stream()
-                x, sf, ub, pu, pw, pk = slot_bufs(n * r.nrhs * 16, r.dims[2] * r.nrhs * 16, r.nrhs * 8)
-                omega = 2 * np.pi * r.freq
-                ain, phi, is_s = self.sources(r)
-                prof = rhs.profiles(layers, ain, is_s, omega, r.dims[2], r.dx, 0.0, rhs.WT_PREC_F64, stream=st_,
-                                    out=(pu, pw, pk))
-                rhs.background_fields(r.dims[0], r.dims[1], r.dims[2], r.dx, prof, phi, stream=st_, out=ub)
-                s.scattered_source(bg[0], bg[1], bg[2], ub, out=sf, nrhs=r.nrhs)
-                st = WtSolveStats()
-                check(L.wt_solver_solve(s._h, 32, r.nrhs, sf.ptr, x.ptr, 1, self.tol, self.max_iter,
-                                        ctypes.byref(st)), "solve")
-                rhs.total_field(x, ub, n * r.nrhs, stream=st_)
-                r.stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
-                if self.return_x:
-                    r.x = x.to_host(np.complex128, (n, r.nrhs))
+                self._solve(r, e.solver, prob, slot)
             except Exception as ex:  # reported per request
                 r.error = str(ex)
             r.end = time.perf_counter()
             with self.cv:
-                key = (r.model_key, r.dims, r.freq)
-                if e is not None:
-                    e.busy = False
-                    e.last = time.time()
-                    if self.keep and r.error is None:
-                        self.cache[key] = e
-                        self.cache.move_to_end(key)
-                    else:
-                        e.solver.close()
-                        self.used -= e.nbytes
-                elif need:
-                    self.used -= need
+                e.busy = False
+                e.last = time.time()
+                if r.error is not None or not self.keep:
+                    # a failed request's factor is not trusted by later ones
+                    self._drop(r.key(), e)
                 self.done.append(r)
                 self.cv.notify_all()
 
     def run(self, requests, realtime=True):
         """Feed requests at their arrival times (or all at once) and wait."""
-        L = lib()
-        bufs = []
-
-        def make_slot():
-            # per-slot device buffers that only grow: solution, source and
-            # background blocks, then the two profiles and the wavenumbers
-            held = [None] * 6
-
-            def get(block, prof, kh):
-                sizes = (block, block, block, prof, prof, kh)
-                for i, nb in enumerate(sizes):
-                    if held[i] is None or held[i].nbytes < nb:
-                        if held[i] is not None:
-                            held[i].free()
-                        held[i] = rhs.DeviceBuffer(nb)
-                return held
-            bufs.append(held)
-            return get
-
-        threads = [threading.Thread(target=self._worker, args=(make_slot(),), daemon=True)
-                   for _ in range(self.slots)]
+        slots = [self._slot() for _ in range(self.slots)]
+        threads = [threading.Thread(target=self._worker, args=(get,), daemon=True) for get, _ in slots]
+        with self.cv:
+            self.closed = False
         t0 = time.perf_counter()
-        for r in requests:
-            r.arrival = r.arrival  # relative seconds
         for t in threads:
             t.start()
         for r in sorted(requests, key=lambda q: q.arrival):
@@ -228,24 +330,33 @@ class Engine:
                 if dt > 0:
                     time.sleep(dt)
             r.arrival_abs = t0 + r.arrival if realtime else t0
-            self.submit(r)
+            try:
+                self.submit(r)
+            except Exception as ex:  # e.g. an unreadable project
+                r.error = str(ex)
+                r.start = r.end = time.perf_counter()
+                with self.cv:
+                    self.done.append(r)
         with self.cv:
             self.closed = True
             self.cv.notify_all()
         for t in threads:
             t.join()
         wall = time.perf_counter() - t0
-        for held in bufs:
-            for b in held:
-                if b is not None:
-                    b.free()
+        for _, release in slots:
+            release()
         return wall
 
+    def accounted(self):
+        """True when the reserved bytes equal those of the cached entries."""
+        with self.cv:
+            return self.used == sum(e.nbytes for e in self.cache.values())
+
     def close(self):
-        for e in self.cache.values():
-            e.solver.close()
-        self.cache.clear()
-        self.used = 0
+        with self.cv:
+            for key in list(self.cache):
+                self._drop(key, self.cache[key])
+            self.cache.clear()
 
 
 def poisson_stream(rate, duration, seed, side=(32, 80), freq=(0.05, 0.5), nrhs=(1, 64), reuse=0.3, dx=200.0,
@@ -273,15 +384,24 @@ def poisson_stream(rate, duration, seed, side=(32, 80), freq=(0.05, 0.5), nrhs=(
     return reqs
 
 
+def project_requests(name, nrhs, src_seed, start_id=0):
+    """One request per frequency of an on-disk project (the frequencies of
+    its ini file, readini.f90:168-178), all arriving at once."""
+    from . import io
+    p = io.read_ini(name + ".ini")
+    return [Request(start_id + i, None, None, p["dx"], f, nrhs, src_seed + i, 0.0, project=name)
+            for i, f in enumerate(p["freq"])]
+
+
 def latency_report(reqs, wall):
     ok = [r for r in reqs if r.error is None and r.end is not None]
     lat = np.array([r.end - r.arrival_abs for r in ok]) * 1e3
     svc = np.array([r.end - r.start for r in ok]) * 1e3
-    rhs = sum(r.nrhs for r in ok)
+    rhs_ = sum(r.nrhs for r in ok)
     return {"requests": len(reqs), "completed": len(ok), "errors": len(reqs) - len(ok),
             "p50_latency_ms": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_ms": float(np.percentile(lat, 99)) if len(lat) else None,
             "p50_service_ms": float(np.percentile(svc, 50)) if len(svc) else None,
-            "rhs_per_s": rhs / wall, "wall_s": wall,
+            "rhs_per_s": rhs_ / wall, "wall_s": wall,
             "factor_reuse": sum(1 for r in ok if r.reused),
             "max_rel_residual": max((r.stats["rel_residual"] for r in ok), default=None)}

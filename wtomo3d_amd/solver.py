@@ -14,7 +14,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import lib, check, WtSolveStats
+from ._lib import lib, check, WtSolveStats, NotConverged, WT_ERR_NOCONV
 
 _vp = ctypes.c_void_p
 
@@ -147,16 +147,21 @@ class Solver:
         return out
 
     def solve(self, b, prec=32, tol=1e-8, max_iter=20):
-        """b: (n,) or (n, nrhs) complex; returns (x, stats dict)."""
+        """b: (n,) or (n, nrhs) complex; returns (x, stats dict).  Raises
+        NotConverged (with .x and .stats) when refinement ends above tol;
+        tol <= 0 runs exactly max_iter solves."""
         b = np.asarray(b)
         one = b.ndim == 1
         B = np.ascontiguousarray(b.reshape(self.n, -1), np.complex128)
         X = np.zeros_like(B)
         st = WtSolveStats()
-        check(lib().wt_solver_solve(self._h, prec, B.shape[1], _p(B), _p(X), 0, tol, max_iter,
-                                    ctypes.byref(st)), "wt_solver_solve")
+        rc = lib().wt_solver_solve(self._h, prec, B.shape[1], _p(B), _p(X), 0, tol, max_iter, ctypes.byref(st))
         stats = {k: getattr(st, k) for k, _ in WtSolveStats._fields_}
-        return (X[:, 0] if one else X), stats
+        x = X[:, 0] if one else X
+        if rc == WT_ERR_NOCONV:
+            raise NotConverged("wt_solver_solve", stats, x)
+        check(rc, "wt_solver_solve")
+        return x, stats
 
     def stream(self):
         """The HIP stream of this handle (a ctypes void pointer)."""
