@@ -233,6 +233,22 @@ int wt_haskgrn(int prec, int nl, const void *vp, const void *vs, const void *rho
                int nz, double dz, double zorig, void *d_u, void *d_w, double *d_pic, int32_t *status,
                void *stream);
 
+/* wt_haskgrn with no device allocation.  No counterpart in the reference:
+ * an entry point for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, outputs and errors as wt_haskgrn, with at most
+ * WT_HASKGRN_NOALLOC_MAX_SOURCES sources (more: WT_ERR_ARG and a message
+ * on stderr).  The staged inputs and the status live in a fixed device
+ * array of 67 KB per GPU, kept until the process ends; it belongs to the
+ * current device, and the stream must be one of that device's.  Calls that
+ * overlap on one GPU race on that array; calls made one after another are
+ * safe (each returns when its work is done). */
+#define WT_HASKGRN_NOALLOC_MAX_SOURCES 4096
+int wt_haskgrn_noalloc(int prec, int nl, const void *vp, const void *vs, const void *rho, const void *top,
+                       const void *vpd, const void *vsd, int ns, const void *ain, const int32_t *is_s,
+                       double omega, int nz, double dz, double zorig, void *d_u, void *d_w, double *d_pic,
+                       int32_t *status, void *stream);
+
 /* 3-D background fields of plane waves from their depth profiles (the
  * expansion elwave's missing planewvexp.f90 would do; defined by this
  * library):

@@ -144,6 +144,8 @@ def _declare(L):
     L.wt_haskgrn.argtypes = [i32, i32] + [_vp] * 6 + [i32, _vp, _vp, ctypes.c_double, i32, ctypes.c_double,
                                                      ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
     L.wt_haskgrn.restype = ctypes.c_int
+    L.wt_haskgrn_noalloc.argtypes = L.wt_haskgrn.argtypes
+    L.wt_haskgrn_noalloc.restype = ctypes.c_int
     L.wt_planewave_layered.argtypes = [i32] * 3 + [ctypes.c_double, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp]
     L.wt_planewave_layered.restype = ctypes.c_int
     L.wt_solver_scattered_source.argtypes = [_vp, _vp, _vp, _vp, i32, i32, _vp, _vp, _vp]
