@@ -39,7 +39,8 @@ typedef struct wt_grid_params {
     int32_t nx, ny, nz;
     int32_t ipml;      /* INT(pmld/dx) + 1, readini.f90:83 */
     int32_t fst;       /* freesurf(1): free surface at iz = 1 */
-    int32_t usemin;    /* Min-cell free surface; 0 (traction-free) is unsupported */
+    int32_t usemin;    /* 1: Min-cell free surface; 0: traction-free top rows (TPFREE,
+                        * tpfree.f:55-420, at iz = 1 when fst is set) */
     double dx;         /* grid spacing [m]; dx = dy = dz (readini.f90:55-59) */
     double pmld;       /* PML width [m] */
     double pmlf;       /* 3 ln(1/pmlr) / (2 pmld^3), readini.f90:84 */

@@ -162,6 +162,45 @@ static int FN(node_blocks_)(const FN(prm_) *P, const CT *lam, const CT *mu,
     R vpreal = vp;
     CT one = FN(mk_)(1, 0), zero = FN(mk_)(0, 0);
     CT r1x = one, r1y = one, r1z = one, r2x = zero, r2y = zero, r2z = zero;
+    if (mode == 3) {
+        /* tpfree.f:55-420: the stretch factors r1 of the x and y PML (kmax = 1,
+         * fdom = 0, the same expression as pml3d.f's r1), r1z = 1, no mass
+         * term; only the z+1 corners (cells 5-8 below the node) and the centre
+         * couple.  The centre's VW entry uses the lambda cells where UW uses
+         * mu, as tpfree.f:321 has it. */
+        if (FN(pml_axis_)(P, ix, nx, vpreal, eyeom, &r1x, &r2x)) return 1;
+        if (FN(pml_axis_)(P, iy, ny, vpreal, eyeom, &r1y, &r2y)) return 1;
+        R dxo8 = (R)1 / (dx * (R)8);
+        CT X = FN(scl_)(dxo8, r1x), Y = FN(scl_)(dxo8, r1y), Z = FN(scl_)(dxo8, r1z);
+        for (int k = 0; k < 27; ++k)
+            for (int e = 0; e < 9; ++e) blk[k][e] = zero;
+        /* corners: label index, cell, signs of the x and y terms */
+        static const int ck[4] = {2, 20, 8, 26}, cc[4] = {5, 6, 7, 8}, sx[4] = {-1, -1, 1, 1}, sy[4] = {-1, 1, -1, 1};
+        for (int q = 0; q < 4; ++q) {
+            int c = cc[q];
+            CT *b = blk[ck[q]];
+            CT xu = FN(mul_)(X, u[c]), yu = FN(mul_)(Y, u[c]), xd = FN(mul_)(X, d[c]), yd = FN(mul_)(Y, d[c]);
+            b[0] = FN(mul_)(Z, u[c]);                      /* UU */
+            b[2] = sx[q] > 0 ? xu : FN(neg_)(xu);          /* UW */
+            b[4] = FN(mul_)(Z, u[c]);                      /* VV */
+            b[5] = sy[q] > 0 ? yu : FN(neg_)(yu);          /* VW */
+            b[6] = sx[q] > 0 ? xd : FN(neg_)(xd);          /* WU */
+            b[7] = sy[q] > 0 ? yd : FN(neg_)(yd);          /* WV */
+            b[8] = FN(mul_)(Z, lp2u[c]);                   /* WW */
+        }
+        CT *b = blk[13];
+        CT su = FN(add_)(FN(add_)(FN(add_)(u[8], u[7]), u[6]), u[5]);
+        CT sl = FN(add_)(FN(add_)(FN(add_)(lp2u[8], lp2u[7]), lp2u[6]), lp2u[5]);
+        CT dy = FN(add_)(FN(sub_)(FN(sub_)(d[5], d[8]), d[6]), d[7]);
+        b[0] = FN(neg_)(FN(mul_)(Z, su));
+        b[2] = FN(mul_)(X, FN(sub_)(FN(add_)(FN(sub_)(u[5], u[8]), u[6]), u[7]));
+        b[4] = FN(neg_)(FN(mul_)(Z, su));
+        b[5] = FN(mul_)(Y, dy);
+        b[6] = FN(mul_)(X, FN(sub_)(FN(add_)(FN(sub_)(d[5], d[8]), d[6]), d[7]));
+        b[7] = FN(mul_)(Y, dy);
+        b[8] = FN(neg_)(FN(mul_)(Z, sl));
+        return 0;
+    }
     if (mode == 1) {
         if (FN(pml_axis_)(P, ix, nx, vpreal, eyeom, &r1x, &r2x)) return 1;
         if (FN(pml_axis_)(P, iy, ny, vpreal, eyeom, &r1y, &r2y)) return 1;
@@ -372,7 +411,7 @@ static int FN(node_blocks_)(const FN(prm_) *P, const CT *lam, const CT *mu,
 }
 
 /* Region class of a node, fdfd3d.f:50-115.  Returns 0 interior, 1 PML,
- * 2 Dirichlet, -1 unsupported (traction-free top, usemin = false);
+ * 2 Dirichlet, 3 traction-free top (iz = 1, usemin = false, fdfd3d.f:73-85);
  * *izp is the z index the PML form is evaluated at. */
 static int FN(region_)(const FN(prm_) *P, int iz, int ix, int iy, int *izp)
 {
@@ -384,7 +423,7 @@ static int FN(region_)(const FN(prm_) *P, int iz, int ix, int iy, int *izp)
     if (iz <= ip) {
         if (iz == 1) {
             if (!P->fst) return 2;
-            if (!P->usemin) return -1;
+            if (!P->usemin) return 3;
         } else if (!P->fst) {
             return 1;
         }
@@ -413,7 +452,6 @@ int FN(ora_assemble_)(int nx, int ny, int nz, R dx, R om_re, R om_im,
         for (int ix = 1; ix <= nx; ++ix)
             for (int iz = 1; iz <= nz; ++iz) {
                 int izp, mode = FN(region_)(&P, iz, ix, iy, &izp);
-                if (mode < 0) return 2;
                 if (FN(node_blocks_)(&P, L, M, rho, iz, ix, iy, mode, izp, blk)) return 1;
                 int has[27];
                 for (int oy = -1; oy <= 1; ++oy)

@@ -20,12 +20,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 import oracle_lib as O  # noqa: E402
 
-# (name, nx, ny, nz, dx, freq, pmld, pmlr, fst, tau, seed)
+# (name, nx, ny, nz, dx, freq, pmld, pmlr, fst, tau, seed, usemin); usemin = 0
+# is the traction-free top (TPFREE, fdfd3d.f:73-85)
 CASES = [
-    ("g667_fs", 6, 6, 7, 250.0, 0.3, 250.0, 1e-3, 1, 999.999, 11),
-    ("g667_fixed", 6, 6, 7, 250.0, 0.3, 250.0, 1e-3, 0, 999.999, 12),
-    ("g667_tau", 6, 6, 7, 250.0, 0.25, 250.0, 1e-3, 1, 3.0, 13),
-    ("g6610_thinz", 6, 6, 10, 400.0, 0.15, 4000.0, 1e-3, 1, 999.999, 14),
+    ("g667_fs", 6, 6, 7, 250.0, 0.3, 250.0, 1e-3, 1, 999.999, 11, 1),
+    ("g667_fixed", 6, 6, 7, 250.0, 0.3, 250.0, 1e-3, 0, 999.999, 12, 1),
+    ("g667_tau", 6, 6, 7, 250.0, 0.25, 250.0, 1e-3, 1, 3.0, 13, 1),
+    ("g6610_thinz", 6, 6, 10, 400.0, 0.15, 4000.0, 1e-3, 1, 999.999, 14, 1),
+    ("g987_tpfree", 9, 8, 7, 250.0, 0.3, 500.0, 1e-3, 1, 999.999, 15, 0),
+    ("g667_tpfree_tau", 6, 6, 7, 250.0, 0.25, 250.0, 1e-3, 1, 3.0, 16, 0),
 ]
 
 
@@ -43,19 +46,19 @@ def main():
     if not O.have_ref():
         raise SystemExit("oracle/_ref is not built: make -C oracle")
     meta = {}
-    for name, nx, ny, nz, dx, freq, pmld, pmlr, fst, tau, seed in CASES:
+    for name, nx, ny, nz, dx, freq, pmld, pmlr, fst, tau, seed, usemin in CASES:
         out = {}
         for prec in (32, 64):
             vp, vs, rho = model(nx, ny, nz, seed, prec)
             ierr, ir, jc, a = O.ref_assemble(nx, ny, nz, dx, freq, vp, vs, rho, pmld, pmlr,
-                                             fst=bool(fst), prec=prec, tau=tau)
+                                             fst=bool(fst), usemin=bool(usemin), prec=prec, tau=tau)
             assert ierr == 0
             out["vals%d" % prec] = a
             if prec == 64:
                 out.update(vp=vp, vs=vs, rho=rho, irptr=ir, jcptr=jc)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         meta[name] = dict(nx=nx, ny=ny, nz=nz, dx=dx, freq=freq, pmld=pmld, pmlr=pmlr, fst=fst,
-                          tau=tau, seed=seed)
+                          tau=tau, seed=seed, usemin=usemin)
     # 24^3 plain-method configuration: digests + sampled rows
     nx = ny = nz = 24
     ir, jc = O.ref_graph(nx, ny, nz)

@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 CASE_NAMES = [k for k, v in cases().items() if isinstance(v, dict) and "seed" in v]
 
 
-def _params(nx, ny, nz, dx, om, pmld, pmlf, ipml, fst=True):
-    return W._lib.WtGridParams(nx, ny, nz, ipml, int(fst), 1, dx, pmld, pmlf, om.real, om.imag)
+def _params(nx, ny, nz, dx, om, pmld, pmlf, ipml, fst=True, usemin=True):
+    return W._lib.WtGridParams(nx, ny, nz, ipml, int(fst), int(usemin), dx, pmld, pmlf, om.real, om.imag)
 
 
 @pytest.mark.parametrize("name", CASE_NAMES)
@@ -48,10 +48,11 @@ def _gpu_case(name, prec):
     dx, om, pmld, pmlf, ipml = ref_params(m, prec)
     ft = np.float32 if prec == 32 else np.float64
     lam, mu, rho = lame_ref(g["vp"].astype(ft), g["vs"].astype(ft), g["rho"].astype(ft), prec)
-    p = _params(m["nx"], m["ny"], m["nz"], dx, om, pmld, pmlf, ipml, bool(m["fst"]))
+    usemin = bool(m.get("usemin", 1))
+    p = _params(m["nx"], m["ny"], m["nz"], dx, om, pmld, pmlf, ipml, bool(m["fst"]), usemin)
     a = W.assemble(p, lam, mu, rho, prec=prec)
     rc, o = O.assemble(m["nx"], m["ny"], m["nz"], dx, om, lam, mu, rho, pmld, pmlf, ipml,
-                       fst=bool(m["fst"]), prec=prec)
+                       fst=bool(m["fst"]), usemin=usemin, prec=prec)
     assert rc == 0
     return a, o, g["vals%d" % prec]
 
@@ -61,9 +62,11 @@ def _gpu_case(name, prec):
 def test_gpu_values_vs_oracle_and_reference(name, prec):
     a, o, ref = _gpu_case(name, prec)
     # the kernel and the oracle evaluate the same operations in the same
-    # order; float32 may differ by a few ulps amplified by cancellation
+    # order, but the kernel's complex products may fuse into multiply-adds
+    # (the oracle is compiled without contraction): a few ulps, amplified
+    # where stretched PML terms cancel (observed <= 6.1e-15 in FP64)
     assert np.array_equal(a.view(np.uint8), o.view(np.uint8)) or \
-        rel_err_entries(a, o)[0] <= (1e-15 if prec == 64 else 4e-6)
+        rel_err_entries(a, o)[0] <= (1e-14 if prec == 64 else 4e-6)
     rel, zm = rel_err_entries(a, ref)
     assert zm == 0
     assert rel <= (1e-12 if prec == 64 else 3e-5)
@@ -94,11 +97,13 @@ def test_gpu_values_layered_64x64x48(prec):
 
 @pytest.mark.parametrize("prec", [32, 64])
 @pytest.mark.parametrize("dims", [(6, 5, 150), (4, 3, 65), (5, 4, 128), (5, 3, 16), (7, 6, 17), (4, 6, 49)])
-def test_gpu_values_several_z_segments(dims, prec):
+@pytest.mark.parametrize("usemin", [True, False])
+def test_gpu_values_several_z_segments(dims, prec, usemin):
     """Columns longer than one 64-node segment (a one-node last segment, an
     exact multiple, three segments) and columns ending at, one past and one
     past three 16-node output chunks: every chunk's rows land at their CSR
-    offsets; heterogeneous random model, PML on every side."""
+    offsets; heterogeneous random model, PML on every side, with the
+    Min-cell (usemin) and the traction-free top."""
     nx, ny, nz = dims
     rng = np.random.default_rng(77 + nz)
     ft = np.float32 if prec == 32 else np.float64
@@ -108,8 +113,8 @@ def test_gpu_values_several_z_segments(dims, prec):
     meta = dict(dx=250.0, freq=0.3, pmld=500.0, pmlr=1e-3)
     dx_, om, pmld, pmlf, ipml = ref_params(meta, prec)
     lam, mu, rho1 = lame_ref(vp, vs, rho, prec)
-    a = W.assemble(_params(nx, ny, nz, dx_, om, pmld, pmlf, ipml), lam, mu, rho1, prec=prec)
-    rc, o = O.assemble(nx, ny, nz, dx_, om, lam, mu, rho1, pmld, pmlf, ipml, prec=prec)
+    a = W.assemble(_params(nx, ny, nz, dx_, om, pmld, pmlf, ipml, True, usemin), lam, mu, rho1, prec=prec)
+    rc, o = O.assemble(nx, ny, nz, dx_, om, lam, mu, rho1, pmld, pmlf, ipml, usemin=usemin, prec=prec)
     assert rc == 0
     # a misplaced entry would be off by O(1); rounding of the random model's
     # cell averages stays at the assembly bar (1e-12 FP64, 3e-5 FP32)

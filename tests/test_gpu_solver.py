@@ -327,3 +327,30 @@ def test_refinement_reports_non_convergence():
     X, st = s.solve(B, prec=32, tol=1e-10, max_iter=30)
     assert st["converged"] == 1 and st["stalled"] == 0
     s.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_traction_free_top_solves(prec):
+    """The traction-free top (usemin = false, TPFREE) assembled on the GPU,
+    factored and solved: against SuperLU of the oracle's operator."""
+    import oracle_lib as O
+    nx, ny, nz, dx = 10, 9, 11, 250.0
+    vp, vs, rho = __import__("solve_cases").layered_model(nx, ny, nz, dx, 6, 0.05)
+    lam, mu = O.lame(vp, vs, rho, 64)
+    ipml, pmlf = O.pml_params(dx, 2 * dx, 1e-3, 64)
+    om = 2 * np.pi * 0.3
+    rc, vals = O.assemble(nx, ny, nz, dx, om, lam, mu, rho.ravel(), 2 * dx, pmlf, ipml, usemin=False, prec=64)
+    assert rc == 0
+    A = SO.csr(*O.graph(nx, ny, nz), vals)
+    s = Solver(nx, ny, nz)
+    s.set_model(lam, mu, rho.ravel())
+    s.assemble(W._lib.WtGridParams(nx, ny, nz, ipml, 1, 0, dx, 2 * dx, pmlf, om, 0.0))
+    assert np.abs(s.values() - vals).max() <= 1e-13 * np.abs(vals).max()
+    B, Xt = rhs_block(A, 3, 61)
+    Xo = SO.solve_superlu(A, B)
+    s.factor(prec)
+    X, st = s.solve(B, prec=prec, tol=1e-10, max_iter=30)
+    s.close()
+    assert SO.rel_residual(A, X, B) <= 1e-10
+    err_oracle = np.max(np.abs(Xo - Xt)) / np.max(np.abs(Xt))
+    assert np.max(np.abs(X - Xt)) / np.max(np.abs(Xt)) <= 10 * err_oracle + 1e-11

@@ -64,7 +64,7 @@ def _oracle_case(name, prec):
     ft = np.float32 if prec == 32 else np.float64
     lam, mu, rho = lame_ref(g["vp"].astype(ft), g["vs"].astype(ft), g["rho"].astype(ft), prec)
     rc, a = O.assemble(m["nx"], m["ny"], m["nz"], dx, om, lam, mu, rho, pmld, pmlf, ipml,
-                       fst=bool(m["fst"]), prec=prec)
+                       fst=bool(m["fst"]), usemin=bool(m.get("usemin", 1)), prec=prec)
     assert rc == 0
     return a, g["vals%d" % prec]
 
@@ -197,3 +197,42 @@ def test_dirichlet_rows_identity():
     rows = (3 * np.where(fixed)[0][:, None] + np.arange(3)[None, :]).ravel()
     B = A[rows]
     assert np.array_equal(B.toarray(), sp.identity(A.shape[0], format="csr")[rows].toarray())
+
+
+def test_traction_free_rows_give_half_the_surface_traction():
+    """TPFREE (tpfree.f:55-420): at a top node outside the horizontal PML of
+    a homogeneous model the rows of a linear displacement field are half the
+    traction (sigma_xz, sigma_yz, sigma_zz) it exerts on the surface, a
+    rigid translation gives zero rows, and there is no mass term."""
+    nx, ny, nz, dx = 9, 9, 6, 250.0
+    vp, vs, rho = 6000.0, 3500.0, 2600.0
+    mu = vs * vs * rho
+    lam = vp * vp * rho - 2 * mu
+    sh = (ny, nx, nz)
+    L, M = O.lame(np.full(sh, vp), np.full(sh, vs), np.full(sh, rho), 64)
+    ipml, pmlf = O.pml_params(dx, 2 * dx, 1e-3, 64)
+    rows = {}
+    # (omega = 0 makes r1 = 0/0 at every node, in the reference as here)
+    for om in (2 * np.pi * 0.05, 2 * np.pi * 0.4):
+        rc, a = O.assemble(nx, ny, nz, dx, om, L, M, np.full(nx * ny * nz, rho), 2 * dx, pmlf, ipml, fst=True,
+                           usemin=False, prec=64)
+        assert rc == 0
+        ir, jc = O.graph(nx, ny, nz)
+        rows[om] = sp.csr_matrix((a, jc - 1, ir - 1), shape=(3 * nx * ny * nz,) * 2)
+    A = rows[2 * np.pi * 0.05]
+    g = np.arange(nx * ny * nz)
+    iz, ix, iy = g % nz, (g // nz) % nx, g // (nx * nz)
+    x, y, z = ix * dx, iy * dx, iz * dx
+    o = 0 * x
+    c = int(np.nonzero((iz == 0) & (ix == 4) & (iy == 4))[0][0])
+    # no frequency dependence at a top node away from the PML (no mass term)
+    assert abs(rows[2 * np.pi * 0.4][3 * c:3 * c + 3] - A[3 * c:3 * c + 3]).max() == 0
+    F = lambda a, b, cc: np.stack([a, b, cc], 1).ravel()  # noqa: E731
+    # field -> traction (sigma_xz, sigma_yz, sigma_zz)
+    cases = [(F(1 + o, o, o), (0, 0, 0)), (F(o, o, 1 + o), (0, 0, 0)), (F(y, o, o), (0, 0, 0)),
+             (F(z, o, o), (mu, 0, 0)), (F(o, z, o), (0, mu, 0)), (F(o, o, z), (0, 0, lam + 2 * mu)),
+             (F(x, o, o), (0, 0, lam)), (F(o, y, o), (0, 0, lam)), (F(o, o, x), (mu, 0, 0)),
+             (F(o, o, y), (0, mu, 0))]
+    for u, t in cases:
+        r = (A @ u)[3 * c:3 * c + 3]
+        assert np.allclose(r, 0.5 * np.asarray(t, float), rtol=1e-12, atol=1e-12 * (lam + 2 * mu)), (r, t)

@@ -506,7 +506,8 @@ __device__ inline void segment_of_block(int64_t nblk, int nzs, int nx, int& zs, 
 // iz) whose tile z-slot is tz: mass weights (calcwts.f:28-71, with the
 // moduli of setcoefs3d.f:224-225) and PML stretch factors (pml3d.f:207-295);
 // the octant cells come from the segment's cell table s_cell.  Returns 0
-// interior, 1 PML, 2 Dirichlet.
+// interior, 1 PML, 2 Dirichlet, 3 traction-free top (tpfree.f, iz = 1 with
+// usemin = false; then rx1, ry1, rz1 hold r1x dxo8, r1y dxo8, r1z dxo8).
 template <typename R, int TZN, int CW>
 __device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz, int tz,
                                  const cpx<R> (*s_lam)[TZN], const cpx<R> (*s_mu)[TZN],
@@ -520,7 +521,7 @@ __device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz
     else if (iz == nz || ix == 1 || ix == nx || iy == 1 || iy == ny) mode = 2;
     else if (iz <= ip) {
         if (iz == 1 && !p.fst) mode = 2;
-        else if (iz == 1 && !p.usemin) { if (report) atomicOr(err, 2); }
+        else if (iz == 1 && !p.usemin) mode = 3;
         else if (!p.fst) mode = 1;
         else if (inx && iny) mode = 0;
         else { mode = 1; izp = ip + 1; }
@@ -562,9 +563,22 @@ __device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz
         ok = pml_axis(izp, nz, ip, pmld, dx, pmlf, vp, eyeom, r1z, r2z) && ok;
         if (!ok && report) atomicOr(err, 1);
     }
-    c.rx1 = (r2x / dx2) / R(4);
-    c.ry1 = (r2y / dx2) / R(4);
-    c.rz1 = (r2z / dx2) / R(4);
+    if (mode == 3) {
+        // tpfree.f:67-124: the x and y stretch factors r1 of the PML form
+        // (kmax = 1, fdom = 0), r1z = 1, scaled by 1 / (8 dx)
+        const R pmld = (R)p.pmld, pmlf = (R)p.pmlf;
+        bool ok = pml_axis(ix, nx, ip, pmld, dx, pmlf, vp, eyeom, r1x, r2x);
+        ok = pml_axis(iy, ny, ip, pmld, dx, pmlf, vp, eyeom, r1y, r2y) && ok;
+        if (!ok && report) atomicOr(err, 1);
+        const R dxo8 = R(1) / (dx * R(8));
+        c.rx1 = dxo8 * r1x;
+        c.ry1 = dxo8 * r1y;
+        c.rz1 = dxo8 * r1z;
+    } else {
+        c.rx1 = (r2x / dx2) / R(4);
+        c.ry1 = (r2y / dx2) / R(4);
+        c.rz1 = (r2z / dx2) / R(4);
+    }
     c.rx2 = (((r1x * r1x) / dxx) / R(4)) / R(2);
     c.ry2 = (((r1y * r1y) / dxx) / R(4)) / R(2);
     c.rz2 = (((r1z * r1z) / dxx) / R(4)) / R(2);
@@ -572,6 +586,53 @@ __device__ inline int node_setup(const wt_grid_params& p, int ix, int iy, int iz
     c.rxz = ((r1x * r1z) / dxx4) / R(4);
     c.ryz = ((r1y * r1z) / dxx4) / R(4);
     return mode;
+}
+
+// The traction-free top (tpfree.f:55-420, fdfd3d.f:73-85): at iz = 1 with
+// usemin = false the node couples only to its four z+1 corners, through the
+// octant cell below the node on that side (5..8), and to itself; no mass
+// term.  c.rx1, c.ry1, c.rz1 hold r1x/(8 dx), r1y/(8 dx), r1z/(8 dx).  The
+// centre's VW entry takes the lambda cells where UW takes mu, as
+// tpfree.f:321 has it.  Writes the five 3x3 blocks of node (1, ix, iy) into
+// the chunk span that starts at that node (a non-Dirichlet node: every x, y
+// neighbour exists).
+template <typename R, typename V>
+__device__ void tpfree_store(const NodeCoef<R>& c, int ix, int iy, int nx, int ny, int nz, V* stage)
+{
+    const cpx<R> X = c.rx1, Y = c.ry1, Z = c.rz1;
+    auto lp2u = [&](int i) { return (c.D(i) + c.U(i)) + c.U(i); };
+    const int cxn = 1 + (ix > 1) + (ix < nx), cyn = 1 + (iy > 1) + (iy < ny);
+    const int czn = 1 + (nz > 1);  // iz = 1: itself and z + 1
+    const int rl = 3 * cxn * cyn * czn;
+    auto put = [&](int ox, int oy, int oz, const cpx<R> b[9]) {
+        const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
+        const int pos = ((ry * cxn + rx) * czn + oz) * 3;
+        for (int r = 0; r < 3; ++r)
+            for (int e = 0; e < 3; ++e) {
+                V w;
+                w.x = b[3 * r + e].re;
+                w.y = b[3 * r + e].im;
+                stage[pos + r * rl + e] = w;
+            }
+    };
+    const cpx<R> z0(R(0), R(0));
+    // corners (ox, oy) with cells 5 (-,-), 6 (-,+), 7 (+,-), 8 (+,+)
+    for (int q = 0; q < 4; ++q) {
+        const int ox = q < 2 ? -1 : 1, oy = (q & 1) ? 1 : -1, cell = 5 + q;
+        const cpx<R> u = c.U(cell), d = c.D(cell);
+        const cpx<R> xu = X * u, yu = Y * u, xd = X * d, yd = Y * d;
+        const cpx<R> b[9] = {Z * u, z0, ox > 0 ? xu : -xu,
+                             z0, Z * u, oy > 0 ? yu : -yu,
+                             ox > 0 ? xd : -xd, oy > 0 ? yd : -yd, Z * lp2u(cell)};
+        put(ox, oy, 1, b);
+    }
+    const cpx<R> su = ((c.U(8) + c.U(7)) + c.U(6)) + c.U(5);
+    const cpx<R> sl = ((lp2u(8) + lp2u(7)) + lp2u(6)) + lp2u(5);
+    const cpx<R> dy = ((c.D(5) - c.D(8)) - c.D(6)) + c.D(7);
+    const cpx<R> b[9] = {-(Z * su), z0, X * (((c.U(5) - c.U(8)) + c.U(6)) - c.U(7)),
+                         z0, -(Z * su), Y * dy,
+                         X * (((c.D(5) - c.D(8)) + c.D(6)) - c.D(7)), Y * dy, -(Z * sl)};
+    put(0, 0, 0, b);
 }
 
 // Marks every coefficient as possibly changed here (an empty asm statement).
@@ -759,7 +820,9 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                 const int posN = 9 * cxy * (col_units(mN, nz) - col_units(m0, nz)) + ((ry * cxn + rx) * cznN + rz) * 3;
                 auto emit = [&](int e, cpx<R> val) {
                     if (!ok) return;
-                    if (modeN == 2) val = cpx<R>(R(0), R(0));  // a group label is never the centre
+                    // a group label is never the centre; traction-free nodes
+                    // get their corners after the labels
+                    if (modeN >= 2) val = cpx<R>(R(0), R(0));
                     V w;
                     w.x = val.re;
                     w.y = val.im;
@@ -791,10 +854,11 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
             if (topfree && oz == -1) ro = R(0);
             cpx<R> v[9];
             label_block<R>(k, c, ro, v);
-            if (mode == 2) {
+            if (mode >= 2) {
+                // Dirichlet: identity centre block; traction-free: written after the labels
 #pragma unroll
                 for (int e = 0; e < 9; ++e)
-                    v[e] = cpx<R>((k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
+                    v[e] = cpx<R>((mode == 2 && k == 13 && (e == 0 || e == 4 || e == 8)) ? R(1) : R(0), R(0));
             }
             const int ry = oy > -1 ? (iy > 1) + (oy > 0) : 0, rx = ox > -1 ? (ix > 1) + (ox > 0) : 0;
             const int rz = oz > -1 ? (iz > 1) + (oz > 0) : 0;
@@ -810,6 +874,12 @@ __global__ __launch_bounds__(ASM_THREADS, 2) void assemble_values_kernel(
                 }
         }
         __syncthreads();
+        if (p.fst && !p.usemin && m0 == 0) {  // block-uniform: the chunk holds the top node
+            // the traction-free top node (lane 0 of wave 0 holds node iz = 1)
+            // overwrites its five coupled blocks in the span
+            if (t == 0 && mode == 3) tpfree_store<R>(c, ix, iy, nx, ny, nz, stage);
+            __syncthreads();
+        }
         // Span out.  The LDS reads of a half are issued before its first
         // store, so a wave waits on LDS twice per chunk; the stores drain
         // while the next chunk is evaluated.
@@ -894,7 +964,6 @@ int wt_launch_values(const wt_grid_params& p, const void* d_lam, const void* d_m
                      void* d_vals, int* d_err, hipStream_t stream)
 {
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return WT_ERR_ARG;
-    if (!p.usemin && p.fst) return WT_ERR_UNSUPPORTED;
     const int64_t nblk = (int64_t)((p.nz + ASM_WAVE - 1) / ASM_WAVE) * p.nx * p.ny;
     if (nblk > 0x7fffffff) return WT_ERR_ARG;
     hipLaunchKernelGGL(assemble_values_kernel<R>, dim3((unsigned)nblk), dim3(ASM_THREADS), 0, stream, p,

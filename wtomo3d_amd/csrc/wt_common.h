@@ -16,9 +16,11 @@
         }                                                                         \
     } while (0)
 
-// Complex value with explicit component arithmetic.  Operation order is
-// spelled out (no fused multiply-add contraction in the assembly unit) so the
-// assembled values are reproducible bit for bit.
+// Complex value with explicit component arithmetic, the operation order
+// spelled out.  These operators are compiled with the default contraction:
+// a product and the sum it feeds may fuse into one multiply-add (rounded
+// once), so results can differ from a contraction-free evaluation by a few
+// ulps.
 template <typename R>
 struct cpx {
     R re, im;
