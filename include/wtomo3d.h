@@ -276,6 +276,20 @@ int wt_planewave_layered(int nx, int ny, int nz, double dx, int prec, int nrhs, 
 int wt_solver_scattered_source(wt_solver *h, const void *lam_bg, const void *mu_bg, const double *rho_bg,
                                int on_device, int nrhs, const void *d_ubg, const double *srcfn, void *d_sfld);
 
+/* The same source without a solver handle: both operators (the model's
+ * lam, mu, rho and the background's, complex128 / double, (nz,nx,ny) z
+ * fastest; host, or device with on_device) assembled at *p into
+ * stream-ordered scratch with the CSR pattern, then
+ *   sfld_j = srcfn_j (A_bg - A) u_bg_j.
+ * d_ubg, d_sfld: device, row-major with row strides ldu, ldo (<= 0: nrhs),
+ * so the source can land in a column range of a wider block.  Scratch:
+ * 2 nnz complex128 + the pattern.  Runs on the given stream and returns when
+ * it is done. */
+int wt_scattered_source_grid(const wt_grid_params *p, const void *lam, const void *mu, const double *rho,
+                             const void *lam_bg, const void *mu_bg, const double *rho_bg, int on_device, int nrhs,
+                             const void *d_ubg, int64_t ldu, const double *srcfn, void *d_sfld, int64_t ldo,
+                             void *stream);
+
 /* Total field from the scattered field: d_uh += d_ubg over count complex128
  * values (device); replaces the totfld branch of MAKEUH (makeuh.f:41-59).
  * Runs on the given stream (NULL: default) and returns when it is done. */
@@ -387,6 +401,9 @@ int wt_batch_destroy(wt_batch *b);
 int wt_batch_info(const wt_batch *b, int64_t *info, double *flops);
 /* unk_off[nmodels + 1]: first unknown of each model; group_of[nmodels]. */
 int wt_batch_offsets(const wt_batch *b, int64_t *unk_off, int32_t *group_of);
+/* The HIP stream of the batch's first group (for callers that put their
+ * own device work in order with the batch). */
+int wt_batch_stream(const wt_batch *b, void **stream);
 /* Capture the factorization and the refinement step in HIP graphs (default
  * on) or launch eagerly. */
 int wt_batch_set_graphs(wt_batch *b, int enable);
@@ -438,6 +455,10 @@ int wt_dev_sync(void);
 /* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0): the
  * reference point for HBM traffic counters. */
 int wt_dev_copy(void *d_dst, const void *d_src, size_t bytes);
+/* rows x width bytes from d_src (row pitch spitch) to d_dst (row pitch
+ * dpitch), device to device, on the given stream; returns when done. */
+int wt_dev_copy2d(void *d_dst, size_t dpitch, const void *d_src, size_t spitch, size_t width, size_t rows,
+                  void *stream);
 /* Number of visible GPUs (0 without one). */
 int wt_device_count(void);
 

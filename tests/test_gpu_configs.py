@@ -16,14 +16,15 @@ C4  256 layered models of uneven size, each its own frequency and 1-16
     right-hand sides: every model's residual and wavefield, and a sample of
     models against single-model complex128 solves.
 C5  a bounded sample of the serving stream (sides 32-80, 1-64 sources,
-    30 % factor reuse): each result bit-identical alone and in the mix.
+    30 % factor reuse): each result the same alone and in the mix -- bit for
+    bit on its own handle, within the refinement tolerance when it shared a
+    batch with other small models.
 
 The reference's own solve (STRUMPACK) is not available (parity unpinned
 against it); these tests pin the solves by independent FP64 solves, by the
 FP64 residual and by known wavefields.
 """
 import ctypes
-import hashlib
 import os
 import sys
 
@@ -180,14 +181,6 @@ def test_c4_batch_256_models():
         assert d <= 1e-6
 
 
-class _Digest(serving.Engine):
-    """Keeps a digest of each request's solution instead of the solution."""
-
-    def _solve(self, r, s, prob, slot):
-        super()._solve(r, s, prob, slot)
-        x = slot(0, 0, 0)[0].to_host(np.complex128, (s.n, r.nrhs))
-        r.digest = hashlib.sha256(x.tobytes()).hexdigest()
-        r.norm = float(np.linalg.norm(x))
 
 
 @pytest.mark.timeout(900)
@@ -197,13 +190,13 @@ def test_c5_serving_alone_equals_mix():
                                       reuse=0.3, dx=200.0)
     alone = stream()
     say("C5: %d requests, sides %s, alone" % (len(alone), sorted({max(r.dims) for r in alone})))
-    e1 = _Digest(hbm_budget=200e9, slots=1, tol=1e-8, lookahead=1)
+    e1 = serving.Engine(hbm_budget=200e9, slots=1, tol=1e-8, lookahead=1, return_solutions=True)
     e1.run(alone, realtime=False)
     f1 = e1.factorizations
     e1.close()
     mix = stream()
-    say("C5: the same requests on 4 concurrent slots")
-    e4 = _Digest(hbm_budget=200e9, slots=4, tol=1e-8)
+    say("C5: the same requests on 4 concurrent slots, small models batched")
+    e4 = serving.Engine(hbm_budget=200e9, slots=4, tol=1e-8, return_solutions=True)
     e4.run(mix, realtime=False)
     assert e4.accounted()
     f4 = e4.factorizations
@@ -213,8 +206,19 @@ def test_c5_serving_alone_equals_mix():
     # in order, with the factors that fit kept: one factorization per (model,
     # frequency); in the mix a factor evicted before its reuse is rebuilt
     assert f1 == len(keys) and len(keys) <= f4 <= len(alone)
+    worst = 0.0
     for a, b in zip(alone, mix):
         assert a.error is None and b.error is None, (a.rid, a.error, b.error)
         assert a.stats["converged"] == 1 and a.stats["rel_residual"] <= 1e-8
-        assert a.digest == b.digest, a.rid
-    say("C5: %d results bit-identical alone and mixed, %d factorizations" % (len(alone), f4))
+        assert b.stats["converged"] == 1 and b.stats["rel_residual"] <= 1e-8
+        if b.batched:
+            # a shared batch rounds differently; both solves are refined to 1e-8
+            d = float(np.linalg.norm(a.x - b.x) / np.linalg.norm(a.x))
+            worst = max(worst, d)
+            assert d <= 1e-6, (a.rid, d)
+        else:
+            # each request on its own handle, deterministic kernels
+            assert np.array_equal(a.x, b.x), a.rid
+        a.x = b.x = None
+    say("C5: %d results alone == mixed (%d batched, max difference %.2e), %d factorizations"
+        % (len(alone), sum(r.batched for r in mix), worst, f4))

@@ -155,6 +155,44 @@ def test_scattered_source_invariants_and_wide_blocks():
     s.close()
 
 
+def test_scattered_source_without_a_handle():
+    """wt_scattered_source_grid (both operators assembled in scratch, host
+    model arrays) equals the handle's source bit for bit, writes into a
+    column range of a wider block (row strides), and matches the reference's
+    HETFLDSRC fixture with its srcfn."""
+    import ctypes
+    from wtomo3d_amd import lib
+    from wtomo3d_amd.rhs import DeviceBuffer
+    g, s, u, srcfn = _golden_solver()
+    lam, mu, rho = media.lame(g["vp"], g["vs"], g["rho"])
+    lb, mb, rb = media.lame(g["vpb"], g["vsb"], g["rhob"])
+    rng = np.random.default_rng(5)
+    nr, wide, c0 = 5, 9, 3
+    B = np.ascontiguousarray(u[:, None] * (1.0 + 0.1 * rng.standard_normal(nr))[None, :])
+    ref = s.scattered_source(lb, mb, rb, B, None)
+    n = s.n
+    s.close()
+    _, nx, ny, nz, dx, freq, pmld, pmlr, _ = _golden_scatter(64)
+    dxp, om, pmldp, pmlf, ipml = ref_params(dict(dx=dx, freq=freq, pmld=pmld, pmlr=pmlr), 64)
+    p = grid_params(nx, ny, nz, dxp, om, pmld=pmldp)
+    p.pmlf, p.ipml = pmlf, ipml
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    dU = DeviceBuffer.from_host(B)
+    dO = DeviceBuffer.from_host(np.zeros((n, wide), np.complex128))
+    dst = ctypes.c_void_p(dO.ptr.value + c0 * 16)
+    assert lib().wt_scattered_source_grid(ctypes.byref(p), P(lam), P(mu), P(rho), P(lb), P(mb), P(rb), 0, nr,
+                                          dU.ptr, nr, None, dst, wide, None) == 0
+    O = dO.to_host(np.complex128, (n, wide))
+    assert np.array_equal(O[:, c0:c0 + nr], ref)
+    assert not np.any(O[:, :c0]) and not np.any(O[:, c0 + nr:])
+    du = DeviceBuffer.from_host(np.ascontiguousarray(u))
+    f = np.ascontiguousarray([srcfn], np.complex128)
+    assert lib().wt_scattered_source_grid(ctypes.byref(p), P(lam), P(mu), P(rho), P(lb), P(mb), P(rb), 0, 1,
+                                          du.ptr, 1, P(f), dO.ptr, 1, None) == 0
+    sf = dO.to_host(np.complex128, (n,))
+    assert np.abs(sf - g["sfld64"]).max() <= 1e-10 * np.abs(g["sfld64"]).max()
+
+
 def test_total_field_from_scattered_source():
     """Plane waves in the layered background, scattered by a perturbed model:
     solving A u_h = (A_bg - A) u_bg gives a total field u_h + u_bg with

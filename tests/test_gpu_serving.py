@@ -24,7 +24,7 @@ def test_alone_equals_mix_and_reuse():
     e1.run(alone, realtime=False)
     e1.close()
     mix = _requests()
-    e3 = serving.Engine(hbm_budget=50e9, slots=3, tol=1e-10, return_solutions=True)
+    e3 = serving.Engine(hbm_budget=50e9, slots=3, tol=1e-10, return_solutions=True, group_max=1)
     e3.run(mix, realtime=False)
     e3.close()
     for a, b in zip(alone, mix):
@@ -33,6 +33,32 @@ def test_alone_equals_mix_and_reuse():
         # each request runs on its own handle with deterministic kernels
         assert np.array_equal(a.x, b.x)
     assert alone[3].reused  # same model and frequency as request 0, run after it
+
+
+def test_small_models_batched_equal_alone():
+    """Pending small models share one batch (one level-by-level
+    factorization and one refined block solve); each result agrees with the
+    same request solved alone on its own handle, and a later request reuses
+    the batch's factor."""
+    reqs = _requests()
+    alone = serving.Engine(hbm_budget=50e9, slots=1, tol=1e-10, return_solutions=True, lookahead=1)
+    alone.run(reqs, realtime=False)
+    alone.close()
+    grp = _requests()
+    e = serving.Engine(hbm_budget=50e9, slots=2, tol=1e-10, return_solutions=True, group_max=8, group_nodes=48 ** 3)
+    e.run(grp, realtime=False)
+    assert e.groups >= 1 and e.accounted()
+    assert sum(r.batched for r in grp) >= 2
+    e.close()
+    assert e.used == 0
+    for a, b in zip(reqs, grp):
+        assert a.error is None and b.error is None, (a.error, b.error)
+        assert b.stats["converged"] == 1 and b.stats["rel_residual"] <= 1e-10
+        d = np.linalg.norm(a.x - b.x) / np.linalg.norm(a.x)
+        assert d <= 1e-7, (b.rid, b.batched, d)
+        if not b.batched:
+            assert np.array_equal(a.x, b.x)
+    assert any(r.reused for r in grp)
 
 
 def test_admission_budget():

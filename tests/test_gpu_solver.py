@@ -286,10 +286,10 @@ def test_fp32_factor_as_accurate_as_dense_partial_pivoting():
     assert r_gpu <= 10 * r_dense, (r_gpu, r_dense)
 
 
-@pytest.mark.parametrize("nrhs", [64, 40, 33, 32])
+@pytest.mark.parametrize("nrhs", [64, 40, 33, 32, 17, 16, 5])
 def test_wide_block_residual_matches_numpy(nrhs):
-    """Blocks of more than 32 columns take the node-wise residual kernel,
-    32 and fewer the row-wise one: the FP64 residual the refinement reports
+    """Blocks of more than 16 columns take the node-wise residual kernel,
+    16 and fewer the row-wise one: the FP64 residual the refinement reports
     (and stops on) agrees with numpy's on the returned solution, and the
     block solves each column as it would alone."""
     s, A, vals = _setup(6, 5, 7, 64, seed=8)

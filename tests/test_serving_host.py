@@ -27,6 +27,7 @@ class MockEngine(serving.Engine):
     ``fail`` raise in their solve, those in ``noconv`` end unconverged."""
 
     def __init__(self, fail=(), noconv=(), factor_s=0.05, solve_s=0.01, nbytes=1000, **kw):
+        kw.setdefault("group_max", 1)
         super().__init__(**kw)
         self.fail, self.noconv = set(fail), set(noconv)
         self.factor_s, self.solve_s, self.nbytes = factor_s, solve_s, nbytes
@@ -62,6 +63,24 @@ class MockEngine(serving.Engine):
 
     def _slot(self):
         return (lambda *a: None), (lambda: None)
+
+    def _factor_group(self, rs, probs):
+        time.sleep(self.factor_s)
+        with self.lock:
+            self.log.append(("group", tuple(r.rid for r in rs)))
+        return FakeSolver(self.log)
+
+    def _solve_group(self, rs, e, slot):
+        if e.batch.closed:
+            raise RuntimeError("closed batch")
+        with self.cv:
+            self.max_used = max(self.max_used, self.used)
+        time.sleep(self.solve_s)
+        for r in rs:
+            assert r.key() in e.index
+            r.stats = {"rel_residual": 0.0, "iterations": 1}
+            if r.rid in self.fail:
+                r.error = "injected failure"
 
 
 def req(rid, key, freq=0.3, dims=(32, 32, 32)):
@@ -196,3 +215,50 @@ def test_project_requests(tmp_path):
     prob = serving.Engine._load_project(name)
     assert prob.dims == (12, 11, 10) and prob.dx == 250.0 and prob.usemin and prob.fst
     assert rs[0].key() != rs[1].key()
+
+
+def test_small_models_share_one_batch():
+    """Pending new small models are admitted together into one batch entry
+    that serves every member's key; later requests for those keys reuse it,
+    and its bytes are held once."""
+    e = MockEngine(slots=1, hbm_budget=100_000, group_max=4, group_nodes=48 ** 3, lookahead=8)
+    rs = [req(i, 10 + i) for i in range(4)] + [req(4, 11), req(5, 13)]
+    e.run(rs, realtime=False)
+    assert all(r.error is None for r in rs)
+    groups = [x for x in e.log if isinstance(x, tuple) and x[0] == "group"]
+    assert groups == [("group", (0, 1, 2, 3))] and e.groups == 1 and e.factorizations == 4
+    assert rs[4].reused and rs[5].reused
+    assert len(e.cache) == 4 and len({id(v) for v in e.cache.values()}) == 1
+    assert e.used == 4000 and e.accounted()
+    e.close()
+    assert e.used == 0 and not e.cache and e.log.count("close") == 1
+
+
+def test_large_models_are_not_grouped():
+    e = MockEngine(slots=2, hbm_budget=100_000, group_max=4, group_nodes=48 ** 3)
+    rs = [req(i, 20 + i, dims=(64, 48, 48)) for i in range(3)]
+    e.run(rs, realtime=False)
+    assert not [x for x in e.log if isinstance(x, tuple) and x[0] == "group"]
+    assert e.factorizations == 3 and e.accounted()
+    e.close()
+
+
+def test_failed_group_member_drops_the_batch_once():
+    e = MockEngine(slots=1, hbm_budget=100_000, group_max=3, fail={1})
+    rs = [req(i, 30 + i) for i in range(3)] + [req(3, 30)]
+    e.run(rs, realtime=False)
+    assert rs[1].error == "injected failure" and rs[0].error is None and rs[2].error is None
+    assert not rs[3].reused  # the batch was dropped with its failed member: key 30 refactored
+    assert e.accounted() and e.log.count("close") == 1
+    e.close()
+    assert e.used == 0
+
+
+def test_group_eviction_under_budget():
+    e = MockEngine(slots=1, hbm_budget=3500, group_max=3)
+    rs = [req(i, 40 + i) for i in range(6)]
+    e.run(rs, realtime=False)
+    assert all(r.error is None for r in rs)
+    assert e.max_used <= 3500 and e.accounted()
+    e.close()
+    assert e.used == 0

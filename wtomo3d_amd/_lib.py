@@ -132,6 +132,10 @@ def _declare(L):
     L.wt_estimate_bytes.restype = i64
     L.wt_symbolic_info.argtypes = [i32] * 4 + [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.wt_symbolic_info.restype = ctypes.c_int
+    L.wt_dev_copy2d.argtypes = [_vp, ctypes.c_size_t, _vp] + [ctypes.c_size_t] * 3 + [_vp]
+    L.wt_dev_copy2d.restype = ctypes.c_int
+    L.wt_batch_stream.argtypes = [_vp, ctypes.POINTER(ctypes.c_void_p)]
+    L.wt_batch_stream.restype = ctypes.c_int
     L.wt_dev_copy.argtypes = [_vp, _vp, ctypes.c_size_t]
     L.wt_dev_copy.restype = ctypes.c_int
     L.wt_solver_apply.argtypes = [_vp, i32, _vp, _vp, i32]
@@ -150,6 +154,9 @@ def _declare(L):
     L.wt_planewave_layered.restype = ctypes.c_int
     L.wt_solver_scattered_source.argtypes = [_vp, _vp, _vp, _vp, i32, i32, _vp, _vp, _vp]
     L.wt_solver_scattered_source.restype = ctypes.c_int
+    L.wt_scattered_source_grid.argtypes = [ctypes.POINTER(WtGridParams)] + [_vp] * 6 + [i32, i32, _vp, i64, _vp,
+                                                                                _vp, i64, _vp]
+    L.wt_scattered_source_grid.restype = ctypes.c_int
     L.wt_total_field.argtypes = [_vp, _vp, i64, _vp]
     L.wt_total_field.restype = ctypes.c_int
     L.wt_solver_stream.argtypes = [_vp, ctypes.POINTER(ctypes.c_void_p)]

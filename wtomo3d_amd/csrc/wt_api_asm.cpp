@@ -183,6 +183,16 @@ int wt_d2h(void* dst, const void* d_src, size_t bytes)
     WT_HIP(hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost));
     return WT_OK;
 }
+int wt_dev_copy2d(void* d_dst, size_t dpitch, const void* d_src, size_t spitch, size_t width, size_t rows,
+                  void* stream)
+{
+    if (!d_dst || !d_src || width > dpitch || width > spitch) return WT_ERR_ARG;
+    if (!width || !rows) return WT_OK;
+    const hipStream_t st = (hipStream_t)stream;
+    WT_HIP(hipMemcpy2DAsync(d_dst, dpitch, d_src, spitch, width, rows, hipMemcpyDeviceToDevice, st));
+    WT_HIP(hipStreamSynchronize(st));
+    return WT_OK;
+}
 int wt_dev_sync(void)
 {
     WT_HIP(hipDeviceSynchronize());

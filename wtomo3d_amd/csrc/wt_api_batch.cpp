@@ -431,6 +431,13 @@ int wt_batch_offsets(const wt_batch* b, int64_t* unk_off, int32_t* group_of)
     return WT_OK;
 }
 
+int wt_batch_stream(const wt_batch* b, void** stream)
+{
+    if (!b || !stream || b->groups.empty()) return WT_ERR_ARG;
+    *stream = (void*)b->groups[0]->s.stream;
+    return WT_OK;
+}
+
 int wt_batch_set_graphs(wt_batch* b, int enable)
 {
     if (!b) return WT_ERR_ARG;

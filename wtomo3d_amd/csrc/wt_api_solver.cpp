@@ -503,21 +503,11 @@ int wt_solver_scattered_source(wt_solver* h, const void* lam_bg, const void* mu_
     rc = wt_launch_values<double>(s.last_p, lb, mb, rb, ab, s.d_err, s.stream);
     const cpx<double>* u = (const cpx<double>*)d_ubg;
     cpx<double>* out = (cpx<double>*)d_sfld;
+    // column groups of at most 64 (one lane per column), strided in the block
     for (int c0 = 0; !rc && c0 < nrhs; c0 += 64) {
         const int nc = nrhs - c0 < 64 ? nrhs - c0 : 64;
-        if (nc == nrhs) {
-            rc = wt_launch_scatsrc(s.rowptr, s.colidx, s.vals, ab, u, sf, out, s.n, nc, s.stream);
-        } else {
-            // column groups of a wider block go through the refinement buffers
-            if ((rc = ensure_ir(s, 64))) break;
-            if (hipMemcpy2DAsync(s.xb, nc * 16, u + c0, (size_t)nrhs * 16, nc * 16, s.n, hipMemcpyDeviceToDevice,
-                                 s.stream) != hipSuccess) { rc = WT_ERR_HIP; break; }
-            rc = wt_launch_scatsrc(s.rowptr, s.colidx, s.vals, ab, s.xb, sf ? sf + c0 : nullptr, s.rb, s.n, nc,
-                                   s.stream);
-            if (!rc && hipMemcpy2DAsync(out + c0, (size_t)nrhs * 16, s.rb, nc * 16, nc * 16, s.n,
-                                        hipMemcpyDeviceToDevice, s.stream) != hipSuccess)
-                rc = WT_ERR_HIP;
-        }
+        rc = wt_launch_scatsrc(s.rowptr, s.colidx, s.vals, ab, u + c0, sf ? sf + c0 : nullptr, out + c0, s.n, nc,
+                               s.stream, nrhs, nrhs);
     }
     int he = 0;
     if (!rc && (hipMemcpyAsync(&he, s.d_err, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||

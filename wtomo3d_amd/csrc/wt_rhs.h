@@ -15,9 +15,10 @@ template <typename RP>
 int wt_launch_expand(cpx<double>* d_x, int nx, int ny, int nz, double dx, int nrhs, const cpx<RP>* d_u,
                      const cpx<RP>* d_w, const double* d_kh, const double* d_csphi, hipStream_t s);
 
+// u, out row-major with row strides ldu, ldo (<= 0: nrhs)
 int wt_launch_scatsrc(const int32_t* rowptr, const int32_t* colidx, const cpx<double>* a, const cpx<double>* abg,
                       const cpx<double>* u, const cpx<double>* srcfn, cpx<double>* out, int64_t n, int nrhs,
-                      hipStream_t s);
+                      hipStream_t s, int64_t ldu = 0, int64_t ldo = 0);
 
 int wt_launch_add(cpx<double>* y, const cpx<double>* x, int64_t n, hipStream_t s);
 

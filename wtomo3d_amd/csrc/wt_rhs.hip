@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256) void expand_kernel(cpx<double>* __restrict__ x
 
 // sfld = srcfn_c sum_e (Abg_e - A_e) u_bg[col_e]: one wave per matrix row,
 // lane = right-hand side, rows spread over the XCDs in contiguous ranges.
+// u and out are row-major with row strides ldu and ldo (>= nrhs).
 constexpr int SS_ROWS = 4;
 
 __global__ __launch_bounds__(256) void scatsrc_kernel(const int32_t* __restrict__ rowptr,
@@ -245,7 +246,8 @@ __global__ __launch_bounds__(256) void scatsrc_kernel(const int32_t* __restrict_
                                                       const cpx<double>* __restrict__ abg,
                                                       const cpx<double>* __restrict__ u,
                                                       const cpx<double>* __restrict__ srcfn,
-                                                      cpx<double>* __restrict__ out, int64_t n, int nrhs)
+                                                      cpx<double>* __restrict__ out, int64_t n, int nrhs,
+                                                      int64_t ldu, int64_t ldo)
 {
     const int wave = threadIdx.x >> 6, c = threadIdx.x & 63;
     int64_t blk = blockIdx.x;
@@ -259,12 +261,12 @@ __global__ __launch_bounds__(256) void scatsrc_kernel(const int32_t* __restrict_
     const int32_t e1 = rowptr[row + 1];
     for (int32_t e = rowptr[row]; e < e1; ++e) {
         const cpx<double> d(abg[e].re - a[e].re, abg[e].im - a[e].im);
-        const cpx<double> xv = u[(int64_t)colidx[e] * nrhs + c];
+        const cpx<double> xv = u[(int64_t)colidx[e] * ldu + c];
         ar += d.re * xv.re - d.im * xv.im;
         ai += d.re * xv.im + d.im * xv.re;
     }
     const cpx<double> f = srcfn ? srcfn[c] : cpx<double>(1.0, 0.0);
-    out[row * nrhs + c] = cpx<double>(f.re * ar - f.im * ai, f.re * ai + f.im * ar);
+    out[row * ldo + c] = cpx<double>(f.re * ar - f.im * ai, f.re * ai + f.im * ar);
 }
 
 // out[r, j] = sum_e val_e X[col_e, j] for a short list of sparse rows (the
@@ -329,12 +331,14 @@ int wt_launch_expand(cpx<double>* d_x, int nx, int ny, int nz, double dx, int nr
 
 int wt_launch_scatsrc(const int32_t* rowptr, const int32_t* colidx, const cpx<double>* a, const cpx<double>* abg,
                       const cpx<double>* u, const cpx<double>* srcfn, cpx<double>* out, int64_t n, int nrhs,
-                      hipStream_t s)
+                      hipStream_t s, int64_t ldu, int64_t ldo)
 {
-    if (nrhs < 1 || nrhs > 64) return WT_ERR_ARG;
+    if (ldu <= 0) ldu = nrhs;
+    if (ldo <= 0) ldo = nrhs;
+    if (nrhs < 1 || nrhs > 64 || ldu < nrhs || ldo < nrhs) return WT_ERR_ARG;
     const int64_t blocks = (n + SS_ROWS - 1) / SS_ROWS;
     hipLaunchKernelGGL(scatsrc_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rowptr, colidx, a, abg, u, srcfn, out,
-                       n, nrhs);
+                       n, nrhs, ldu, ldo);
     WT_HIP(hipGetLastError());
     return WT_OK;
 }

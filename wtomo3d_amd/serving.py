@@ -19,6 +19,13 @@ failed request, dropped after its request when factors are not kept, or at
 ``close()`` -- and its bytes are released then, so ``used`` always equals
 the bytes of the entries in the cache.
 
+Shared launches.  Pending requests for new small models (at most
+``group_nodes`` grid nodes) are admitted together, up to ``group_max`` of
+them: their models are entered into one ``batch.Batch``, factored level by
+level in the same launches, and their right-hand sides solved in one
+refined block solve.  The batch stays cached under every member's key, so a
+later request for any of them reuses it.
+
 Each request solves the reference's forward problem for plane-wave sources
 on its model: a heterogeneous model over a layered background; Thomson-
 Haskell profiles of the background (HASKGRN) expanded to 3-D background
@@ -47,6 +54,8 @@ from . import media, rhs
 from ._lib import lib, check, WtSolveStats, grid_params, NotConverged, WT_ERR_NOCONV
 from .solver import Solver
 
+_vp = ctypes.c_void_p
+
 
 class Request:
     """One solve request.  ``project`` (a path prefix naming ``name.ini`` and
@@ -62,6 +71,7 @@ class Request:
         self.arrival_abs = None
         self.stats = None
         self.reused = False
+        self.batched = False  # solved in a batch shared with other models
         self.x = None
         self.error = None
 
@@ -73,14 +83,26 @@ class Request:
 
 
 class _Entry:
-    """One cached factor: a solver handle and the bytes reserved for it."""
+    """One cached factor and the bytes reserved for it: a solver handle, or
+    a batch of several models (``batch``, with ``index`` mapping each
+    member's key to its model and ``problems`` the members' problems)."""
 
-    def __init__(self, nbytes):
+    def __init__(self, nbytes, group=False):
         self.solver = None
+        self.batch = None
+        self.group = group
+        self.index = {}
+        self.problems = []
         self.nbytes = nbytes
         self.busy = True     # a request owns it (factoring or solving)
         self.ready = False   # factored
         self.last = time.time()
+
+    def close(self):
+        for h in (self.solver, self.batch):
+            if h is not None:
+                h.close()
+        self.solver = self.batch = None
 
 
 class _Problem:
@@ -103,9 +125,11 @@ class _Problem:
 
 class Engine:
     def __init__(self, hbm_budget=200e9, slots=4, tol=1e-8, max_iter=30, keep_factors=True,
-                 return_solutions=False, lookahead=8):
+                 return_solutions=False, lookahead=8, group_max=8, group_nodes=120_000, report_every=0):
         self.budget, self.slots, self.tol, self.max_iter = hbm_budget, slots, tol, max_iter
         self.keep, self.return_x, self.lookahead = keep_factors, return_solutions, lookahead
+        self.group_max, self.group_nodes = group_max, group_nodes
+        self.report_every = report_every  # print a line every this many finished requests
         self.used = 0
         self.cache = collections.OrderedDict()  # Request.key() -> _Entry
         self.pending = collections.deque()
@@ -113,6 +137,9 @@ class Engine:
         self.closed = False
         self.done = []
         self.factorizations = 0
+        self.groups = 0
+        self.peak_used = 0
+        self.backlog_at_last_arrival = None
         self._problems = {}
         self._plock = threading.Lock()
         self._est = {}
@@ -203,6 +230,79 @@ class Engine:
         if self.return_x:
             r.x = x.to_host(np.complex128, (n, r.nrhs))
 
+    def _factor_group(self, rs, probs):
+        """One batch holding the complex64 factors of several requests'
+        operators, factored level by level together (jobs 1 and 2 for all)."""
+        from .batch import Batch
+        b = Batch([p.dims for p in probs])
+        try:
+            b.set_models([p.model[0] for p in probs], [p.model[1] for p in probs], [p.model[2] for p in probs])
+            b.assemble([p.params(r.freq) for r, p in zip(rs, probs)])
+            b.factor(32)
+        except Exception:
+            b.close()
+            raise
+        return b
+
+    def _solve_group(self, rs, e, slot):
+        """Sources of every request in ``rs`` (models of the batch entry
+        ``e``) written into one block, one refined batch solve (the models
+        not in ``rs`` masked), then each request's total field."""
+        L = lib()
+        b = e.batch
+        st_ = ctypes.c_void_p()
+        check(L.wt_batch_stream(b._h, ctypes.byref(st_)), "wt_batch_stream")
+        nmax = max(r.nrhs for r in rs)
+        B, X = rhs.DeviceBuffer(b.n * nmax * 16), rhs.DeviceBuffer(b.n * nmax * 16)
+        ncol = np.zeros(b.m, np.int32)
+        own = []
+        P = lambda a: a.ctypes.data_as(_vp)  # noqa: E731
+        try:
+            for r in rs:
+                k = e.index[r.key()]
+                prob = e.problems[k]
+                nx, ny, nz = prob.dims
+                n = 3 * nx * ny * nz
+                ain, phi, is_s = self.sources(r)
+                prof = rhs.profiles(prob.layers, ain, is_s, 2 * np.pi * r.freq, nz, prob.dx, 0.0, rhs.WT_PREC_F64,
+                                    stream=st_)
+                ub = rhs.background_fields(nx, ny, nz, prob.dx, prof, phi, stream=st_)
+                for buf in (prof.U, prof.W, prof.kh):
+                    buf.free()
+                own.append((r, k, n, ub))
+                pr = prob.params(r.freq)
+                (lam, mu, rho), (lb, mb, rb) = prob.model, prob.bg
+                dst = _vp(B.ptr.value + int(b.offsets[k]) * nmax * 16)
+                check(L.wt_scattered_source_grid(ctypes.byref(pr), P(lam), P(mu), P(rho), P(lb), P(mb), P(rb), 0,
+                                                 r.nrhs, ub.ptr, r.nrhs, None, dst, nmax, st_),
+                      "wt_scattered_source_grid")
+                ncol[k] = r.nrhs
+            stats = (WtSolveStats * b.m)()
+            rc = L.wt_batch_solve(b._h, 32, nmax, P(ncol), B.ptr, X.ptr, 1, self.tol, self.max_iter, stats)
+            if rc != WT_ERR_NOCONV:
+                check(rc, "wt_batch_solve")
+            for r, k, n, ub in own:
+                st = stats[k]
+                r.stats = {f: getattr(st, f) for f, _ in WtSolveStats._fields_}
+                if not st.converged:
+                    r.error = "request %d: refinement stopped at %.3e > %.1e after %d solves" % (
+                        r.rid, st.rel_residual, self.tol, st.iterations)
+                    continue
+                x = rhs.DeviceBuffer(n * r.nrhs * 16)
+                try:
+                    check(L.wt_dev_copy2d(x.ptr, r.nrhs * 16, _vp(X.ptr.value + int(b.offsets[k]) * nmax * 16),
+                                          nmax * 16, r.nrhs * 16, n, st_), "wt_dev_copy2d")
+                    rhs.total_field(x, ub, n * r.nrhs, stream=st_)
+                    if self.return_x:
+                        r.x = x.to_host(np.complex128, (n, r.nrhs))
+                finally:
+                    x.free()
+        finally:
+            for _, _, _, ub in own:
+                ub.free()
+            B.free()
+            X.free()
+
     def _slot(self):
         """Per-slot device buffers that only grow: solution, source and
         background blocks, then the two profiles and the wavenumbers.
@@ -233,20 +333,33 @@ class Engine:
             self.pending.append(req)
             self.cv.notify_all()
 
-    def _drop(self, key, e):
-        """Remove an entry from the cache and release its bytes (once; the
-        caller holds the lock and the entry is not in use by anyone else)."""
-        if self.cache.get(key) is e:
-            del self.cache[key]
-        if e.solver is not None:
-            e.solver.close()
-            e.solver = None
+    def _drop(self, e):
+        """Remove an entry (under every key it serves) from the cache and
+        release its bytes -- once; the caller holds the lock and no request
+        is using the entry."""
+        for k in [k for k, v in self.cache.items() if v is e]:
+            del self.cache[k]
+        e.close()
         self.used -= e.nbytes
         e.nbytes = 0
 
+    def _small(self, r):
+        return self.group_max > 1 and r.dims[0] * r.dims[1] * r.dims[2] <= self.group_nodes
+
+    def _reserve(self, need):
+        """Evict idle entries, least recently used first, until ``need``
+        more bytes fit; True when they do."""
+        while self.used + need > self.budget:
+            victim = next((v for v in self.cache.values() if not v.busy), None)
+            if victim is None:
+                break
+            self._drop(victim)
+        return self.used + need <= self.budget
+
     def _admit(self):
-        """Pick a pending request that can run now and the cache entry it
-        runs on (caller holds the lock)."""
+        """Pick pending requests that can run now and the cache entry they
+        run on (caller holds the lock): a request on a cached factor, a
+        group of new small models sharing one batch, or one new model."""
         idx = 0
         while idx < min(self.lookahead, len(self.pending)):
             r = self.pending[idx]
@@ -259,7 +372,7 @@ class Engine:
                 e.busy = True
                 self.cache.move_to_end(key)
                 del self.pending[idx]
-                return r, e
+                return [r], e
             need = self._estimate(r.dims)
             if need > self.budget:
                 # can never fit: fail it instead of waiting forever
@@ -269,17 +382,32 @@ class Engine:
                 self.done.append(r)
                 self.cv.notify_all()
                 continue
-            while self.used + need > self.budget:
-                victim = next((k for k, v in self.cache.items() if not v.busy), None)
-                if victim is None:
-                    break
-                self._drop(victim, self.cache[victim])
-            if self.used + need <= self.budget:
+            if self._small(r):
+                group, keys = [r], {key}
+                for q in list(self.pending)[idx + 1:self.lookahead]:
+                    kq = q.key()
+                    if kq not in keys and kq not in self.cache and self._small(q):
+                        group.append(q)
+                        keys.add(kq)
+                        if len(group) == self.group_max:
+                            break
+                if len(group) > 1:
+                    gneed = sum(self._estimate(q.dims) for q in group)
+                    if gneed <= self.budget and self._reserve(gneed):
+                        e = _Entry(gneed, group=True)
+                        for q in group:
+                            self.cache[q.key()] = e
+                            self.pending.remove(q)
+                        self.used += gneed
+                        self.peak_used = max(self.peak_used, self.used)
+                        return group, e
+            if self._reserve(need):
                 e = _Entry(need)
                 self.cache[key] = e
                 self.used += need
+                self.peak_used = max(self.peak_used, self.used)
                 del self.pending[idx]
-                return r, e
+                return [r], e
             idx += 1
         return None, None
 
@@ -287,32 +415,58 @@ class Engine:
         while True:
             with self.cv:
                 while True:
-                    r, e = self._admit()
-                    if r is not None or (self.closed and not self.pending):
+                    rs, e = self._admit()
+                    if rs is not None or (self.closed and not self.pending):
                         break
                     self.cv.wait()
-                if r is None:
+                if rs is None:
                     return
-            r.start = time.perf_counter()
+            t0 = time.perf_counter()
+            for r in rs:
+                r.start = t0
+                r.batched = e.group
             try:
-                prob = self._problem(r)
-                if not e.ready:
-                    e.solver = self._factor(r, prob)
-                    e.ready = True
-                    self.factorizations += 1
+                if e.group:
+                    if not e.ready:
+                        probs = [self._problem(r) for r in rs]
+                        e.batch = self._factor_group(rs, probs)
+                        e.problems = probs
+                        e.index = {r.key(): k for k, r in enumerate(rs)}
+                        e.ready = True
+                        self.factorizations += len(rs)
+                        self.groups += 1
+                    else:
+                        for r in rs:
+                            r.reused = True
+                    self._solve_group(rs, e, slot)
                 else:
-                    r.reused = True
-                self._solve(r, e.solver, prob, slot)
+                    r = rs[0]
+                    prob = self._problem(r)
+                    if not e.ready:
+                        e.solver = self._factor(r, prob)
+                        e.ready = True
+                        self.factorizations += 1
+                    else:
+                        r.reused = True
+                    self._solve(r, e.solver, prob, slot)
             except Exception as ex:  # reported per request
-                r.error = str(ex)
-            r.end = time.perf_counter()
+                for r in rs:
+                    if r.error is None:
+                        r.error = str(ex)
+            t1 = time.perf_counter()
             with self.cv:
+                for r in rs:
+                    r.end = t1
                 e.busy = False
                 e.last = time.time()
-                if r.error is not None or not self.keep:
+                if any(r.error is not None for r in rs) or not self.keep:
                     # a failed request's factor is not trusted by later ones
-                    self._drop(r.key(), e)
-                self.done.append(r)
+                    self._drop(e)
+                before = len(self.done)
+                self.done.extend(rs)
+                if self.report_every and len(self.done) // self.report_every > before // self.report_every:
+                    print("serving: %d requests finished, %d pending, %.0f GB reserved"
+                          % (len(self.done), len(self.pending), self.used / 1e9), flush=True)
                 self.cv.notify_all()
 
     def run(self, requests, realtime=True):
@@ -338,6 +492,8 @@ class Engine:
                 with self.cv:
                     self.done.append(r)
         with self.cv:
+            # requests arrived but not yet finished when the last one arrives
+            self.backlog_at_last_arrival = sum(1 for r in requests if r.end is None and r.error is None)
             self.closed = True
             self.cv.notify_all()
         for t in threads:
@@ -348,14 +504,15 @@ class Engine:
         return wall
 
     def accounted(self):
-        """True when the reserved bytes equal those of the cached entries."""
+        """True when the reserved bytes equal those of the cached entries
+        (an entry serving several keys counted once)."""
         with self.cv:
-            return self.used == sum(e.nbytes for e in self.cache.values())
+            return self.used == sum(e.nbytes for e in {id(v): v for v in self.cache.values()}.values())
 
     def close(self):
         with self.cv:
-            for key in list(self.cache):
-                self._drop(key, self.cache[key])
+            for e in list({id(v): v for v in self.cache.values()}.values()):
+                self._drop(e)
             self.cache.clear()
 
 
@@ -404,4 +561,71 @@ def latency_report(reqs, wall):
             "p50_service_ms": float(np.percentile(svc, 50)) if len(svc) else None,
             "rhs_per_s": rhs_ / wall, "wall_s": wall,
             "factor_reuse": sum(1 for r in ok if r.reused),
+            "batched": sum(1 for r in ok if r.batched),
             "max_rel_residual": max((r.stats["rel_residual"] for r in ok), default=None)}
+
+
+def run_stream(rate, seconds, seed=1005, slots=4, hbm_budget=200e9, side=(32, 80), report_every=20, **engine_kw):
+    """One realtime run of the serving workload (``poisson_stream``) at
+    ``rate`` requests/s for ``seconds`` of arrivals.  Returns the latency
+    report plus the backlog when the last request arrived (requests arrived
+    and not finished), the p50 latency of the first and last third of the
+    arrivals, the peak reserved HBM, the factorizations and the batched
+    groups."""
+    reqs = poisson_stream(rate, seconds, seed=seed, side=side, freq=(0.05, 0.5), nrhs=(1, 64), reuse=0.3, dx=200.0)
+    eng = Engine(hbm_budget=hbm_budget, slots=slots, tol=1e-8, max_iter=30, report_every=report_every, **engine_kw)
+    wall = eng.run(reqs, realtime=True)
+    eng.close()
+    rep = latency_report(reqs, wall)
+    ok = sorted((r for r in reqs if r.error is None), key=lambda r: r.arrival)
+    third = max(len(ok) // 3, 1)
+    lat = lambda rs: float(np.percentile([(r.end - r.arrival_abs) * 1e3 for r in rs], 50)) if rs else None  # noqa
+    rep.update(rate_per_s=rate, window_s=seconds, slots=slots, sides="%d-%d" % side,
+               backlog_at_last_arrival=eng.backlog_at_last_arrival, p50_first_third_ms=lat(ok[:third]),
+               p50_last_third_ms=lat(ok[-third:]), peak_reserved_gb=eng.peak_used / 1e9,
+               factorizations=eng.factorizations, groups=eng.groups,
+               completed_per_s=len(ok) / wall if wall > 0 else None)
+    first, last = rep["p50_first_third_ms"], rep["p50_last_third_ms"]
+    rep["sustained"] = bool(rep["errors"] == 0 and rep["backlog_at_last_arrival"] is not None and
+                            rep["backlog_at_last_arrival"] <= 2 * slots and first and last and last <= 2.0 * first)
+    return rep
+
+
+def sustained_rate(rates, seconds, **kw):
+    """``run_stream`` at each rate in ascending order (stopping after the
+    first rate that is not sustained); returns (highest sustained rate or
+    None, [reports])."""
+    best, reps = None, []
+    for rate in sorted(rates):
+        rep = run_stream(rate, seconds, **kw)
+        reps.append(rep)
+        print("serving: %.2f req/s -> p50 %.0f ms, p99 %.0f ms, backlog %s, sustained %s"
+              % (rate, rep["p50_latency_ms"] or -1, rep["p99_latency_ms"] or -1, rep["backlog_at_last_arrival"],
+                 rep["sustained"]), flush=True)
+        if not rep["sustained"]:
+            break
+        best = rate
+    return best, reps
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="serving workload: sustained-rate search and bounded bursts")
+    ap.add_argument("--rates", default="1,2,3", help="comma-separated request rates to try, ascending")
+    ap.add_argument("--seconds", type=float, default=150.0, help="arrival window per rate")
+    ap.add_argument("--burst-rate", type=float, default=20.0)
+    ap.add_argument("--burst-seconds", type=float, default=0.0, help="also run a bounded window at --burst-rate")
+    ap.add_argument("--group-max", type=int, default=8)
+    ap.add_argument("--out", default=None, help="write the reports as JSON here")
+    a = ap.parse_args()
+    best, reps = sustained_rate([float(v) for v in a.rates.split(",")], a.seconds, group_max=a.group_max)
+    out = {"sustained_rate_per_s": best, "runs": reps}
+    if a.burst_seconds > 0:
+        out["burst"] = run_stream(a.burst_rate, a.burst_seconds, group_max=a.group_max)
+        print("serving: burst %.1f req/s for %.1f s -> %s" % (a.burst_rate, a.burst_seconds,
+                                                            json.dumps(out["burst"])), flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(out, f, indent=1)
+    print(json.dumps(out))
