@@ -164,6 +164,22 @@ int wt_solver_get_values(wt_solver *h, void *vals);
  * (64): replaces STRUMPACK_factor (strumpack_finter.c:84).  Returns
  * WT_ERR_PIVOT when a diagonal tile had an exactly zero pivot. */
 int wt_solver_factor(wt_solver *h, int factor_prec);
+
+/* Pivot report of the last factorization in factor_prec.  Pivoting is
+ * partial within 64-row diagonal tiles; magnitudes are |re| + |im| and each
+ * ratio is taken against the tile's largest entry when its elimination
+ * starts (after the updates of the earlier pivots).  A small pivot ratio or a
+ * large growth flags a factor that refinement may not recover.  No
+ * counterpart in the reference: STRUMPACK's shim passes only a failure code
+ * to ierr (strumpack_finter.c:103-111).  WT_ERR_STATE before a
+ * factorization. */
+typedef struct wt_pivot_report {
+    int32_t zero_pivots;     /* 1: an exactly zero pivot was replaced (the factorization gave WT_ERR_PIVOT) */
+    int32_t tiny_pivots;     /* pivots with |d| <= 2^-12 (fp32) or 2^-26 (fp64) times max|tile| */
+    double min_pivot_ratio;  /* min over pivots of |d| / max|tile| */
+    double max_growth;       /* max over tiles of max|U| / max|tile| */
+} wt_pivot_report;
+int wt_solver_pivot_report(const wt_solver *h, int factor_prec, wt_pivot_report *out);
 /* Tile-shape autotuner of the factorization: factors once per GEMM tile
  * variant with every launch timed, then keeps each launch's fastest variant
  * for the later factorizations of this handle (the handle is left

@@ -238,6 +238,8 @@ def test_singular_operator_reports_zero_pivot(prec):
     v[ir[r]:ir[r + 1]] = 0
     s.set_values(v)
     assert W.lib().wt_solver_factor(s._h, prec) == 7
+    rep = s.pivot_report(prec)
+    assert rep["zero_pivots"] == 1 and rep["tiny_pivots"] >= 1 and rep["min_pivot_ratio"] == 0.0
     s.set_values(np.zeros_like(vals))
     assert W.lib().wt_solver_factor(s._h, prec) == 7
     s.set_values(vals)
@@ -245,6 +247,54 @@ def test_singular_operator_reports_zero_pivot(prec):
     B, Xt = rhs_block(A, 2, 31)
     X, st = s.solve(B, prec=prec, tol=1e-10, max_iter=30)
     assert st["rel_residual"] <= 1e-10
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_near_singular_operator_pivot_report(prec):
+    """The v-row of an interior node replaced by its u-row plus 1e-10 of the
+    original v-row (the two rows share their columns): near-singular, not
+    singular, with the near-dependence kept through rounding to complex64
+    (the smallest singular value is the rounding of the copied row, about
+    1e-7 of the row, where an eigenvalue shift of this non-normal operator
+    is moved far off by the same rounding).  The factorization succeeds,
+    and its pivot report flags tiny pivots and a least pivot ratio far below
+    the unmodified operator's, which has none.  The complex64 factor shares
+    the near-dependence, so refinement of a consistent block still reaches
+    1e-8; what it reports is checked against the host's FP64 residual, and
+    a tolerance below its reach ends with NotConverged (converged = 0)
+    carrying the residual it did reach."""
+    from wtomo3d_amd._lib import NotConverged
+    nx, ny, nz = 6, 5, 7
+    s, A, vals = _setup(nx, ny, nz, 64, seed=9)
+    s.factor(prec)
+    reg = s.pivot_report(prec)
+    node = 2 + 2 * nz + 3 * nx * nz  # interior: z = 3, x = 3, y = 4 (1-based)
+    r1, r2 = 3 * node, 3 * node + 1
+    ip = A.indptr
+    assert np.array_equal(A.indices[ip[r1]:ip[r1 + 1]], A.indices[ip[r2]:ip[r2 + 1]])
+    v = vals.copy()
+    v[ip[r2]:ip[r2 + 1]] = vals[ip[r1]:ip[r1 + 1]] + 1e-10 * vals[ip[r2]:ip[r2 + 1]]
+    s.set_values(v)
+    s.factor(prec)
+    near = s.pivot_report(prec)
+    print("[near-singular] prec %d regular %s shifted %s" % (prec, reg, near), flush=True)
+    assert reg["zero_pivots"] == 0 and reg["tiny_pivots"] == 0
+    assert near["zero_pivots"] == 0 and near["tiny_pivots"] >= 1
+    assert near["min_pivot_ratio"] < 1e-3 * reg["min_pivot_ratio"]
+    if prec == 32:
+        A2 = SO.csr(A.indptr, A.indices, v, base=0)
+        B, _ = rhs_block(A2, 2, 61)
+        X, st = s.solve(B, prec=32, tol=1e-8, max_iter=20)
+        r = SO.rel_residual(A2, X, B)
+        assert st["converged"] == 1 and r <= 1e-8
+        assert abs(r - st["rel_residual"]) <= 1e-6 * r
+        with pytest.raises(NotConverged) as ei:
+            s.solve(B, prec=32, tol=1e-17, max_iter=20)
+        st = ei.value.stats
+        r = SO.rel_residual(A2, ei.value.x, B)
+        assert st["converged"] == 0 and st["rel_residual"] > 1e-17
+        assert abs(r - st["rel_residual"]) <= 1e-6 * r
     s.close()
 
 

@@ -50,6 +50,12 @@ class WtSolveStats(ctypes.Structure):
                 ("converged", ctypes.c_int32), ("stalled", ctypes.c_int32)]
 
 
+class WtPivotReport(ctypes.Structure):
+    """Mirror of ``wt_pivot_report``."""
+    _fields_ = [("zero_pivots", ctypes.c_int32), ("tiny_pivots", ctypes.c_int32),
+                ("min_pivot_ratio", ctypes.c_double), ("max_growth", ctypes.c_double)]
+
+
 _LIB = None
 _vp = ctypes.c_void_p
 
@@ -113,6 +119,8 @@ def _declare(L):
     L.wt_batch_run.argtypes = [_vp, i32, i32, _vp, _vp, _vp, i32, ctypes.c_double, i32, pstats]
     L.wt_solver_autotune.argtypes = [_vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.wt_solver_autotune.restype = ctypes.c_int
+    L.wt_solver_pivot_report.argtypes = [_vp, i32, ctypes.POINTER(WtPivotReport)]
+    L.wt_solver_pivot_report.restype = ctypes.c_int
     L.wt_solver_get_tiles.argtypes = [_vp, i32, i32, i32, _vp, ctypes.POINTER(ctypes.c_int64)]
     L.wt_solver_get_tiles.restype = ctypes.c_int
     L.wt_solver_set_tiles.argtypes = [_vp, i32, i32, i32, _vp, i64]

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <vector>
 #include "wt_common.h"
+#include "wt_rt.h"
 
 int64_t wt_nnz(int nx, int ny, int nz);
 int wt_launch_pattern(int nx, int ny, int nz, int base, int32_t* d_rowptr, int32_t* d_colidx,
@@ -25,8 +26,8 @@ bool valid = false;
 namespace {
 struct DevBuf {
     void* p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    int alloc(size_t n) { return hipMalloc(&p, n ? n : 1) == hipSuccess ? WT_OK : WT_ERR_NOMEM; }
+    ~DevBuf() { if (p) (void)rt_free(p); }
+    int alloc(size_t n) { return rt_malloc(&p, n ? n : 1) == hipSuccess ? WT_OK : WT_ERR_NOMEM; }
 };
 }  // namespace
 
@@ -40,19 +41,21 @@ int64_t wt_graph_nnz(int nx, int ny, int nz)
 
 int wt_graph_dev(int nx, int ny, int nz, int base, int32_t* d_rowptr, int32_t* d_colidx, void* stream)
 {
+    wt_rt_stream_guard g(stream);
     return wt_launch_pattern(nx, ny, nz, base, d_rowptr, d_colidx, (hipStream_t)stream);
 }
 
 int wt_graph(int nx, int ny, int nz, int base, int32_t* rowptr, int32_t* colidx)
 {
     if (nx < 1 || ny < 1 || nz < 1 || !rowptr || !colidx) return WT_ERR_ARG;
+    wt_rt_shared g;  // null-stream launch
     int64_t n = 3LL * nx * ny * nz, nnz = wt_nnz(nx, ny, nz);
     DevBuf r, c;
     if (r.alloc((n + 1) * 4) || c.alloc(nnz * 4)) return WT_ERR_NOMEM;
     int rc = wt_launch_pattern(nx, ny, nz, base, (int32_t*)r.p, (int32_t*)c.p, 0);
     if (rc) return rc;
-    WT_HIP(hipMemcpy(rowptr, r.p, (n + 1) * 4, hipMemcpyDeviceToHost));
-    WT_HIP(hipMemcpy(colidx, c.p, nnz * 4, hipMemcpyDeviceToHost));
+    WT_HIP(rt_memcpy(rowptr, r.p, (n + 1) * 4, hipMemcpyDeviceToHost));
+    WT_HIP(rt_memcpy(colidx, c.p, nnz * 4, hipMemcpyDeviceToHost));
     return WT_OK;
 }
 
@@ -60,6 +63,7 @@ int wt_assemble_dev(const wt_grid_params* p, int prec, const void* d_lam, const 
                     const void* d_rho, void* d_vals, int* d_err, void* stream)
 {
     if (!p) return WT_ERR_ARG;
+    wt_rt_stream_guard g(stream);
     if (prec == WT_PREC_F32)
         return wt_launch_values<float>(*p, d_lam, d_mu, d_rho, d_vals, d_err, (hipStream_t)stream);
     if (prec == WT_PREC_F64)
@@ -72,21 +76,22 @@ int wt_assemble(const wt_grid_params* p, int prec, const void* lam, const void* 
 {
     if (!p || !lam || !mu || !rho || !vals) return WT_ERR_ARG;
     if (prec != WT_PREC_F32 && prec != WT_PREC_F64) return WT_ERR_ARG;
+    wt_rt_shared g;  // null-stream launch
     int64_t nodes = (int64_t)p->nx * p->ny * p->nz, nnz = wt_nnz(p->nx, p->ny, p->nz);
     size_t rs = prec == WT_PREC_F32 ? 4 : 8;
     DevBuf l, m, r, v, e;
     if (l.alloc(nodes * 2 * rs) || m.alloc(nodes * 2 * rs) || r.alloc(nodes * rs) ||
         v.alloc(nnz * 2 * rs) || e.alloc(sizeof(int)))
         return WT_ERR_NOMEM;
-    WT_HIP(hipMemcpy(l.p, lam, nodes * 2 * rs, hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(m.p, mu, nodes * 2 * rs, hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(r.p, rho, nodes * rs, hipMemcpyHostToDevice));
-    WT_HIP(hipMemset(e.p, 0, sizeof(int)));
+    WT_HIP(rt_memcpy(l.p, lam, nodes * 2 * rs, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(m.p, mu, nodes * 2 * rs, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(r.p, rho, nodes * rs, hipMemcpyHostToDevice));
+    WT_HIP(rt_memset(e.p, 0, sizeof(int)));
     int rc = wt_assemble_dev(p, prec, l.p, m.p, r.p, v.p, (int*)e.p, nullptr);
     if (rc) return rc;
     int herr = 0;
-    WT_HIP(hipMemcpy(&herr, e.p, sizeof(int), hipMemcpyDeviceToHost));
-    WT_HIP(hipMemcpy(vals, v.p, nnz * 2 * rs, hipMemcpyDeviceToHost));
+    WT_HIP(rt_memcpy(&herr, e.p, sizeof(int), hipMemcpyDeviceToHost));
+    WT_HIP(rt_memcpy(vals, v.p, nnz * 2 * rs, hipMemcpyDeviceToHost));
     return herr ? WT_ERR_ARG : WT_OK;
 }
 
@@ -96,6 +101,7 @@ void wt_asmble(int lunlog, int nzero, int nx, int ny, int nz, const int32_t* irp
 {
     (void)lunlog;
     (void)jcptr;
+    wt_rt_shared g;  // null-stream launch
     *ierr = 0;
     int64_t n = 3LL * nx * ny * nz, nnz = wt_graph_nnz(nx, ny, nz);
     // Row-length check of asmble.f90:556-564 and total check :619-626.
@@ -126,19 +132,19 @@ void wt_asmble(int lunlog, int nzero, int nx, int ny, int nz, const int32_t* irp
         return;
     }
     if (wt_resident::d_vals && wt_resident::nnz != nnz) {
-        (void)hipFree(wt_resident::d_vals);
+        (void)rt_free(wt_resident::d_vals);
         wt_resident::d_vals = nullptr;
     }
-    if (!wt_resident::d_vals && hipMalloc(&wt_resident::d_vals, nnz * 8) != hipSuccess) {
+    if (!wt_resident::d_vals && rt_malloc(&wt_resident::d_vals, nnz * 8) != hipSuccess) {
         wt_resident::d_vals = nullptr;
         *ierr = 1;
         return;
     }
     wt_resident::valid = false;
-    if (hipMemcpy(l.p, da, nodes * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(m.p, mu, nodes * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(r.p, rho, nodes * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(e.p, 0, sizeof(int)) != hipSuccess) {
+    if (rt_memcpy(l.p, da, nodes * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        rt_memcpy(m.p, mu, nodes * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        rt_memcpy(r.p, rho, nodes * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        rt_memset(e.p, 0, sizeof(int)) != hipSuccess) {
         *ierr = 1;
         return;
     }
@@ -147,8 +153,8 @@ void wt_asmble(int lunlog, int nzero, int nx, int ny, int nz, const int32_t* irp
         return;
     }
     int herr = 0;
-    if (hipMemcpy(&herr, e.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(a, wt_resident::d_vals, nnz * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (rt_memcpy(&herr, e.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        rt_memcpy(a, wt_resident::d_vals, nnz * 8, hipMemcpyDeviceToHost) != hipSuccess) {
         *ierr = 1;
         return;
     }
@@ -166,21 +172,21 @@ void wt_asmble(int lunlog, int nzero, int nx, int ny, int nz, const int32_t* irp
 int wt_dev_alloc(size_t bytes, void** d_ptr)
 {
     if (!d_ptr) return WT_ERR_ARG;
-    return hipMalloc(d_ptr, bytes ? bytes : 1) == hipSuccess ? WT_OK : WT_ERR_NOMEM;
+    return rt_malloc(d_ptr, bytes ? bytes : 1) == hipSuccess ? WT_OK : WT_ERR_NOMEM;
 }
 int wt_dev_free(void* d_ptr)
 {
-    WT_HIP(hipFree(d_ptr));
+    WT_HIP(rt_free(d_ptr));
     return WT_OK;
 }
 int wt_h2d(void* d_dst, const void* src, size_t bytes)
 {
-    WT_HIP(hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(d_dst, src, bytes, hipMemcpyHostToDevice));
     return WT_OK;
 }
 int wt_d2h(void* dst, const void* d_src, size_t bytes)
 {
-    WT_HIP(hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost));
+    WT_HIP(rt_memcpy(dst, d_src, bytes, hipMemcpyDeviceToHost));
     return WT_OK;
 }
 int wt_dev_copy2d(void* d_dst, size_t dpitch, const void* d_src, size_t spitch, size_t width, size_t rows,
@@ -189,13 +195,14 @@ int wt_dev_copy2d(void* d_dst, size_t dpitch, const void* d_src, size_t spitch, 
     if (!d_dst || !d_src || width > dpitch || width > spitch) return WT_ERR_ARG;
     if (!width || !rows) return WT_OK;
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     WT_HIP(hipMemcpy2DAsync(d_dst, dpitch, d_src, spitch, width, rows, hipMemcpyDeviceToDevice, st));
     WT_HIP(hipStreamSynchronize(st));
     return WT_OK;
 }
 int wt_dev_sync(void)
 {
-    WT_HIP(hipDeviceSynchronize());
+    WT_HIP(rt_device_sync());
     return WT_OK;
 }
 int wt_device_count(void)

@@ -14,6 +14,7 @@
 #include <new>
 #include <vector>
 
+#include "wt_rt.h"
 #include "wt_solver.h"
 #include "wt_vec.h"
 
@@ -44,7 +45,7 @@ template <typename T>
 int dalloc(T** p, int64_t count)
 {
     *p = nullptr;
-    if (hipMalloc((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T)) != hipSuccess) {
+    if (rt_malloc((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return WT_ERR_NOMEM;
     }
@@ -54,10 +55,10 @@ int dalloc(T** p, int64_t count)
 void free_group(group* g)
 {
     if (!g) return;
-    (void)hipFree(g->d_seg);
-    (void)hipFree(g->d_ncol);
-    (void)hipFree(g->seg_part);
-    (void)hipFree(g->seg_out);
+    (void)rt_free(g->d_seg);
+    (void)rt_free(g->d_ncol);
+    (void)rt_free(g->seg_part);
+    (void)rt_free(g->seg_out);
     wt_solver_free(g->s);
     delete g;
 }
@@ -73,8 +74,8 @@ int64_t numeric_bytes(const wt_symbolic& S, int64_t cs, int64_t nrhs)
 int seg_buffers(group& g, int nrhs)
 {
     if (g.seg_cap >= nrhs) return WT_OK;
-    (void)hipFree(g.seg_part);
-    (void)hipFree(g.seg_out);
+    (void)rt_free(g.seg_part);
+    (void)rt_free(g.seg_out);
     g.seg_part = g.seg_out = nullptr;
     g.seg_cap = 0;
     int rc;
@@ -125,6 +126,7 @@ int refine_group(group& g, wt_numeric<R>& num, int nrhs, const int32_t* ncol, do
     if (graph && (!num.sgraph || num.sgraph_nrhs != nrhs)) {
         if (num.sgraph) (void)hipGraphExecDestroy(num.sgraph);
         num.sgraph = nullptr;
+        wt_rt_capture excl;
         WT_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
         rc = refine_step<R>(g, num, nrhs);
         hipGraph_t gr = nullptr;
@@ -202,10 +204,10 @@ struct num_pool {
     {
         for (int i = 0; i < NB; ++i) {
             if (cap[i] >= need[i] && p[i]) continue;
-            (void)hipFree(p[i]);
+            (void)rt_free(p[i]);
             p[i] = nullptr;
             cap[i] = 0;
-            if (hipMalloc(&p[i], (size_t)(need[i] > 0 ? need[i] : 16)) != hipSuccess) {
+            if (rt_malloc(&p[i], (size_t)(need[i] > 0 ? need[i] : 16)) != hipSuccess) {
                 p[i] = nullptr;
                 return WT_ERR_NOMEM;
             }
@@ -216,7 +218,7 @@ struct num_pool {
     void release()
     {
         for (int i = 0; i < NB; ++i) {
-            (void)hipFree(p[i]);
+            (void)rt_free(p[i]);
             p[i] = nullptr;
             cap[i] = 0;
         }
@@ -267,7 +269,7 @@ int run_groups(wt_batch* b, int nrhs, const int32_t* ncol, const void* rhs, void
     for (group* g : b->groups) {
         const wt_symbolic& S = g->s.S;
         const int64_t v[num_pool::NB] = {S.factor_len * cs, S.cb_len[0] * cs, S.cb_len[1] * cs, S.tile_len * cs,
-                                         S.piv_len * 4, 4, S.n * nrhs * cs, S.w_len[0] * nrhs * cs,
+                                         S.piv_len * 4, WT_PIVSTAT_LEN * 4, S.n * nrhs * cs, S.w_len[0] * nrhs * cs,
                                          S.w_len[1] * nrhs * cs};
         for (int i = 0; i < num_pool::NB; ++i) need[i] = need[i] > v[i] ? need[i] : v[i];
     }
@@ -383,7 +385,7 @@ int wt_batch_create(int nmodels, const int32_t* dims, int leaf_nodes, int64_t ma
         }
         if ((rc = wt_solver_init_union(g->s, gd, leaf_nodes))) break;
         if ((rc = dalloc(&g->d_seg, g->nm() + 1)) || (rc = dalloc(&g->d_ncol, g->nm()))) break;
-        if (hipMemcpy(g->d_seg, g->unk_off.data(), (g->nm() + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+        if (rt_memcpy(g->d_seg, g->unk_off.data(), (g->nm() + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
             rc = WT_ERR_HIP;
         g->s.use_graphs = true;
     }

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "wt_rhs.h"
+#include "wt_rt.h"
 
 int64_t wt_nnz(int nx, int ny, int nz);
 int wt_launch_pattern(int nx, int ny, int nz, int base, int32_t* d_rowptr, int32_t* d_colidx, hipStream_t stream);
@@ -19,8 +20,8 @@ struct Buf {
     void* p = nullptr;
     hipStream_t s;
     explicit Buf(hipStream_t st) : s(st) {}
-    ~Buf() { if (p) (void)hipFreeAsync(p, s); }
-    int alloc(size_t n) { return hipMallocAsync(&p, n ? n : 1, s) == hipSuccess ? WT_OK : WT_ERR_NOMEM; }
+    ~Buf() { if (p) (void)rt_free_async(p, s); }
+    int alloc(size_t n) { return rt_malloc_async(&p, n ? n : 1, s) == hipSuccess ? WT_OK : WT_ERR_NOMEM; }
 };
 
 template <typename RS>
@@ -56,6 +57,7 @@ int wt_haskgrn(int prec, int nl, const void* vp, const void* vs, const void* rho
                double zorig, void* d_u, void* d_w, double* d_pic, int32_t* status, void* stream)
 {
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     if (!vp || !vs || !rho || !top || !vpd || !vsd || !ain || !is_s || !d_u || !d_w || !d_pic || !status)
         return WT_ERR_ARG;
     if (nl < 1 || nl > WT_HASK_MAXL || ns < 1 || nz < 1) return WT_ERR_ARG;
@@ -74,6 +76,7 @@ int wt_planewave_layered(int nx, int ny, int nz, double dx, int prec, int nrhs, 
                          const double* d_kh, const double* phi, void* d_x, void* stream)
 {
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     if (nx < 1 || ny < 1 || nz < 1 || nrhs < 1 || !d_u || !d_w || !d_kh || !phi || !d_x) return WT_ERR_ARG;
     std::vector<double> cs((size_t)2 * nrhs);
     for (int j = 0; j < nrhs; ++j) {
@@ -108,6 +111,7 @@ int wt_scattered_source_grid(const wt_grid_params* p, const void* lam, const voi
     if (ldo <= 0) ldo = nrhs;
     if (ldu < nrhs || ldo < nrhs) return WT_ERR_ARG;
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     const int64_t nodes = (int64_t)p->nx * p->ny * p->nz, n = 3 * nodes, nnz = wt_nnz(p->nx, p->ny, p->nz);
     // stream-ordered scratch: the pattern, both operators, the models when
     // they come from the host, srcfn and the error word
@@ -161,6 +165,7 @@ int wt_total_field(void* d_uh, const void* d_ubg, int64_t count, void* stream)
 {
     if (!d_uh || !d_ubg || count < 0) return WT_ERR_ARG;
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     int rc = wt_launch_add((cpx<double>*)d_uh, (const cpx<double>*)d_ubg, count, st);
     if (rc) return rc;
     WT_HIP(hipStreamSynchronize(st));

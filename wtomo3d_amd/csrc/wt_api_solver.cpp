@@ -10,6 +10,7 @@
 #include "wt_solver.h"
 #include "wt_rhs.h"
 #include "wt_model.h"
+#include "wt_rt.h"
 #include "wt_vec.h"
 
 int64_t wt_nnz(int nx, int ny, int nz);
@@ -28,7 +29,7 @@ int dalloc(T** p, int64_t count)
 {
     *p = nullptr;
     if (count <= 0) count = 1;
-    if (hipMalloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
+    if (rt_malloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return WT_ERR_NOMEM;
     }
@@ -38,7 +39,7 @@ int dalloc(T** p, int64_t count)
 template <typename T>
 int salloc(T** p, int64_t count, hipStream_t st)
 {
-    return hipMallocAsync((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T), st) == hipSuccess ? WT_OK
+    return rt_malloc_async((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T), st) == hipSuccess ? WT_OK
                                                                                              : WT_ERR_NOMEM;
 }
 
@@ -51,7 +52,7 @@ namespace {
 int ensure_ir(wt_solver_impl& s, int nrhs)
 {
     if (s.ir_cap >= nrhs) return WT_OK;
-    (void)hipFree(s.xb); (void)hipFree(s.rb); (void)hipFree(s.bb); (void)hipFree(s.partial); (void)hipFree(s.norms);
+    (void)rt_free(s.xb); (void)rt_free(s.rb); (void)rt_free(s.bb); (void)rt_free(s.partial); (void)rt_free(s.norms);
     s.xb = s.rb = s.bb = nullptr;
     s.partial = s.norms = nullptr;
     s.ir_cap = 0;
@@ -135,6 +136,20 @@ int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b
     }
     if (!std::isfinite(rel)) return WT_ERR_PIVOT;
     return conv ? WT_OK : WT_ERR_NOCONV;
+}
+
+template <typename R>
+int pivot_report(const wt_numeric<R>& num, wt_pivot_report* out)
+{
+    if (!num.have_pivstat) return WT_ERR_STATE;
+    float mr, g;
+    std::memcpy(&mr, &num.pivstat[2], 4);
+    std::memcpy(&g, &num.pivstat[3], 4);
+    out->zero_pivots = num.pivstat[0] ? 1 : 0;
+    out->tiny_pivots = num.pivstat[1];
+    out->min_pivot_ratio = mr;
+    out->max_growth = g;
+    return WT_OK;
 }
 
 }  // namespace
@@ -270,6 +285,7 @@ int wt_dispersion(int64_t nodes, const double* vp, const double* vs, const doubl
 {
     if (nodes < 1 || !vp || !vs || !qpi || !qsi || !rho || !lam || !mu) return WT_ERR_ARG;
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     if (on_device) {
         int rc = dispersion_dev(nodes, vp, vs, qpi, qsi, rho, freqbase, omega_re, (cpx<double>*)lam, (cpx<double>*)mu,
                                 st);
@@ -278,7 +294,7 @@ int wt_dispersion(int64_t nodes, const double* vp, const double* vs, const doubl
         return WT_OK;
     }
     double* d = nullptr;
-    if (hipMallocAsync((void**)&d, (size_t)nodes * (5 * 8 + 32), st) != hipSuccess) return WT_ERR_NOMEM;
+    if (rt_malloc_async((void**)&d, (size_t)nodes * (5 * 8 + 32), st) != hipSuccess) return WT_ERR_NOMEM;
     const double* src[5] = {vp, vs, qpi, qsi, rho};
     int rc = WT_OK;
     for (int k = 0; k < 5 && !rc; ++k)
@@ -290,7 +306,7 @@ int wt_dispersion(int64_t nodes, const double* vp, const double* vs, const doubl
     if (!rc && (hipMemcpyAsync(lam, dl, nodes * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipMemcpyAsync(mu, dm, nodes * 16, hipMemcpyDeviceToHost, st) != hipSuccess))
         rc = WT_ERR_HIP;
-    (void)hipFreeAsync(d, st);
+    (void)rt_free_async(d, st);
     if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = WT_ERR_HIP;
     return rc;
 }
@@ -305,7 +321,7 @@ int wt_solver_set_model_q(wt_solver* h, const double* vp, const double* vs, cons
     if (!s.lam && ((rc = dalloc(&s.lam, nodes)) || (rc = dalloc(&s.mu, nodes)) || (rc = dalloc(&s.rho, nodes))))
         return rc;
     double* d = nullptr;
-    if (hipMallocAsync((void**)&d, (size_t)nodes * 4 * 8, s.stream) != hipSuccess) return WT_ERR_NOMEM;
+    if (rt_malloc_async((void**)&d, (size_t)nodes * 4 * 8, s.stream) != hipSuccess) return WT_ERR_NOMEM;
     const double* src[4] = {vp, vs, qpi, qsi};
     rc = WT_OK;
     for (int k = 0; k < 4 && !rc; ++k)
@@ -314,7 +330,7 @@ int wt_solver_set_model_q(wt_solver* h, const double* vp, const double* vs, cons
     if (!rc && hipMemcpyAsync(s.rho, rho, nodes * 8, hipMemcpyHostToDevice, s.stream) != hipSuccess) rc = WT_ERR_HIP;
     if (!rc) rc = dispersion_dev(nodes, d, d + nodes, d + 2 * nodes, d + 3 * nodes, s.rho, freqbase, omega_re, s.lam,
                                  s.mu, s.stream);
-    (void)hipFreeAsync(d, s.stream);
+    (void)rt_free_async(d, s.stream);
     if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
     return rc;
 }
@@ -360,7 +376,7 @@ int wt_solver_set_values(wt_solver* h, const void* vals, int prec, int on_device
         rc = hipMemcpyAsync(t, vals, s.nnz * 8, k, s.stream) == hipSuccess ? WT_OK : WT_ERR_HIP;
         if (!rc) rc = wt_widen_c64(t, s.vals, s.nnz, s.stream);
         if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
-        (void)hipFree(t);
+        (void)rt_free(t);
         if (rc) return rc;
     } else {
         return WT_ERR_ARG;
@@ -373,7 +389,7 @@ int wt_solver_set_values(wt_solver* h, const void* vals, int prec, int on_device
 int wt_solver_get_values(wt_solver* h, void* vals)
 {
     if (!h || !vals || !h->s.have_vals) return WT_ERR_ARG;
-    WT_HIP(hipMemcpy(vals, h->s.vals, h->s.nnz * 16, hipMemcpyDeviceToHost));
+    WT_HIP(rt_memcpy(vals, h->s.vals, h->s.nnz * 16, hipMemcpyDeviceToHost));
     return WT_OK;
 }
 
@@ -382,6 +398,14 @@ int wt_solver_factor(wt_solver* h, int factor_prec)
     if (!h) return WT_ERR_ARG;
     if (factor_prec == WT_PREC_F32) return wt_solver_factor_run<float>(h->s, h->s.f32);
     if (factor_prec == WT_PREC_F64) return wt_solver_factor_run<double>(h->s, h->s.f64);
+    return WT_ERR_ARG;
+}
+
+int wt_solver_pivot_report(const wt_solver* h, int factor_prec, wt_pivot_report* out)
+{
+    if (!h || !out) return WT_ERR_ARG;
+    if (factor_prec == WT_PREC_F32) return pivot_report(h->s.f32, out);
+    if (factor_prec == WT_PREC_F64) return pivot_report(h->s.f64, out);
     return WT_ERR_ARG;
 }
 
@@ -451,12 +475,12 @@ int wt_solver_planewaves(wt_solver* h, int nrhs, const double* dirs, double kmag
     if (!h || !dirs || !d_x || nrhs < 1) return WT_ERR_ARG;
     wt_solver_impl& s = h->s;
     double* dd = nullptr;
-    if (hipMalloc(&dd, 3 * nrhs * sizeof(double)) != hipSuccess) return WT_ERR_NOMEM;
+    if (rt_malloc(&dd, 3 * nrhs * sizeof(double)) != hipSuccess) return WT_ERR_NOMEM;
     int rc = hipMemcpyAsync(dd, dirs, 3 * nrhs * sizeof(double), hipMemcpyHostToDevice, s.stream) == hipSuccess
                  ? WT_OK : WT_ERR_HIP;
     if (!rc) rc = wt_planewave((cpx<double>*)d_x, s.S.nx, s.S.ny, s.S.nz, dx, kmag, dd, nrhs, s.stream);
     if (hipStreamSynchronize(s.stream) != hipSuccess && !rc) rc = WT_ERR_HIP;
-    (void)hipFree(dd);
+    (void)rt_free(dd);
     return rc;
 }
 
@@ -481,7 +505,7 @@ int wt_solver_scattered_source(wt_solver* h, const void* lam_bg, const void* mu_
     // handle's stream, without waiting for other streams
     auto cleanup = [&] {
         for (void* p : {(void*)lb, (void*)mb, (void*)rb, (void*)ab, (void*)sf})
-            if (p) (void)hipFreeAsync(p, s.stream);
+            if (p) (void)rt_free_async(p, s.stream);
         (void)hipStreamSynchronize(s.stream);
     };
     if ((rc = salloc(&lb, nodes, s.stream)) || (rc = salloc(&mb, nodes, s.stream)) ||
@@ -522,9 +546,10 @@ int wt_solver_scattered_source(wt_solver* h, const void* lam_bg, const void* mu_
 int wt_dev_copy(void* d_dst, const void* d_src, size_t bytes)
 {
     if (!d_dst || !d_src || bytes % 16) return WT_ERR_ARG;
+    wt_rt_shared g;  // null-stream launch
     int rc = wt_copy<double>((const cpx<double>*)d_src, (cpx<double>*)d_dst, (int64_t)(bytes / 16), nullptr);
     if (rc) return rc;
-    WT_HIP(hipDeviceSynchronize());
+    WT_HIP(rt_device_sync());
     return WT_OK;
 }
 

@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "wt_gemm.h"
+#include "wt_rt.h"
 
 namespace {
 template <typename R>
@@ -60,6 +61,7 @@ extern "C" int wt_gemm_host_var(int prec, int var, int M, int N, int K, const vo
                                 double* ms)
 {
     if (mode < WT_GEMM_SUB || mode > WT_GEMM_NSET) return WT_ERR_ARG;
+    wt_rt_shared g;  // synchronous copies and null-stream launches
     if (prec == WT_PREC_F32) return run_gemm<float>(var, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
     if (prec == WT_PREC_F64) return run_gemm<double>(var, M, N, K, A, lda, B, ldb, C, ldc, mode, bperm, reps, ms);
     return WT_ERR_ARG;

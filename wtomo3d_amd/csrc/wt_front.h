@@ -20,6 +20,14 @@ struct wt_dfront {
 };
 
 int wt_zero_i32(int* p, int n, hipStream_t s);
+// Pivot statistics of one factorization, kept by the diagonal-tile kernels
+// in WT_PIVSTAT_LEN ints: [0] 1 if an exactly zero pivot was replaced,
+// [1] the number of pivots with |d| <= sqrt(eps) max|tile|, [2] the bits of
+// the smallest |d| / max|tile|, [3] the bits of the largest max|U| / max|tile|
+// (non-negative floats, whose bits order as unsigned integers).  Magnitudes
+// are |re| + |im|; max|tile| is taken when the tile's elimination starts.
+#define WT_PIVSTAT_LEN 4
+int wt_pivstat_init(int* p, hipStream_t s);
 template <typename R>
 int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s);
 template <typename Ri, typename R>

@@ -148,6 +148,24 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __rest
 // wave-uniform LDS broadcast: wave 0 builds L^{-1} while wave 1 builds
 // U^{-1} (one after the other in FP64, whose second buffer would not fit in
 // LDS) and the other waves write the factored tile out.
+// Adds one tile's pivot statistics to the factorization's (see
+// WT_PIVSTAT_LEN): one call per tile, by one lane.
+__device__ __forceinline__ void pivstat_publish(int* status, int ntiny, float min_ratio, float growth)
+{
+    if (ntiny) atomicAdd(status + 1, ntiny);
+    atomicMin((unsigned*)status + 2, __float_as_uint(min_ratio));
+    atomicMax((unsigned*)status + 3, __float_as_uint(growth));
+}
+
+// The tiny-pivot threshold relative to the tile's largest entry: about the
+// square root of the precision's unit roundoff (2^-12 for fp32, 2^-26 for
+// fp64).
+template <typename R>
+__device__ constexpr R tiny_pivot_rel()
+{
+    return sizeof(R) == 4 ? R(2.44140625e-4) : R(1.490116119384765625e-8);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restrict__ fr,
                                                         const int32_t* __restrict__ list, int64_t kb,
@@ -158,6 +176,7 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restr
     constexpr int NX = sizeof(R) == 4 ? 2 : 1;  // inverse buffers that fit beside T
     __shared__ cpx<R> T[NB][NB + 1];          // T[col][row]
     __shared__ cpx<R> X[NX][NB][NB + 1];      // X[.][row][col] of an inverse
+    __shared__ R wmax[4];                     // per wave: max |re|+|im| of the loaded tile
     const wt_dfront F = fr[list[blockIdx.x]];
     if (kb >= F.p) return;
     const int w = (int)(F.p - kb < NB ? F.p - kb : NB);
@@ -166,16 +185,23 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restr
     const int tid = threadIdx.x, lane = tid & 63, wq = tid >> 6;
     {
         cpx<R> v[CPT];
+        R amax = R(0);
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int c = wq + 4 * k;
             v[k] = (c < w && lane < w) ? base[(int64_t)c * ld + lane] : cpx<R>(R(0), R(0));
+            amax = fmax(amax, fabs(v[k].re) + fabs(v[k].im));
         }
 #pragma unroll
         for (int k = 0; k < CPT; ++k) T[wq + 4 * k][lane] = v[k];
+        for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off));
+        if (lane == 0) wmax[wq] = amax;
     }
     int myperm = lane;  // wave 0: tile row that ends at position `lane`
     __syncthreads();
+    const R a0 = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    float min_ratio = __builtin_inff();  // thread 0's running pivot statistics
+    int ntiny = 0;
     for (int j = 0; j < w; ++j) {
         if (wq == 0) {
             R a = R(-1);
@@ -200,6 +226,11 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restr
         }
         __syncthreads();
         cpx<R> d = T[j][j];
+        if (tid == 0) {
+            const R ad = fabs(d.re) + fabs(d.im);
+            min_ratio = fminf(min_ratio, a0 > R(0) ? (float)(ad / a0) : 0.f);
+            ntiny += ad <= tiny_pivot_rel<R>() * a0;
+        }
         if (d.re == R(0) && d.im == R(0)) {
             if (tid == 0) atomicOr(status, 1);
             d = cpx<R>(R(1e-30), R(0));
@@ -262,10 +293,21 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(const wt_dfront* __restr
     } else if (wq == 0) {
         lower_inv(X[0]);
     }
+    R umax = R(0);  // largest |U| entry among this thread's stores
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int cc = wq + 4 * k;
-        if (cc < w && lane < w) base[(int64_t)cc * ld + lane] = T[cc][lane];
+        if (cc < w && lane < w) {
+            const cpx<R> t = T[cc][lane];
+            base[(int64_t)cc * ld + lane] = t;
+            if (lane <= cc) umax = fmax(umax, fabs(t.re) + fabs(t.im));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) umax = fmax(umax, __shfl_xor(umax, off));
+    if (lane == 0) {
+        const float growth = a0 > R(0) ? (float)(umax / a0) : 0.f;
+        if (wq == 0) pivstat_publish(status, ntiny, min_ratio, growth);
+        else atomicMax((unsigned*)status + 3, __float_as_uint(growth));
     }
     __syncthreads();
     store_inv(X[0], Li);
@@ -357,6 +399,12 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
         bool open = lane < w;  // not yet chosen as a pivot row
         int pos = lane;        // position of this row once chosen
         C dme(1.f, 0.f);       // 1 / own pivot
+        float a0 = 0.f;        // max |re|+|im| of the tile (entries outside it are 0)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) a0 = fmaxf(a0, fabsf(a[c].re) + fabsf(a[c].im));
+        a0 = wave_max(a0);
+        float min_ratio = __builtin_inff();
+        int ntiny = 0;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             if (j < w) {
@@ -366,6 +414,9 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
                 if (hit == 0) hit = __ballot(open);  // NaN column: first open row
                 const int pv = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(hit));
                 C d = read_lane(a[j], pv);
+                const float ad = fabsf(d.re) + fabsf(d.im);
+                min_ratio = fminf(min_ratio, a0 > 0.f ? ad / a0 : 0.f);
+                ntiny += ad <= tiny_pivot_rel<float>() * a0;
                 if (d.re == 0.f && d.im == 0.f) {
                     if (lane == 0) atomicOr(status, 1);
                     d = C(1e-30f, 0.f);
@@ -389,6 +440,15 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
             }
         }
         if (lane < w) perm[F.pivoff + kb + pos] = lane;
+        {
+            // this row is row `pos` of U from column pos on
+            float umax = 0.f;
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c >= pos) umax = fmaxf(umax, fabsf(a[c].re) + fabsf(a[c].im));
+            umax = wave_max(lane < w ? umax : 0.f);
+            if (lane == 0) pivstat_publish(status, ntiny, min_ratio, a0 > 0.f ? umax / a0 : 0.f);
+        }
         // packed L\U to its positions; L below the diagonal and U~^T = (D^{-1} U)^T to LDS
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
@@ -775,6 +835,19 @@ static __global__ void zero_i32_kernel(int* __restrict__ p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+static __global__ void pivstat_init_kernel(int* __restrict__ p)
+{
+    const int i = threadIdx.x;
+    if (i < WT_PIVSTAT_LEN) p[i] = i == 2 ? 0x7f800000 : 0;  // [2] starts at +inf
+}
+
+int wt_pivstat_init(int* p, hipStream_t s)
+{
+    hipLaunchKernelGGL(pivstat_init_kernel, dim3(1), dim3(64), 0, s, p);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
 }
 
 // A fill kernel rather than a memset node, so that captured sequences

@@ -11,7 +11,7 @@
 // 256 threads = 4 waves; fp32 uses v_mfma_f32_32x32x2f32 (default block
 // 128 x 128 as 2 x 2 waves of 64 x 64), fp64 uses v_mfma_f64_16x16x4f64
 // (default block 128 x 64, waves of 64 x 32); the other tile variants are
-// listed with wt_gemm_tile.  K is consumed in slices of 8 complex values,
+// listed with wt_gemm_tile.  K is consumed in slices of 16 complex values,
 // staged through double-buffered LDS as separate real and imaginary planes.
 #include "wt_gemm.h"
 
@@ -67,7 +67,11 @@ using gv2 = __attribute__((address_space(1))) v2<R>;
 typedef __attribute__((address_space(1))) const int32_t gi32;
 
 constexpr int NT = 256;  // threads per block
-constexpr int BK = 8;    // complex k per slice
+// complex k per slice.  The slice depth does not change the order in which
+// any accumulator sums its k terms (the MFMA sequence over k is the same),
+// so results do not depend on it; 16 halves the barriers per k of 8 and
+// measured 14-26 % faster for fp64 (1 block per CU) and 2-9 % for fp32.
+constexpr int BK = 16;
 
 template <typename R, int V>
 __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kernel(const wt_gemm_prob<R>* __restrict__ probs,

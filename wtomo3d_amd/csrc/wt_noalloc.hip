@@ -18,6 +18,7 @@
 #include <mutex>
 
 #include "wt_rhs.h"
+#include "wt_rt.h"
 
 namespace {
 
@@ -108,6 +109,7 @@ extern "C" int wt_haskgrn_noalloc(int prec, int nl, const void* vp, const void* 
                 kMaxSources);
         return WT_ERR_ARG;
     }
+    wt_rt_stream_guard g(stream);
     unsigned char* sc = nullptr;
     int rc = scratch_on_current_device(&sc);
     if (rc) return rc;

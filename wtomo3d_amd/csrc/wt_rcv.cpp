@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "wt_rhs.h"
+#include "wt_rt.h"
 
 namespace {
 
@@ -195,22 +196,22 @@ int wt_receivers_create(int nx, int ny, int nz, double dx, const int32_t* freesu
     if (!r) return WT_ERR_NOMEM;
     r->rows = (int)rp.size() - 1;
     const size_t ne = cols.size() ? cols.size() : 1;
-    if (hipMalloc(&r->rowptr, rp.size() * 4) != hipSuccess || hipMalloc(&r->col, ne * 4) != hipSuccess ||
-        hipMalloc(&r->val, ne * 16) != hipSuccess) {
-        (void)hipFree(r->rowptr);
-        (void)hipFree(r->col);
-        (void)hipFree(r->val);
+    if (rt_malloc(&r->rowptr, rp.size() * 4) != hipSuccess || rt_malloc(&r->col, ne * 4) != hipSuccess ||
+        rt_malloc(&r->val, ne * 16) != hipSuccess) {
+        (void)rt_free(r->rowptr);
+        (void)rt_free(r->col);
+        (void)rt_free(r->val);
         delete r;
         return WT_ERR_NOMEM;
     }
     std::vector<cpx<double>> v(vals.size());
     for (size_t i = 0; i < vals.size(); ++i) v[i] = cpx<double>(vals[i].real(), vals[i].imag());
-    if (hipMemcpy(r->rowptr, rp.data(), rp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        (cols.size() && hipMemcpy(r->col, cols.data(), cols.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
-        (v.size() && hipMemcpy(r->val, v.data(), v.size() * 16, hipMemcpyHostToDevice) != hipSuccess)) {
-        (void)hipFree(r->rowptr);
-        (void)hipFree(r->col);
-        (void)hipFree(r->val);
+    if (rt_memcpy(r->rowptr, rp.data(), rp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        (cols.size() && rt_memcpy(r->col, cols.data(), cols.size() * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (v.size() && rt_memcpy(r->val, v.data(), v.size() * 16, hipMemcpyHostToDevice) != hipSuccess)) {
+        (void)rt_free(r->rowptr);
+        (void)rt_free(r->col);
+        (void)rt_free(r->val);
         delete r;
         return WT_ERR_HIP;
     }
@@ -224,6 +225,7 @@ int wt_receivers_extract(const wt_receivers* r, int nrhs, const void* d_field, v
 {
     if (!r || !d_field || !d_out || nrhs < 1) return WT_ERR_ARG;
     const hipStream_t st = (hipStream_t)stream;
+    wt_rt_stream_guard g(stream);
     int rc = wt_launch_rows(r->rowptr, r->col, r->val, (const cpx<double>*)d_field, (cpx<double>*)d_out, r->rows,
                             nrhs, st);
     if (rc) return rc;
@@ -234,9 +236,9 @@ int wt_receivers_extract(const wt_receivers* r, int nrhs, const void* d_field, v
 int wt_receivers_destroy(wt_receivers* r)
 {
     if (!r) return WT_ERR_ARG;
-    (void)hipFree(r->rowptr);
-    (void)hipFree(r->col);
-    (void)hipFree(r->val);
+    (void)rt_free(r->rowptr);
+    (void)rt_free(r->col);
+    (void)rt_free(r->val);
     delete r;
     return WT_OK;
 }

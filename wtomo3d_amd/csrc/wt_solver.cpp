@@ -16,6 +16,7 @@
 #include <cstring>
 #include <thread>
 
+#include "wt_rt.h"
 #include "wt_vec.h"
 
 int wt_launch_pattern(int nx, int ny, int nz, int base, int32_t* d_rowptr, int32_t* d_colidx, hipStream_t stream);
@@ -26,7 +27,7 @@ static int dalloc(T** p, int64_t count)
 {
     *p = nullptr;
     if (count <= 0) count = 1;
-    if (hipMalloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
+    if (rt_malloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return WT_ERR_NOMEM;
     }
@@ -35,7 +36,7 @@ static int dalloc(T** p, int64_t count)
 template <typename T>
 static void dfree(T*& p)
 {
-    if (p) (void)hipFree((void*)p);
+    if (p) (void)rt_free((void*)p);
     p = nullptr;
 }
 
@@ -47,9 +48,9 @@ int wt_gemm_plan<R>::upload()
     tstride = (int64_t)tiles[0].size();
     if (dalloc(&d_probs, (int64_t)probs.size()) || dalloc(&d_tiles, WT_GEMM_NVAR * tstride)) return WT_ERR_NOMEM;
     if (!probs.empty())
-        WT_HIP(hipMemcpy(d_probs, probs.data(), probs.size() * sizeof(probs[0]), hipMemcpyHostToDevice));
+        WT_HIP(rt_memcpy(d_probs, probs.data(), probs.size() * sizeof(probs[0]), hipMemcpyHostToDevice));
     for (int v = 0; v < WT_GEMM_NVAR && tstride > 0; ++v)
-        WT_HIP(hipMemcpy(d_tiles + v * tstride, tiles[v].data(), tstride * sizeof(int64_t), hipMemcpyHostToDevice));
+        WT_HIP(rt_memcpy(d_tiles + v * tstride, tiles[v].data(), tstride * sizeof(int64_t), hipMemcpyHostToDevice));
     return WT_OK;
 }
 template <typename R>
@@ -187,8 +188,8 @@ int wt_solver_init_union(wt_solver_impl& s, const std::vector<wt_grid_dims>& gri
     s.n = s.S.n;
     s.nnz = s.S.nnz;
     if ((rc = dalloc(&s.rowptr, s.n + 1)) || (rc = dalloc(&s.colidx, s.nnz))) return rc;
-    WT_HIP(hipMemcpy(s.rowptr, hr.data(), (s.n + 1) * 4, hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(s.colidx, hc.data(), s.nnz * 4, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.rowptr, hr.data(), (s.n + 1) * 4, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.colidx, hc.data(), s.nnz * 4, hipMemcpyHostToDevice));
     return upload_symbolic(s);
 }
 
@@ -258,15 +259,15 @@ static int upload_symbolic(wt_solver_impl& s)
         (rc = dalloc(&s.dst, S.nnz)) || (rc = dalloc(&s.iperm, S.n)) || (rc = dalloc(&s.upd, (int64_t)S.upd_idx.size())) ||
         (rc = dalloc(&s.relmap, (int64_t)S.relmap.size())) || (rc = dalloc(&s.d_err, 1)))
         return rc;
-    WT_HIP(hipMemcpy(s.fronts, df.data(), df.size() * sizeof(wt_dfront), hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(s.lists, s.hlists.data(), s.hlists.size() * 4, hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(s.cols, s.hcols.data(), s.hcols.size() * 4, hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(s.dst, S.dst.data(), S.nnz * 8, hipMemcpyHostToDevice));
-    WT_HIP(hipMemcpy(s.iperm, S.iperm.data(), S.n * 8, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.fronts, df.data(), df.size() * sizeof(wt_dfront), hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.lists, s.hlists.data(), s.hlists.size() * 4, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.cols, s.hcols.data(), s.hcols.size() * 4, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.dst, S.dst.data(), S.nnz * 8, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.iperm, S.iperm.data(), S.n * 8, hipMemcpyHostToDevice));
     if (!S.upd_idx.empty())
-        WT_HIP(hipMemcpy(s.upd, S.upd_idx.data(), S.upd_idx.size() * 8, hipMemcpyHostToDevice));
+        WT_HIP(rt_memcpy(s.upd, S.upd_idx.data(), S.upd_idx.size() * 8, hipMemcpyHostToDevice));
     if (!S.relmap.empty())
-        WT_HIP(hipMemcpy(s.relmap, S.relmap.data(), S.relmap.size() * 4, hipMemcpyHostToDevice));
+        WT_HIP(rt_memcpy(s.relmap, S.relmap.data(), S.relmap.size() * 4, hipMemcpyHostToDevice));
     // the dst map now lives on the device only
     s.S.dst.clear();
     s.S.dst.shrink_to_fit();
@@ -404,7 +405,7 @@ static int alloc_numeric(wt_solver_impl& s, wt_numeric<R>& num)
         if (!num.factor || !num.cb[0] || !num.cb[1] || !num.tiles || !num.perm || !num.status) return WT_ERR_STATE;
     } else if ((rc = dalloc(&num.factor, S.factor_len)) || (rc = dalloc(&num.cb[0], S.cb_len[0])) ||
                (rc = dalloc(&num.cb[1], S.cb_len[1])) || (rc = dalloc(&num.tiles, S.tile_len)) ||
-               (rc = dalloc(&num.perm, S.piv_len)) || (rc = dalloc(&num.status, 1))) {
+               (rc = dalloc(&num.perm, S.piv_len)) || (rc = dalloc(&num.status, WT_PIVSTAT_LEN))) {
         num.release();
         return rc;
     }
@@ -427,7 +428,7 @@ static int enqueue_factor(wt_solver_impl& s, wt_numeric<R>& num, std::vector<hip
     wt_prof& P = s.prof;
     const double cs = sizeof(cpx<R>);
     int rc;
-    if ((rc = wt_zero_i32(num.status, 1, st))) return rc;
+    if ((rc = wt_pivstat_init(num.status, st))) return rc;
     if ((rc = P.run(st, WT_PROF_FILL, 0, S.factor_len * cs,
                     [&] { return wt_fill_zero<R>(num.factor, S.factor_len, st); })))
         return rc;
@@ -480,12 +481,13 @@ static int factor_prepare(wt_solver_impl& s, wt_numeric<R>& num)
 template <typename R>
 static int factor_finish(wt_solver_impl& s, wt_numeric<R>& num)
 {
-    int hs = 0;
-    WT_HIP(hipMemcpyAsync(&hs, num.status, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    num.have_pivstat = false;
+    WT_HIP(hipMemcpyAsync(num.pivstat, num.status, sizeof(num.pivstat), hipMemcpyDeviceToHost, s.stream));
     WT_HIP(hipStreamSynchronize(s.stream));
+    num.have_pivstat = true;
     s.prof.flush();
     s.factored_prec = sizeof(R) == 4 ? 32 : 64;
-    return hs ? WT_ERR_PIVOT : WT_OK;
+    return num.pivstat[0] ? WT_ERR_PIVOT : WT_OK;
 }
 
 template <typename R>
@@ -517,6 +519,7 @@ int wt_solver_factor_launch(wt_solver_impl& s, wt_numeric<R>& num)
     if (rc) return rc;
     if (s.use_graphs && !s.prof.on) {
         if (!num.fgraph) {
+            wt_rt_capture excl;
             WT_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
             rc = enqueue_factor(s, num, nullptr);
             hipGraph_t g = nullptr;
@@ -691,7 +694,7 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
     }
     if ((rc = dalloc(&num.d_sp, (int64_t)num.hsp.size()))) return rc;
     if (!num.hsp.empty())
-        WT_HIP(hipMemcpy(num.d_sp, num.hsp.data(), num.hsp.size() * 4, hipMemcpyHostToDevice));
+        WT_HIP(rt_memcpy(num.d_sp, num.hsp.data(), num.hsp.size() * 4, hipMemcpyHostToDevice));
     if ((rc = num.splan.upload())) return rc;
     num.snrhs = nrhs;
     return WT_OK;

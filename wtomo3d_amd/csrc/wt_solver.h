@@ -109,7 +109,9 @@ struct wt_numeric {
     cpx<R>* cb[2] = {nullptr, nullptr};
     cpx<R>* tiles = nullptr;
     int32_t* perm = nullptr;
-    int* status = nullptr;
+    int* status = nullptr;  // WT_PIVSTAT_LEN pivot statistics on the device
+    int pivstat[WT_PIVSTAT_LEN] = {0, 0, 0x7f800000, 0};  // host copy after the factorization
+    bool have_pivstat = false;
     wt_gemm_plan<R> fplan;
     std::vector<wt_step> fsteps;
     // block solve

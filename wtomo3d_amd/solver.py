@@ -14,7 +14,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import lib, check, WtSolveStats, NotConverged, WT_ERR_NOCONV
+from ._lib import lib, check, WtSolveStats, WtPivotReport, NotConverged, WT_ERR_NOCONV
 
 _vp = ctypes.c_void_p
 
@@ -101,6 +101,13 @@ class Solver:
 
     def factor(self, prec=32):
         check(lib().wt_solver_factor(self._h, prec), "wt_solver_factor")
+
+    def pivot_report(self, prec=32):
+        """Pivot statistics of the last factorization in ``prec``: zero_pivots,
+        tiny_pivots, min_pivot_ratio and max_growth (``wt_pivot_report``)."""
+        r = WtPivotReport()
+        check(lib().wt_solver_pivot_report(self._h, prec, ctypes.byref(r)), "wt_solver_pivot_report")
+        return {k: getattr(r, k) for k, _ in WtPivotReport._fields_}
 
     def autotune(self, prec=32):
         """Times every factorization GEMM launch under each tile variant and
