@@ -28,7 +28,7 @@ int run_gemm(int var, int M, int N, int K, const void* A, int64_t lda, const voi
         wt_gemm_prob<R> p;
         p.A = dA; p.B = dB; p.C = dC; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
         p.M = M; p.N = N; p.K = K; p.mode = mode; p.bperm = dp; p.pad = 0;
-        if (var < 0) var = (sizeof(R) == 8 || (N > 64 && wt_gemm_tiles<R>(M, N, 0) >= 1024)) ? 0 : 2;
+        if (var < 0) var = (sizeof(R) == 4 && N > 64 && wt_gemm_tiles<R>(M, N, 0) >= 1024) ? 0 : 2;
         int64_t ts[2] = {0, wt_gemm_tiles<R>(M, N, var)};
         if (hipMemcpy(dA, A, na * cs, hipMemcpyHostToDevice) || hipMemcpy(dB, B, nbv * cs, hipMemcpyHostToDevice) ||
             (bperm && hipMemcpy(dp, bperm, K * 4, hipMemcpyHostToDevice)) ||

@@ -11,6 +11,13 @@
 // block: trailing updates beyond the super-panel are deferred to one
 // rank-WT_NB2 GEMM (a multiple of WT_NB).
 #define WT_NB2 512
+// Super-panel width of the factorization alone (a multiple of WT_NB): the
+// rank-WT_NB updates inside a super-panel have K = 64, the deferred update
+// has K = WT_FNB2.  Independent of the solve's WT_NB2: the layout of the
+// stored factor (packed L\U and 64 x 64 tile inverses) does not depend on it.
+#ifndef WT_FNB2
+#define WT_FNB2 512
+#endif
 
 // Device copy of the per-front offsets of wt_front (wt_symbolic.h).
 struct wt_dfront {

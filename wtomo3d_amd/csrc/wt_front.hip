@@ -403,8 +403,7 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
 #pragma unroll
         for (int c = 0; c < NB; ++c) a0 = fmaxf(a0, fabsf(a[c].re) + fabsf(a[c].im));
         a0 = wave_max(a0);
-        float min_ratio = __builtin_inff();
-        int ntiny = 0;
+        float pab = 0.f;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             if (j < w) {
@@ -414,9 +413,10 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
                 if (hit == 0) hit = __ballot(open);  // NaN column: first open row
                 const int pv = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(hit));
                 C d = read_lane(a[j], pv);
-                const float ad = fabsf(d.re) + fabsf(d.im);
-                min_ratio = fminf(min_ratio, a0 > 0.f ? ad / a0 : 0.f);
-                ntiny += ad <= tiny_pivot_rel<float>() * a0;
+                // lane j keeps |pivot j| for the statistics taken after the
+                // loop (kept out of this fully unrolled loop, which holds
+                // the tile in registers only while its body stays small)
+                if (lane == j) pab = fabsf(d.re) + fabsf(d.im);
                 if (d.re == 0.f && d.im == 0.f) {
                     if (lane == 0) atomicOr(status, 1);
                     d = C(1e-30f, 0.f);
@@ -447,7 +447,10 @@ __global__ __launch_bounds__(128) void diag_tile_wave_kernel(const wt_dfront* __
             for (int c = 0; c < NB; ++c)
                 if (c >= pos) umax = fmaxf(umax, fabsf(a[c].re) + fabsf(a[c].im));
             umax = wave_max(lane < w ? umax : 0.f);
-            if (lane == 0) pivstat_publish(status, ntiny, min_ratio, a0 > 0.f ? umax / a0 : 0.f);
+            float ratio = lane < w ? (a0 > 0.f ? pab / a0 : 0.f) : __builtin_inff();
+            const int ntiny = __popcll(__ballot(lane < w && pab <= tiny_pivot_rel<float>() * a0));
+            for (int off = 32; off > 0; off >>= 1) ratio = fminf(ratio, __shfl_xor(ratio, off));
+            if (lane == 0) pivstat_publish(status, ntiny, ratio, a0 > 0.f ? umax / a0 : 0.f);
         }
         // packed L\U to its positions; L below the diagonal and U~^T = (D^{-1} U)^T to LDS
 #pragma unroll

@@ -10,7 +10,8 @@
 // column-major C, and the epilogue stores are contiguous across lanes.
 // 256 threads = 4 waves; fp32 uses v_mfma_f32_32x32x2f32 (default block
 // 128 x 128 as 2 x 2 waves of 64 x 64), fp64 uses v_mfma_f64_16x16x4f64
-// (default block 128 x 64, waves of 64 x 32); the other tile variants are
+// (the solver's default for fp64 is the 64 x 64 block, two per CU); the
+// tile variants are
 // listed with wt_gemm_tile.  K is consumed in slices of 16 complex values,
 // staged through double-buffered LDS as separate real and imaginary planes.
 #include "wt_gemm.h"
@@ -70,11 +71,22 @@ constexpr int NT = 256;  // threads per block
 // complex k per slice.  The slice depth does not change the order in which
 // any accumulator sums its k terms (the MFMA sequence over k is the same),
 // so results do not depend on it; 16 halves the barriers per k of 8 and
-// measured 14-26 % faster for fp64 (1 block per CU) and 2-9 % for fp32.
+// measured 14-26 % faster for fp64 and 2-9 % for fp32.
 constexpr int BK = 16;
 
+// Blocks per CU the register allocation is sized for.  fp32: 2.  fp64: 2
+// for the 64 x 64 tile, whose 70 KB of LDS lets two blocks share a CU once
+// each wave fits in 256 registers (measured 1.5x the single-block rate:
+// 75 against 48 TF/s at 4096^3); 1 for the 128-row tiles, whose 100 KB of
+// LDS allows only one.
 template <typename R, int V>
-__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void gemm_batched_kernel(const wt_gemm_prob<R>* __restrict__ probs,
+constexpr int gemm_occupancy()
+{
+    return sizeof(R) == 4 ? 2 : (V == 2 ? 2 : 1);
+}
+
+template <typename R, int V>
+__global__ __launch_bounds__(NT, (gemm_occupancy<R, V>())) void gemm_batched_kernel(const wt_gemm_prob<R>* __restrict__ probs,
                                                             const int64_t* __restrict__ tile_start,
                                                             int nprob, int64_t ntiles)
 {

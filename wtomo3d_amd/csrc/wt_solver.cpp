@@ -300,7 +300,7 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
     const wt_symbolic& S = s.S;
     const int nl = (int)S.levels.size();
     const int64_t nb = S.nb;
-    const int64_t NB2 = WT_NB2;
+    const int64_t NB2 = WT_FNB2;
     num.fplan.release();
     num.fsteps.clear();
     // The contribution blocks need no clearing: each front's CB is first

@@ -73,7 +73,9 @@ struct wt_gemm_plan {
         const double a0 = (double)cur.ntiles[0] * wt_gemm_tile<R, 0>::BM * wt_gemm_tile<R, 0>::BN;
         const double a2 = (double)cur.ntiles[2] * wt_gemm_tile<R, 2>::BM * wt_gemm_tile<R, 2>::BN;
         cur.var = (!narrow && cur.ntiles[0] >= 1024 && a0 <= 1.15 * a2) ? 0 : 2;
-        if (sizeof(R) == 8) cur.var = 0;  // fp64: the 128 x 64 tile throughout (measured fastest)
+        // fp64: the 64 x 64 tile throughout (two blocks per CU; measured
+        // fastest on large and narrow launches alike)
+        if (sizeof(R) == 8) cur.var = 2;
         launches.push_back(cur);
         return (int)launches.size() - 1;
     }

@@ -562,7 +562,8 @@ def latency_report(reqs, wall):
             "rhs_per_s": rhs_ / wall, "wall_s": wall,
             "factor_reuse": sum(1 for r in ok if r.reused),
             "batched": sum(1 for r in ok if r.batched),
-            "max_rel_residual": max((r.stats["rel_residual"] for r in ok), default=None)}
+            "max_rel_residual": max((r.stats["rel_residual"] for r in ok), default=None),
+            "error_samples": sorted({str(r.error) for r in reqs if r.error is not None})[:5]}
 
 
 def run_stream(rate, seconds, seed=1005, slots=4, hbm_budget=200e9, side=(32, 80), report_every=20, **engine_kw):
