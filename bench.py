@@ -43,6 +43,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0  # dense bf16 matrix
 PEAK_HBM_GBS = 8000.0
 METRIC = "RHS solves/s at 96x96x64, FP32-factor+IR vs FP64 @1e-8 resid; p99 latency"
 
@@ -68,6 +69,8 @@ def parse():
                     help="measure the tile shapes even when the package holds a tuning table for this workload")
     ap.add_argument("--save-tuning", metavar="PATH", default=None,
                     help="write the tuned tile shapes (factorization and block solve) to PATH as JSON")
+    ap.add_argument("--no-split", action="store_true",
+                    help="keep every complex64 product on the fp32 matrix cores (no bf16-split GEMMs)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the multi-frequency layered sample")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="requests/s of the serving sample")
     ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the serving sample")
@@ -144,9 +147,10 @@ def reference_assembly(nx, ny, nz, dx, freq, vp, vs, rho, gpu_ms):
 
 
 GEMM_KERNEL = "gemm_batched_kernel<float, 0>"
+SPLIT_KERNEL = "gemm_split_kernel"
 
 
-def gemm_traffic():
+def gemm_traffic(kernel):
     """HBM bytes per launch of the roofline kernel from the committed counter
     summary (profiles/gemm_traffic.json, two rocprofv3 --pmc passes of this
     benchmark scaled by a copy-kernel calibration), when it covers it."""
@@ -156,9 +160,36 @@ def gemm_traffic():
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != GEMM_KERNEL:
+    t = t.get(kernel) if "kernel" not in t else (t if t["kernel"] == kernel else None)
+    if not t:
         return None
     return {"bytes_per_launch": t["bytes_per_dispatch"], "source": t.get("source", path)}
+
+
+def gemm_roofline(split, kernel, achieved, traffic, g, all_gemm, steps):
+    """Roofline of the factorization's dominant GEMM kernel.  On the fp32
+    matrix cores: 8 M N K per complex product against the FP32 peak (3M
+    executes 3/4 of it).  As bf16 splits: the executed bf16 work, 3/4 (3M)
+    x 6 split pairs of the same count, against the bf16 peak, with the
+    fp32-equivalent rate beside it."""
+    r = {"bound": "mfma", "unit": "TFLOP/s",
+         "traffic": traffic["bytes_per_launch"] if traffic else None,
+         "traffic_source": traffic["source"] if traffic else None,
+         "flops_per_launch": g["flops"] / max(g["launches"], 1),
+         "avg_launch_ms": g["ms"] / max(g["launches"], 1),
+         "launches_per_step": g["launches"] / steps, "ms_per_step": g["ms"] / steps,
+         "all_factor_gemms": all_gemm}
+    if split:
+        executed = achieved * 0.75 * 6
+        r.update(kernel=kernel + ": the factorization's large complex64 GEMMs, each fp32 product as six bf16 "
+                 "split pairs (3M, fp32 accumulate); achieved counts the executed bf16 flops",
+                 achieved=executed, peak=PEAK_BF16_TFLOPS, frac=executed / PEAK_BF16_TFLOPS,
+                 fp32_equivalent_tflops=achieved, fp32_equivalent_frac=achieved / PEAK_FP32_TFLOPS)
+    else:
+        r.update(kernel=kernel + ": the factorization's complex GEMMs on the 128x128 tile (3M, fp32 MFMA); "
+                 "all_factor_gemms counts every tile variant", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+                 frac=achieved / PEAK_FP32_TFLOPS, executed_frac_3m=0.75 * achieved / PEAK_FP32_TFLOPS)
+    return r
 
 
 def progress(msg):
@@ -239,8 +270,11 @@ def main():
         W._lib.check(L.wt_solver_solve(s._h, prec, nrhs, dB, dX, 1, tol, maxit, ctypes.byref(st)), "solve")
         return st.rel_residual, st.iterations
 
+    if a.no_split:
+        s.set_gemm_split(False)
     tune = None
-    table = os.path.join(ROOT, "wtomo3d_amd", "tuning", "fp32_%dx%dx%d_nrhs%d.json" % (nx, ny, nz, nrhs))
+    table = os.path.join(ROOT, "wtomo3d_amd", "tuning", "fp32_%dx%dx%d_nrhs%d%s.json"
+                         % (nx, ny, nz, nrhs, "_nosplit" if a.no_split else ""))
     if not a.no_autotune and not a.retune and os.path.exists(table):
         with open(table) as f:
             tt = json.load(f)
@@ -345,15 +379,20 @@ def main():
         except RuntimeError as e:
             fp64 = {"error": str(e), "hbm_bytes_estimate": need}
 
-    g_fac = prof["gemm_factor"]
+    g_spl = prof["gemm_factor_split"]
+    split_dominant = g_spl["ms"] > prof["gemm_factor"]["ms"]
+    g_fac = g_spl if split_dominant else prof["gemm_factor"]
+    kname = SPLIT_KERNEL if split_dominant else GEMM_KERNEL
     achieved = g_fac["flops"] / (g_fac["ms"] * 1e-3) / 1e12 if g_fac["ms"] > 0 else 0.0
     # every factorization GEMM, whichever tile variant the autotuner gave it
     g_nar = prof["gemm_factor_narrow"]
-    g_all_ms = g_fac["ms"] + g_nar["ms"]
-    g_all_tf = (g_fac["flops"] + g_nar["flops"]) / (g_all_ms * 1e-3) / 1e12 if g_all_ms > 0 else 0.0
+    g_parts = [prof["gemm_factor"], g_nar, g_spl]
+    g_all_ms = sum(g["ms"] for g in g_parts)
+    g_all_tf = sum(g["flops"] for g in g_parts) / (g_all_ms * 1e-3) / 1e12 if g_all_ms > 0 else 0.0
     all_gemm = {"achieved": g_all_tf, "frac": g_all_tf / PEAK_FP32_TFLOPS, "ms_per_step": g_all_ms / a.steps,
-                "launches_per_step": (g_fac["launches"] + g_nar["launches"]) / a.steps}
-    traffic = gemm_traffic()
+                "launches_per_step": sum(g["launches"] for g in g_parts) / a.steps,
+                "note": "8 M N K per complex product (fp32 complex-product rate) against the FP32 matrix peak"}
+    traffic = gemm_traffic(kname)
     # block solve (all right-hand sides at once): factor entries read once per
     # solve, 8 flops per entry and right-hand side; time = the solve GEMMs,
     # the super-panel triangular solves and the gathers / permutations
@@ -382,22 +421,13 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "RHS/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": t32 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 factor / f64 refinement", "data": "synthetic",
+        "dtype": "c64 factor%s / f64 refinement" % (" (large GEMMs: fp32 products as bf16 splits, fp32 accumulate)"
+                                                    if split_dominant else ""), "data": "synthetic",
         "config": {"workload": "heterogeneous %dx%dx%d dx=200m, 0.3 Hz, %d plane-wave RHS, FP32 factor + "
                                "FP64 IR to 1e-8" % (nx, ny, nz, nrhs),
                    "nx": nx, "ny": ny, "nz": nz, "nrhs": nrhs, "unknowns": info["n"], "nnz": info["nnz"],
                    "fronts": info["fronts"], "factor_flops": info["flops"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": GEMM_KERNEL + ": the factorization's complex GEMMs on the "
-                     "128x128 tile (3M, fp32 MFMA); all_factor_gemms counts every tile variant", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "traffic_source": traffic["source"] if traffic else None,
-                     "flops_per_launch": g_fac["flops"] / max(g_fac["launches"], 1),
-                     "avg_launch_ms": g_fac["ms"] / max(g_fac["launches"], 1),
-                     "launches_per_step": g_fac["launches"] / a.steps,
-                     "ms_per_step": g_fac["ms"] / a.steps,
-                     "executed_frac_3m": 0.75 * achieved / PEAK_FP32_TFLOPS,
-                     "all_factor_gemms": all_gemm},
+        "roofline": gemm_roofline(split_dominant, kname, achieved, traffic, g_fac, all_gemm, a.steps),
         "cpu_baseline": cpu,
         "rel_residual": max(res32), "refinement_solves": max(it32),
         "fp64": fp64,

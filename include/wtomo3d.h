@@ -196,6 +196,14 @@ int wt_solver_get_tiles(wt_solver *h, int factor_prec, int which, int nrhs, int3
 /* Capture the factorization in a HIP graph on its first run and replay it
  * (default off for a single handle; profiling runs eagerly). */
 int wt_solver_set_graphs(wt_solver *h, int enable);
+/* complex64 GEMMs on the bf16 matrix cores (default on): the large launches
+ * of the factorization and solve form each real fp32 product from a
+ * three-way bf16 split of both operands (six partial products, fp32
+ * accumulation), which keeps about the error of one fp32 rounding per
+ * product; off keeps every product on the fp32 matrix cores.  Applies to
+ * plans built later and maps the current ones' large tiles.  No
+ * counterpart in the reference (STRUMPACK's complex64 GEMMs are fp32). */
+int wt_solver_set_gemm_split(wt_solver *h, int enable);
 int wt_solver_set_tiles(wt_solver *h, int factor_prec, int which, int nrhs, const int32_t *vars, int64_t count);
 int wt_solver_release(wt_solver *h, int factor_prec);
 
@@ -352,12 +360,12 @@ int wt_receivers_extract(const wt_receivers *r, int nrhs, const void *d_field, v
 int wt_receivers_destroy(wt_receivers *r);
 
 /* Per-kernel timing with HIP events on the solver's stream: enable (1) or
- * disable (0) and reset.  Classes: 0 factor GEMMs in the square-tile
- * kernel (trailing and contribution-block updates, wide panels), 1 solve
- * GEMMs, 2 diagonal-tile LU, 3 extend-add, 4 fills / value scatter, 5
- * assembly, 6 FP64 residual, 7 reorder / gather, 8 super-panel triangular
- * solves, 9 factor GEMMs in the narrow-tile kernel (launches whose
- * problems all have N <= 64).  _read returns launches, summed kernel milliseconds,
+ * disable (0) and reset.  Classes: 0 factor GEMMs on the 128 x 128 tile
+ * (fp32 matrix cores), 1 solve GEMMs, 2 diagonal-tile LU, 3 extend-add,
+ * 4 fills / value scatter, 5 assembly, 6 FP64 residual, 7 reorder /
+ * gather, 8 super-panel triangular solves, 9 factor GEMMs on the other
+ * tiles, 10 complex64 factor GEMMs as bf16 splits (wt_solver_set_gemm_split).
+ * _read returns launches, summed kernel milliseconds,
  * algorithmic flops (8 M N K per complex GEMM, 8 per residual complex
  * multiply-add) and algorithmic bytes.  No counterpart in the reference
  * (STRUMPACK's verbose statistics, strumpack_finter.c:32-33, are printed). */

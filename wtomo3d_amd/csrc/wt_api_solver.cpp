@@ -424,6 +424,22 @@ int wt_solver_set_graphs(wt_solver* h, int enable)
     return WT_OK;
 }
 
+int wt_solver_set_gemm_split(wt_solver* h, int enable)
+{
+    if (!h) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    s.gemm_split = enable != 0;
+    for (wt_gemm_plan<float>* pl : {&s.f32.fplan, &s.f32.splan}) {
+        pl->allow_split = s.gemm_split;
+        for (auto& L : pl->launches) {
+            if (!s.gemm_split && L.var == WT_GEMM_SPLIT) L.var = 0;
+            else if (s.gemm_split && L.var == 0) L.var = WT_GEMM_SPLIT;
+        }
+    }
+    s.f32.drop_graphs();
+    return WT_OK;
+}
+
 int wt_solver_get_tiles(wt_solver* h, int factor_prec, int which, int nrhs, int32_t* vars, int64_t* count)
 {
     if (!h || !count) return WT_ERR_ARG;

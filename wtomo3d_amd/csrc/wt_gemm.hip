@@ -254,6 +254,219 @@ __global__ __launch_bounds__(NT, (gemm_occupancy<R, V>())) void gemm_batched_ker
             }
 }
 
+// ---- fp32 products on the bf16 matrix cores (tile variant 4) ----
+//
+// Each fp32 operand x is split into three bf16 parts, x = h + m + l with
+// h = bf16(x), m = bf16(x - h), l = bf16(x - h - m), each rounded to
+// nearest even (both differences are exact in fp32), so the parts carry the
+// 24 bits of x's significand.  A real product a b is formed from the six
+// pairs of the split parts down to order 2^-16 |a b|,
+//   a b = h h' + (h m' + m h') + (h l' + m m' + l h') + O(2^-23 |a b|),
+// each pair exact in fp32 and accumulated in fp32 by v_mfma_f32_32x32x16_bf16,
+// so a product carries about the error of one fp32 rounding.  Six bf16
+// MFMAs cost 6 x 32768 / 2.5 PFLOP/s against 8 x 4096 / 157 TFLOP/s for the
+// same 32 x 32 x 16 real product on the fp32 matrix cores (2.7x less time at
+// equal utilisation).  The 3M form is kept: the planes are Re, Im and
+// Re + Im (the sum rounded in fp32, then split), nine bf16 planes per
+// operand.
+//
+// Geometry: 256 threads, a 128 x 128 tile as 2 x 2 waves of 64 x 64 (2 x 2
+// MFMA blocks of 32 x 32 each, three accumulators per block: T1 = Re Re,
+// T2 = Im Im, T3 = Sum Sum), K in slices of 16 = one MFMA depth.  A slice
+// is staged in LDS as [plane][row][16 k] with k contiguous, which is the
+// MFMA operand order (lane l reads 8 consecutive k of row l % 32), double
+// buffered: 2 x 2 x 9 x 128 x 16 bf16 = 144 KB, one block per CU.  The two
+// 16-byte halves of a row swap places every 4 rows, so that the 16-byte
+// reads and writes of 8 consecutive rows cover all 32 banks.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
+{
+    h = (__bf16)x;
+    const float r = x - (float)h;
+    m = (__bf16)r;
+    l = (__bf16)(r - (float)m);
+}
+
+// 16-byte chunk (0 or 1) of a row's 16 k as stored: swapped every 4 rows
+__device__ __forceinline__ int split_chunk(int row, int chunk) { return chunk ^ ((row >> 2) & 1); }
+
+__global__ __launch_bounds__(NT, 1) void gemm_split_kernel(const wt_gemm_prob<float>* __restrict__ probs,
+                                                            const int64_t* __restrict__ tile_start, int nprob,
+                                                            int64_t ntiles)
+{
+    typedef MF<float> M_;
+    constexpr int BM = 128, BN = 128, KC = 16, NP = 9;
+    __shared__ __align__(16) __bf16 sA[2][NP][BM][KC];
+    __shared__ __align__(16) __bf16 sB[2][NP][BN][KC];
+
+    int64_t b = blockIdx.x;
+    {
+        int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, i = b / 8;
+        b = x * q + (x < r ? x : r) + i;
+    }
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const wt_gemm_prob<float> P = probs[lo];
+    const gv2<float>* gA = (const gv2<float>*)P.A;
+    const gv2<float>* gB = (const gv2<float>*)P.B;
+    gv2<float>* gC = (gv2<float>*)P.C;
+    gi32* gperm = (gi32*)P.bperm;
+    int64_t t = b - tile_start[lo];
+    const int tiles_m = (P.M + BM - 1) / BM, tiles_n = (P.N + BN - 1) / BN;
+    constexpr int GM = 8;
+    int64_t grp = t / ((int64_t)GM * tiles_n);
+    int first_m = (int)(grp * GM);
+    int gm = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+    int64_t tin = t - grp * (int64_t)GM * tiles_n;
+    const int tm = first_m + (int)(tin % gm), tn = (int)(tin / gm);
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
+
+    f32x16 acc[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[c][i][j][v] = 0.f;
+
+    const int K = P.K, nkt = (K + KC - 1) / KC;
+    // loader: row (of A) / column (of B) lr of the tile, k chunk kh (8 k)
+    const int lr = tid & 127, kh = tid >> 7;
+    v2<float> ra[8], rb[8];
+    auto load = [&](int kt) {
+        const int k0 = kt * KC + kh * 8;
+        const int m = m0 + lr, n = n0 + lr;
+        const gv2<float>* pa = gA + (int64_t)k0 * P.lda + m;
+        if (m < P.M && k0 + 8 <= K) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) ra[e] = pa[(int64_t)e * P.lda];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                ra[e] = (m < P.M && k0 + e < K) ? pa[(int64_t)e * P.lda] : v2<float>{0.f, 0.f};
+        }
+        if (n < P.N && k0 + 8 <= K && !gperm) {
+            const gv2<float>* pb = gB + (int64_t)n * P.ldb + k0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) rb[e] = pb[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + e;
+                rb[e] = (k < K && n < P.N) ? gB[(int64_t)n * P.ldb + (gperm ? gperm[k] : k)] : v2<float>{0.f, 0.f};
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        const int c = split_chunk(lr, kh) * 8;
+        // one value kind (A Re, Im, Re + Im, then B's) at a time, its three
+        // planes written before the next kind is split
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            bf16x8 ph, pm, pl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const v2<float> v = q < 3 ? ra[e] : rb[e];
+                const float x = (q % 3) == 0 ? v.x : ((q % 3) == 1 ? v.y : v.x + v.y);
+                __bf16 h, m, l;
+                split3(x, h, m, l);
+                ph[e] = h;
+                pm[e] = m;
+                pl[e] = l;
+            }
+            __bf16(*dst)[BM][KC] = q < 3 ? sA[buf] : sB[buf];
+            const int p0 = 3 * (q % 3);
+            *(bf16x8*)&dst[p0][lr][c] = ph;
+            *(bf16x8*)&dst[p0 + 1][lr][c] = pm;
+            *(bf16x8*)&dst[p0 + 2][lr][c] = pl;
+        }
+    };
+
+    if (nkt > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    const int orow = lane & 31, oc = split_chunk(orow, lane >> 5) * 8;
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nkt) load(kt + 1);
+        // X = B^T (rows n), Y = A^T (columns m): D = X Y is a tile of C^T
+        bf16x8 x[2][NP], y[2][NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) x[j][p] = *(const bf16x8*)&sB[cur][p][wn0 + 32 * j + orow][oc];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) y[i][p] = *(const bf16x8*)&sA[cur][p][wm0 + 32 * i + orow][oc];
+        }
+        // pairs (x part, y part), smallest first; twelve independent
+        // accumulators between two updates of one
+        constexpr int PX[6] = {0, 1, 2, 0, 1, 0}, PY[6] = {2, 1, 0, 1, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[c][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[j][3 * c + PX[s]],
+                                                                               y[i][3 * c + PY[s]], acc[c][i][j], 0,
+                                                                               0, 0);
+        if (kt + 1 < nkt) store(cur ^ 1);
+        // issue order: the next slice's loads, this slice's LDS reads, half
+        // of the MFMAs, then the other half each followed by six VALU
+        // operations of the next slice's split, then its LDS writes
+        // (measured 2-4 % over letting the scheduler group the split after
+        // all MFMAs)
+        __builtin_amdgcn_sched_group_barrier(0x020, 16, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 36, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 36, 0);
+#pragma unroll
+        for (int i = 0; i < 36; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x200, 18, 0);
+        __syncthreads();
+    }
+
+    const int mode = P.mode;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int m = m0 + wm0 + i * 32 + M_::dcol(lane);
+                const int n = n0 + wn0 + j * 32 + M_::drow(lane, v);
+                if (m < P.M && n < P.N) {
+                    const float t1 = acc[0][i][j][v], t2 = acc[1][i][j][v], t3 = acc[2][i][j][v];
+                    const float re = t1 - t2, im = (t3 - t1) - t2;
+                    gv2<float>* pc = gC + (int64_t)n * P.ldc + m;
+                    if (mode == WT_GEMM_SET) {
+                        *pc = v2<float>{re, im};
+                    } else if (mode == WT_GEMM_NSET) {
+                        *pc = v2<float>{-re, -im};
+                    } else {
+                        const v2<float> c = *pc;
+                        if (mode == WT_GEMM_SUB) *pc = v2<float>{c.x - re, c.y - im};
+                        else *pc = v2<float>{c.x + re, c.y + im};
+                    }
+                }
+            }
+}
 
 }  // namespace
 
@@ -275,6 +488,14 @@ int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, 
     case 3:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 3>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                            d_tile_start, nprob, ntiles);
+        break;
+    case WT_GEMM_SPLIT:
+        if constexpr (sizeof(R) == 4)
+            hipLaunchKernelGGL(gemm_split_kernel, dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs, d_tile_start,
+                               nprob, ntiles);
+        else
+            hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                               d_tile_start, nprob, ntiles);
         break;
     default:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 0>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,

@@ -302,6 +302,7 @@ static int build_factor_plan(wt_solver_impl& s, wt_numeric<R>& num)
     const int64_t nb = S.nb;
     const int64_t NB2 = WT_FNB2;
     num.fplan.release();
+    num.fplan.allow_split = s.gemm_split;
     num.fsteps.clear();
     // The contribution blocks need no clearing: each front's CB is first
     // written by its own CB = -L21 U12 product, and its children's
@@ -457,8 +458,13 @@ static int enqueue_factor(wt_solver_impl& s, wt_numeric<R>& num, std::vector<hip
             break;
         case WT_STEP_GEMM:
             if (ev) (void)hipEventRecord((*ev)[2 * t.gemm], st);
-            rc = P.run(st, num.fplan.launches[t.gemm].var ? WT_PROF_GEMM_FACTOR_NARROW : WT_PROF_GEMM_FACTOR,
-                       num.fplan.launches[t.gemm].flops, 0, [&] { return num.fplan.run(t.gemm, st); });
+        {
+            const int v = num.fplan.launches[t.gemm].var;
+            const int cls = v == 0 ? WT_PROF_GEMM_FACTOR
+                                   : (v == WT_GEMM_SPLIT && sizeof(R) == 4 ? WT_PROF_GEMM_FACTOR_SPLIT
+                                                                           : WT_PROF_GEMM_FACTOR_NARROW);
+            rc = P.run(st, cls, num.fplan.launches[t.gemm].flops, 0, [&] { return num.fplan.run(t.gemm, st); });
+        }
             if (ev) (void)hipEventRecord((*ev)[2 * t.gemm + 1], st);
             break;
         default: rc = WT_ERR_STATE;
@@ -557,7 +563,7 @@ int wt_solver_autotune(wt_solver_impl& s, wt_numeric<R>& num, double* speedup)
         if (hipEventCreate(&e) != hipSuccess) rc = WT_ERR_HIP;
     const bool prof = s.prof.on;
     s.prof.on = false;
-    for (int v = 0; v < WT_GEMM_NVAR && rc == WT_OK; ++v) {
+    for (int v = 0; v < (s.gemm_split ? wt_gemm_nvar<R>() : WT_GEMM_SPLIT) && rc == WT_OK; ++v) {
         for (auto& L : num.fplan.launches) L.var = v;
         if ((rc = enqueue_factor(s, num, &ev)) || (rc = factor_finish(s, num))) break;
         for (size_t i = 0; i < nl; ++i) {
@@ -600,6 +606,7 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
     dfree(num.d_sp);
     num.hsp.clear();
     num.splan.release();
+    num.splan.allow_split = s.gemm_split;
     num.fwd.clear();
     num.bwd.clear();
     int rc;
@@ -716,7 +723,7 @@ static int tune_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
         if (hipEventCreate(&e) != hipSuccess) rc = WT_ERR_HIP;
     const bool prof = s.prof.on;
     s.prof.on = false;
-    for (int v = 0; v < WT_GEMM_NVAR && rc == WT_OK; ++v) {
+    for (int v = 0; v < (s.gemm_split ? wt_gemm_nvar<R>() : WT_GEMM_SPLIT) && rc == WT_OK; ++v) {
         for (auto& L : num.splan.launches) L.var = v;
         if ((rc = enqueue_solve(s, num, nrhs, &ev))) break;
         if (hipStreamSynchronize(s.stream) != hipSuccess) { rc = WT_ERR_HIP; break; }
@@ -853,6 +860,8 @@ template int wt_solver_plan_tiles<float>(wt_solver_impl&, wt_numeric<float>&, in
                                          int64_t*);
 template int wt_solver_plan_tiles<double>(wt_solver_impl&, wt_numeric<double>&, int, int, int32_t*, const int32_t*,
                                           int64_t*);
+template void wt_numeric<float>::drop_graphs();
+template void wt_numeric<double>::drop_graphs();
 template int wt_solver_factor_run<float>(wt_solver_impl&, wt_numeric<float>&);
 template int wt_solver_factor_run<double>(wt_solver_impl&, wt_numeric<double>&);
 template int wt_solver_factor_launch<float>(wt_solver_impl&, wt_numeric<float>&);

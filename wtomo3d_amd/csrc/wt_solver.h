@@ -29,6 +29,7 @@ struct wt_gemm_plan {
     int64_t* d_tiles = nullptr;  // WT_GEMM_NVAR arrays of tiles[0].size(), variant-major
     int64_t tstride = 0;
     wt_gemm_launch_rec cur{};
+    bool allow_split = true;  // fp32: the bf16-split tile may be chosen
     void begin()
     {
         cur.prob_off = (int64_t)probs.size();
@@ -73,6 +74,9 @@ struct wt_gemm_plan {
         const double a0 = (double)cur.ntiles[0] * wt_gemm_tile<R, 0>::BM * wt_gemm_tile<R, 0>::BN;
         const double a2 = (double)cur.ntiles[2] * wt_gemm_tile<R, 2>::BM * wt_gemm_tile<R, 2>::BN;
         cur.var = (!narrow && cur.ntiles[0] >= 1024 && a0 <= 1.15 * a2) ? 0 : 2;
+        // fp32 large launches: the same 128 x 128 tile with its products as
+        // bf16 splits, unless the handle keeps to the fp32 matrix cores
+        if (sizeof(R) == 4 && cur.var == 0 && allow_split) cur.var = WT_GEMM_SPLIT;
         // fp64: the 64 x 64 tile throughout (two blocks per CU; measured
         // fastest on large and narrow launches alike)
         if (sizeof(R) == 8) cur.var = 2;
@@ -142,7 +146,8 @@ struct wt_numeric {
 // wt_solver_profile): launches, summed kernel time and algorithmic work.
 enum wt_prof_class {
     WT_PROF_GEMM_FACTOR, WT_PROF_GEMM_SOLVE, WT_PROF_DIAG, WT_PROF_EXTADD, WT_PROF_FILL,
-    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_TRSM, WT_PROF_GEMM_FACTOR_NARROW, WT_PROF_NCLASS
+    WT_PROF_ASSEMBLE, WT_PROF_RESIDUAL, WT_PROF_PERMUTE, WT_PROF_TRSM, WT_PROF_GEMM_FACTOR_NARROW,
+    WT_PROF_GEMM_FACTOR_SPLIT, WT_PROF_NCLASS
 };
 struct wt_prof {
     bool on = false;
@@ -248,6 +253,7 @@ struct wt_solver_impl {
     int factored_prec = 0;
     double last_factor_ms = 0;
     bool use_graphs = false;  // capture and replay the factorization
+    bool gemm_split = true;   // fp32 GEMMs may run as bf16 splits (variant WT_GEMM_SPLIT)
 };
 
 struct wt_grid_dims {

@@ -116,6 +116,11 @@ class Solver:
         check(lib().wt_solver_autotune(self._h, prec, ctypes.byref(sp)), "wt_solver_autotune")
         return sp.value
 
+    def set_gemm_split(self, enable=True):
+        """complex64 GEMMs as three-way bf16 splits on the bf16 matrix cores
+        (default), or all on the fp32 matrix cores."""
+        check(lib().wt_solver_set_gemm_split(self._h, int(bool(enable))), "wt_solver_set_gemm_split")
+
     def set_graphs(self, enable=True):
         """Capture the factorization in a HIP graph and replay it."""
         check(lib().wt_solver_set_graphs(self._h, int(bool(enable))), "wt_solver_set_graphs")
@@ -137,7 +142,7 @@ class Solver:
         check(lib().wt_solver_release(self._h, prec), "wt_solver_release")
 
     PROFILE_CLASSES = ("gemm_factor", "gemm_solve", "diag_tile", "extend_add", "fill_scatter",
-                       "assemble", "residual", "reorder", "panel_trsm", "gemm_factor_narrow")
+                       "assemble", "residual", "reorder", "panel_trsm", "gemm_factor_narrow", "gemm_factor_split")
 
     def profile(self, enable=True):
         """Enable (and reset) per-kernel HIP-event timing on the solver stream."""
