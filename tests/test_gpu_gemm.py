@@ -53,34 +53,37 @@ def test_gemm_row_permuted_b(prec):
 
 
 @pytest.mark.parametrize("prec", [32, 64])
-@pytest.mark.parametrize("var", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("var", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (64, 64, 64), (130, 70, 9), (257, 129, 300), (65, 65, 8)])
 def test_gemm_tile_variants(prec, var, shape):
     """Every tile variant (the autotuner picks among them per launch) gives
     the product to the precision's rounding; mode SUB with padded leading
-    dimensions.  Variant 4 in fp32 is the three-way bf16 split."""
+    dimensions.  Variants 4 and 5 in fp32 are the three-way bf16 splits."""
     M, N, K = shape
     err = _run(prec, M, N, K, 0, lda=M + 1, ldb=K + 2, ldc=M + 3, seed=11, var=var)
     assert err <= (2e-6 if prec == 32 else 1e-14)
 
 
+@pytest.mark.parametrize("var", [4, 5])
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("shape", [(300, 260, 2000), (129, 257, 33), (1000, 64, 512)])
-def test_split_products_as_accurate_as_fp32(mode, shape):
-    """The bf16 three-way split (fp32 variant 4) against complex128: its
-    error, relative to |A||B| + |C| entry by entry, stays within 1.5x that
-    of the fp32 matrix-core product (variant 0) and below 2e-6, on long K
-    and ragged tiles, in every mode."""
+def test_split_products_as_accurate_as_fp32(var, mode, shape):
+    """The bf16 three-way splits (fp32 variants 4 and 5) against complex128:
+    the error, relative to |A||B| + |C| entry by entry, stays within 1.5x
+    that of the fp32 matrix-core product (variant 0) and below 2e-6, on long
+    K and ragged tiles, in every mode."""
     M, N, K = shape
     e0 = _run(32, M, N, K, mode, lda=M + 2, ldb=K + 1, ldc=M + 1, seed=5, var=0)
-    e4 = _run(32, M, N, K, mode, lda=M + 2, ldb=K + 1, ldc=M + 1, seed=5, var=4)
+    e4 = _run(32, M, N, K, mode, lda=M + 2, ldb=K + 1, ldc=M + 1, seed=5, var=var)
     assert e4 <= 2e-6 and e4 <= 1.5 * e0 + 1e-8, (e4, e0)
 
 
-def test_split_row_permuted_b_and_extreme_scales():
-    """Variant 4 with a row-permuted B, and with operands spanning 2^-60 to
-    2^60 (every split part stays normal in bf16's fp32 exponent range)."""
-    assert _run(32, 64, 200, 64, 1, perm=True, seed=3, var=4) <= 2e-6
+@pytest.mark.parametrize("var", [4, 5])
+def test_split_row_permuted_b_and_extreme_scales(var):
+    """The split variants with a row-permuted B, and with operands spanning
+    2^-60 to 2^60 (every split part stays normal in bf16's fp32 exponent
+    range)."""
+    assert _run(32, 64, 200, 64, 1, perm=True, seed=3, var=var) <= 2e-6
     rng = np.random.default_rng(9)
     M, N, K = 70, 90, 300
     A = (rng.standard_normal((K, M)) * 2.0 ** rng.integers(-60, 60, (K, M))).astype(np.float32)
@@ -88,7 +91,7 @@ def test_split_row_permuted_b_and_extreme_scales():
     B = (rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))).astype(np.complex64)
     C = np.zeros((N, M), np.complex64)
     ms = ctypes.c_double()
-    assert W.lib().wt_gemm_host_var(32, 4, M, N, K, P(A), M, P(B), K, P(C), M, 1, None, 1, ctypes.byref(ms)) == 0
+    assert W.lib().wt_gemm_host_var(32, var, M, N, K, P(A), M, P(B), K, P(C), M, 1, None, 1, ctypes.byref(ms)) == 0
     Am, Bm = A.T.astype(np.complex128), B.T.astype(np.complex128)
     err = np.max(np.abs(C.T - Am @ Bm) / (np.abs(Am) @ np.abs(Bm)))
     assert err <= 2e-6

@@ -433,6 +433,7 @@ int wt_solver_set_gemm_split(wt_solver* h, int enable)
         pl->allow_split = s.gemm_split;
         for (auto& L : pl->launches) {
             if (!s.gemm_split && L.var == WT_GEMM_SPLIT) L.var = 0;
+            else if (!s.gemm_split && L.var == WT_GEMM_SPLIT64) L.var = 2;
             else if (s.gemm_split && L.var == 0) L.var = WT_GEMM_SPLIT;
         }
     }

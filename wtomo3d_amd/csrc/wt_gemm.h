@@ -34,13 +34,14 @@ struct wt_gemm_prob {
 // 128 x 64 (fp64: 64 x 128) tile between them.  Every variant
 // has BM, BN >= 64 = WT_NB, which the aliased panel products rely on (one
 // tile spans all M, or all N, of a 64-wide panel).  The K loop is the same
-// in variants 0-3, so they give identical results.  Variant 4 (fp32 only) is
-// a 128 x 128 tile whose real products run on the bf16 matrix cores as
-// three-way splits of each fp32 operand (wt_gemm.hip): results agree with
-// variants 0-3 to fp32 rounding, not bit for bit.  For fp64, variant 4 is
-// variant 2 again.
-#define WT_GEMM_NVAR 5
+// in variants 0-3, so they give identical results.  Variants 4 and 5 (fp32
+// only) are 128 x 128 and 64 x 64 tiles whose real products run on the bf16
+// matrix cores as three-way splits of each fp32 operand (wt_gemm.hip):
+// results agree with variants 0-3 to fp32 rounding, not bit for bit.  For
+// fp64, variants 4 and 5 are variant 2 again.
+#define WT_GEMM_NVAR 6
 #define WT_GEMM_SPLIT 4
+#define WT_GEMM_SPLIT64 5
 template <typename R, int V>
 struct wt_gemm_tile;
 template <>
@@ -63,8 +64,12 @@ template <>
 struct wt_gemm_tile<float, 4> { static constexpr int BM = 128, BN = 128, WGM = 2, WGN = 2; };
 template <>
 struct wt_gemm_tile<double, 4> { static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2; };
+template <>
+struct wt_gemm_tile<float, 5> { static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2; };
+template <>
+struct wt_gemm_tile<double, 5> { static constexpr int BM = 64, BN = 64, WGM = 2, WGN = 2; };
 
-// Variants worth measuring for precision R (fp64's variant 4 repeats 2).
+// Variants worth measuring for precision R (fp64's variants 4 and 5 repeat 2).
 template <typename R>
 constexpr int wt_gemm_nvar()
 {
@@ -80,6 +85,7 @@ inline int64_t wt_gemm_tiles(int64_t M, int64_t N, int var = 0)
     case 2: BM = wt_gemm_tile<R, 2>::BM; BN = wt_gemm_tile<R, 2>::BN; break;
     case 3: BM = wt_gemm_tile<R, 3>::BM; BN = wt_gemm_tile<R, 3>::BN; break;
     case 4: BM = wt_gemm_tile<R, 4>::BM; BN = wt_gemm_tile<R, 4>::BN; break;
+    case 5: BM = wt_gemm_tile<R, 5>::BM; BN = wt_gemm_tile<R, 5>::BN; break;
     default: BM = wt_gemm_tile<R, 0>::BM; BN = wt_gemm_tile<R, 0>::BN; break;
     }
     return ((M + BM - 1) / BM) * ((N + BN - 1) / BN);

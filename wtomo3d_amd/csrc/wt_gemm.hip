@@ -291,12 +291,18 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // 16-byte chunk (0 or 1) of a row's 16 k as stored: swapped every 4 rows
 __device__ __forceinline__ int split_chunk(int row, int chunk) { return chunk ^ ((row >> 2) & 1); }
 
-__global__ __launch_bounds__(NT, 1) void gemm_split_kernel(const wt_gemm_prob<float>* __restrict__ probs,
-                                                            const int64_t* __restrict__ tile_start, int nprob,
-                                                            int64_t ntiles)
+// T x T output tile (T = 128: waves of 64 x 64, one block per CU, every
+// thread stages one A row and one B column of a slice; T = 64: waves of
+// 32 x 32, two blocks per CU, threads 0-127 stage A and 128-255 stage B).
+template <int T>
+__global__ __launch_bounds__(NT, T == 128 ? 1 : 2) void gemm_split_kernel(const wt_gemm_prob<float>* __restrict__ probs,
+                                                                          const int64_t* __restrict__ tile_start,
+                                                                          int nprob, int64_t ntiles)
 {
     typedef MF<float> M_;
-    constexpr int BM = 128, BN = 128, KC = 16, NP = 9;
+    constexpr int BM = T, BN = T, KC = 16, NP = 9;
+    constexpr int SB = T / 64;           // 32 x 32 MFMA blocks per wave and dimension
+    constexpr bool BOTH = T == 128;      // every thread stages both operands
     __shared__ __align__(16) __bf16 sA[2][NP][BM][KC];
     __shared__ __align__(16) __bf16 sB[2][NP][BN][KC];
 
@@ -327,68 +333,88 @@ __global__ __launch_bounds__(NT, 1) void gemm_split_kernel(const wt_gemm_prob<fl
     const int m0 = tm * BM, n0 = tn * BN;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
+    const int wm0 = (wave & 1) * (T / 2), wn0 = (wave >> 1) * (T / 2);
 
-    f32x16 acc[3][2][2];
+    f32x16 acc[3][SB][SB];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < SB; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < SB; ++j)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[c][i][j][v] = 0.f;
 
     const int K = P.K, nkt = (K + KC - 1) / KC;
-    // loader: row (of A) / column (of B) lr of the tile, k chunk kh (8 k)
-    const int lr = tid & 127, kh = tid >> 7;
+    // loader: row (of A) / column (of B) lr of the tile, k chunk kh (8 k);
+    // with !BOTH, isb selects the operand this thread stages
+    const int lr = tid & (T - 1), kh = BOTH ? tid >> 7 : (tid >> 6) & 1;
+    const bool isb = !BOTH && tid >= 128;
     v2<float> ra[8], rb[8];
-    auto load = [&](int kt) {
-        const int k0 = kt * KC + kh * 8;
-        const int m = m0 + lr, n = n0 + lr;
+    auto load_a = [&](int k0, v2<float> (&r)[8]) {
+        const int m = m0 + lr;
         const gv2<float>* pa = gA + (int64_t)k0 * P.lda + m;
         if (m < P.M && k0 + 8 <= K) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) ra[e] = pa[(int64_t)e * P.lda];
+            for (int e = 0; e < 8; ++e) r[e] = pa[(int64_t)e * P.lda];
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e)
-                ra[e] = (m < P.M && k0 + e < K) ? pa[(int64_t)e * P.lda] : v2<float>{0.f, 0.f};
+                r[e] = (m < P.M && k0 + e < K) ? pa[(int64_t)e * P.lda] : v2<float>{0.f, 0.f};
         }
+    };
+    auto load_b = [&](int k0, v2<float> (&r)[8]) {
+        const int n = n0 + lr;
         if (n < P.N && k0 + 8 <= K && !gperm) {
             const gv2<float>* pb = gB + (int64_t)n * P.ldb + k0;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) rb[e] = pb[e];
+            for (int e = 0; e < 8; ++e) r[e] = pb[e];
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int k = k0 + e;
-                rb[e] = (k < K && n < P.N) ? gB[(int64_t)n * P.ldb + (gperm ? gperm[k] : k)] : v2<float>{0.f, 0.f};
+                r[e] = (k < K && n < P.N) ? gB[(int64_t)n * P.ldb + (gperm ? gperm[k] : k)] : v2<float>{0.f, 0.f};
             }
         }
     };
-    auto store = [&](int buf) {
+    auto load = [&](int kt) {
+        const int k0 = kt * KC + kh * 8;
+        if (BOTH) {
+            load_a(k0, ra);
+            load_b(k0, rb);
+        } else if (isb) {
+            load_b(k0, ra);
+        } else {
+            load_a(k0, ra);
+        }
+    };
+    // one value kind (Re, Im, Re + Im) of one operand at a time, its three
+    // planes written before the next kind is split
+    auto store_kinds = [&](const v2<float> (&r)[8], __bf16 (*dst)[BM][KC]) {
         const int c = split_chunk(lr, kh) * 8;
-        // one value kind (A Re, Im, Re + Im, then B's) at a time, its three
-        // planes written before the next kind is split
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < 3; ++q) {
             bf16x8 ph, pm, pl;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const v2<float> v = q < 3 ? ra[e] : rb[e];
-                const float x = (q % 3) == 0 ? v.x : ((q % 3) == 1 ? v.y : v.x + v.y);
+                const float x = q == 0 ? r[e].x : (q == 1 ? r[e].y : r[e].x + r[e].y);
                 __bf16 h, m, l;
                 split3(x, h, m, l);
                 ph[e] = h;
                 pm[e] = m;
                 pl[e] = l;
             }
-            __bf16(*dst)[BM][KC] = q < 3 ? sA[buf] : sB[buf];
-            const int p0 = 3 * (q % 3);
-            *(bf16x8*)&dst[p0][lr][c] = ph;
-            *(bf16x8*)&dst[p0 + 1][lr][c] = pm;
-            *(bf16x8*)&dst[p0 + 2][lr][c] = pl;
+            *(bf16x8*)&dst[3 * q][lr][c] = ph;
+            *(bf16x8*)&dst[3 * q + 1][lr][c] = pm;
+            *(bf16x8*)&dst[3 * q + 2][lr][c] = pl;
+        }
+    };
+    auto store = [&](int buf) {
+        if (BOTH) {
+            store_kinds(ra, sA[buf]);
+            store_kinds(rb, sB[buf]);
+        } else {
+            store_kinds(ra, isb ? sB[buf] : sA[buf]);
         }
     };
 
@@ -402,15 +428,15 @@ __global__ __launch_bounds__(NT, 1) void gemm_split_kernel(const wt_gemm_prob<fl
         const int cur = kt & 1;
         if (kt + 1 < nkt) load(kt + 1);
         // X = B^T (rows n), Y = A^T (columns m): D = X Y is a tile of C^T
-        bf16x8 x[2][NP], y[2][NP];
+        bf16x8 x[SB][NP], y[SB][NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
 #pragma unroll
-            for (int j = 0; j < 2; ++j) x[j][p] = *(const bf16x8*)&sB[cur][p][wn0 + 32 * j + orow][oc];
+            for (int j = 0; j < SB; ++j) x[j][p] = *(const bf16x8*)&sB[cur][p][wn0 + 32 * j + orow][oc];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) y[i][p] = *(const bf16x8*)&sA[cur][p][wm0 + 32 * i + orow][oc];
+            for (int i = 0; i < SB; ++i) y[i][p] = *(const bf16x8*)&sA[cur][p][wm0 + 32 * i + orow][oc];
         }
-        // pairs (x part, y part), smallest first; twelve independent
+        // pairs (x part, y part), smallest first; 3 SB^2 independent
         // accumulators between two updates of one
         constexpr int PX[6] = {0, 1, 2, 0, 1, 0}, PY[6] = {2, 1, 0, 1, 0, 0};
 #pragma unroll
@@ -418,35 +444,37 @@ __global__ __launch_bounds__(NT, 1) void gemm_split_kernel(const wt_gemm_prob<fl
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < SB; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
+                    for (int j = 0; j < SB; ++j)
                         acc[c][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[j][3 * c + PX[s]],
                                                                                y[i][3 * c + PY[s]], acc[c][i][j], 0,
                                                                                0, 0);
         if (kt + 1 < nkt) store(cur ^ 1);
-        // issue order: the next slice's loads, this slice's LDS reads, half
-        // of the MFMAs, then the other half each followed by six VALU
-        // operations of the next slice's split, then its LDS writes
-        // (measured 2-4 % over letting the scheduler group the split after
-        // all MFMAs)
-        __builtin_amdgcn_sched_group_barrier(0x020, 16, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 36, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 36, 0);
+        if constexpr (BOTH) {
+            // issue order: the next slice's loads, this slice's LDS reads,
+            // half of the MFMAs, then the other half each followed by six
+            // VALU operations of the next slice's split, then its LDS writes
+            // (measured 2-4 % over letting the scheduler group the split
+            // after all MFMAs; the 64 tile has two blocks per CU to overlap)
+            __builtin_amdgcn_sched_group_barrier(0x020, 16, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 36, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 36, 0);
 #pragma unroll
-        for (int i = 0; i < 36; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+            for (int i = 0; i < 36; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x200, 18, 0);
         }
-        __builtin_amdgcn_sched_group_barrier(0x200, 18, 0);
         __syncthreads();
     }
 
     const int mode = P.mode;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < SB; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int m = m0 + wm0 + i * 32 + M_::dcol(lane);
@@ -491,8 +519,16 @@ int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, 
         break;
     case WT_GEMM_SPLIT:
         if constexpr (sizeof(R) == 4)
-            hipLaunchKernelGGL(gemm_split_kernel, dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs, d_tile_start,
-                               nprob, ntiles);
+            hipLaunchKernelGGL((gemm_split_kernel<128>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                               d_tile_start, nprob, ntiles);
+        else
+            hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                               d_tile_start, nprob, ntiles);
+        break;
+    case WT_GEMM_SPLIT64:
+        if constexpr (sizeof(R) == 4)
+            hipLaunchKernelGGL((gemm_split_kernel<64>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                               d_tile_start, nprob, ntiles);
         else
             hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                                d_tile_start, nprob, ntiles);
