@@ -147,7 +147,7 @@ def reference_assembly(nx, ny, nz, dx, freq, vp, vs, rho, gpu_ms):
 
 
 GEMM_KERNEL = "gemm_batched_kernel<float, 0>"
-SPLIT_KERNEL = "gemm_split_kernel"
+SPLIT_KERNEL = "gemm_split_kernel<128>"
 
 
 def gemm_traffic(kernel):

@@ -364,7 +364,8 @@ int wt_receivers_destroy(wt_receivers *r);
  * (fp32 matrix cores), 1 solve GEMMs, 2 diagonal-tile LU, 3 extend-add,
  * 4 fills / value scatter, 5 assembly, 6 FP64 residual, 7 reorder /
  * gather, 8 super-panel triangular solves, 9 factor GEMMs on the other
- * tiles, 10 complex64 factor GEMMs as bf16 splits (wt_solver_set_gemm_split).
+ * tiles (64 x 64 bf16 splits included), 10 complex64 factor GEMMs as bf16
+ * splits on the 128 x 128 tile (wt_solver_set_gemm_split).
  * _read returns launches, summed kernel milliseconds,
  * algorithmic flops (8 M N K per complex GEMM, 8 per residual complex
  * multiply-add) and algorithmic bytes.  No counterpart in the reference

@@ -461,7 +461,7 @@ static int enqueue_factor(wt_solver_impl& s, wt_numeric<R>& num, std::vector<hip
         {
             const int v = num.fplan.launches[t.gemm].var;
             const int cls = v == 0 ? WT_PROF_GEMM_FACTOR
-                                   : (v >= WT_GEMM_SPLIT && sizeof(R) == 4 ? WT_PROF_GEMM_FACTOR_SPLIT
+                                   : (v == WT_GEMM_SPLIT && sizeof(R) == 4 ? WT_PROF_GEMM_FACTOR_SPLIT
                                                                            : WT_PROF_GEMM_FACTOR_NARROW);
             rc = P.run(st, cls, num.fplan.launches[t.gemm].flops, 0, [&] { return num.fplan.run(t.gemm, st); });
         }
