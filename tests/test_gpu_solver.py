@@ -152,7 +152,8 @@ def test_tile_table_round_trip():
         s.autotune(32)
         X1, _ = s.solve(B, prec=32, tol=1e-12, max_iter=30)
         tf, ts = s.get_tiles(32, 0), s.get_tiles(32, 1, 3)
-    assert tf.size > 0 and ts.size > 0 and set(np.unique(tf)) <= {0, 1, 2, 3}
+    # complex64 plans may use the four fp32 tiles and the two bf16-split tiles
+    assert tf.size > 0 and ts.size > 0 and set(np.unique(np.concatenate([tf, ts]))) <= set(range(6))
     with Solver(nx, ny, nz) as s:
         s.set_values(vals)
         s.set_tiles(tf, 32, 0)
