@@ -204,6 +204,13 @@ int wt_solver_set_graphs(wt_solver *h, int enable);
  * plans built later and maps the current ones' large tiles.  No
  * counterpart in the reference (STRUMPACK's complex64 GEMMs are fp32). */
 int wt_solver_set_gemm_split(wt_solver *h, int enable);
+/* Back substitution: the update products y_p -= U12 x_u of a tree level,
+ * few output tiles with a long K near the root, are split along K into
+ * pieces of at least min_depth (default 512) whose partial products are
+ * summed in a fixed order; 0 keeps them whole.  The split depends on the
+ * tree only, so a column is solved the same in any block.  Applies from the
+ * next solve.  No counterpart in the reference (STRUMPACK's solve). */
+int wt_solver_set_splitk(wt_solver *h, int min_depth);
 int wt_solver_set_tiles(wt_solver *h, int factor_prec, int which, int nrhs, const int32_t *vars, int64_t count);
 int wt_solver_release(wt_solver *h, int factor_prec);
 

@@ -123,6 +123,8 @@ def _declare(L):
     L.wt_solver_pivot_report.restype = ctypes.c_int
     L.wt_solver_set_gemm_split.argtypes = [_vp, i32]
     L.wt_solver_set_gemm_split.restype = ctypes.c_int
+    L.wt_solver_set_splitk.argtypes = [_vp, i32]
+    L.wt_solver_set_splitk.restype = ctypes.c_int
     L.wt_solver_get_tiles.argtypes = [_vp, i32, i32, i32, _vp, ctypes.POINTER(ctypes.c_int64)]
     L.wt_solver_get_tiles.restype = ctypes.c_int
     L.wt_solver_set_tiles.argtypes = [_vp, i32, i32, i32, _vp, i64]

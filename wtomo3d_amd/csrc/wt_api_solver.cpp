@@ -82,27 +82,44 @@ int refine(wt_solver_impl& s, wt_numeric<R>& num, int nrhs, const cpx<double>* b
     if ((rc = wt_colnorm2(b, n, nrhs, s.partial, ncb < wt_residual_blocks(n) ? ncb : wt_residual_blocks(n),
                           s.norms + nrhs, q)))
         return rc;
-    WT_HIP(hipMemsetAsync(x, 0, n * nrhs * sizeof(cpx<double>), q));
+    if (max_iter < 1) WT_HIP(hipMemsetAsync(x, 0, n * nrhs * sizeof(cpx<double>), q));
     const cpx<double>* r = b;
     std::vector<double> h(2 * nrhs);
     int it = 0;
     double rel = INFINITY, first = INFINITY, prev = INFINITY;
     bool stalled = false;
+    // For 17..64 columns the residual kernel writes the next solve's input
+    // (scaled, in elimination order) itself, so r is never stored.
+    const bool fused = wt_residual_elim_ok(n, nrhs);
+    bool y_ready = false;
     while (it < max_iter) {
         wt_prof& P = s.prof;
         const double cs = sizeof(cpx<R>), blk = (double)n * nrhs;
         if ((rc = wt_solver_prepare_solve<R>(s, num, nrhs))) return rc;
-        if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (16 + cs) + n * 8.0,
+        if (!y_ready &&
+            (rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (16 + cs) + n * 8.0,
                         [&] { return wt_to_elim<double, R>(r, s.iperm, s.rscale, num.y, n, nrhs, q); })))
             return rc;
         if ((rc = wt_solver_solve_block<R>(s, num, nrhs, nullptr))) return rc;
-        if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * (32 + cs) + n * 8.0,
-                        [&] { return wt_from_elim<R, double>(num.y, s.iperm, x, n, nrhs, 1, q); })))
+        // x = d on the first step (x starts at zero), x += d after
+        const int acc = it > 0;
+        if ((rc = P.run(q, WT_PROF_PERMUTE, 0, blk * ((acc ? 32 : 16) + cs) + n * 8.0,
+                        [&] { return wt_from_elim<R, double>(num.y, s.iperm, x, n, nrhs, acc, q); })))
             return rc;
-        if ((rc = P.run(q, WT_PROF_RESIDUAL, 8.0 * s.nnz * nrhs, s.nnz * 20.0 + (n + 1) * 4.0 + blk * 48.0, [&] {
-                 return wt_residual<double>(s.rowptr, s.colidx, s.vals, x, b, s.rb, s.partial, s.norms, n, nrhs, q);
-             })))
+        if (fused) {
+            if ((rc = P.run(q, WT_PROF_RESIDUAL, 8.0 * s.nnz * nrhs,
+                            s.nnz * 20.0 + (n + 1) * 4.0 + n * 16.0 + blk * (32.0 + cs), [&] {
+                                return wt_residual_elim<double, R>(s.rowptr, s.colidx, s.vals, x, b, s.perm, s.rscale,
+                                                                   num.y, s.partial, s.norms, n, nrhs, q);
+                            })))
+                return rc;
+            y_ready = true;
+        } else if ((rc = P.run(q, WT_PROF_RESIDUAL, 8.0 * s.nnz * nrhs, s.nnz * 20.0 + (n + 1) * 4.0 + blk * 48.0, [&] {
+                        return wt_residual<double>(s.rowptr, s.colidx, s.vals, x, b, s.rb, s.partial, s.norms, n,
+                                                   nrhs, q);
+                    }))) {
             return rc;
+        }
         WT_HIP(hipMemcpyAsync(h.data(), s.norms, 2 * nrhs * sizeof(double), hipMemcpyDeviceToHost, q));
         WT_HIP(hipStreamSynchronize(q));
         P.flush();
@@ -421,6 +438,20 @@ int wt_solver_set_graphs(wt_solver* h, int enable)
 {
     if (!h) return WT_ERR_ARG;
     h->s.use_graphs = enable != 0;
+    return WT_OK;
+}
+
+int wt_solver_set_splitk(wt_solver* h, int min_depth)
+{
+    if (!h || min_depth < 0) return WT_ERR_ARG;
+    wt_solver_impl& s = h->s;
+    s.splitk_depth = min_depth;
+    // the solve plans are rebuilt (same launches, so stored tile tables stay
+    // valid) at the next solve
+    s.f32.snrhs = 0;
+    s.f64.snrhs = 0;
+    s.f32.drop_graphs();
+    s.f64.drop_graphs();
     return WT_OK;
 }
 

@@ -91,6 +91,23 @@ inline int64_t wt_gemm_tiles(int64_t M, int64_t N, int var = 0)
     return ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
 }
 
+// A product split along K: S partial products, each written (mode SET,
+// ld M) to ws[woff + s M N ...], are summed in order s = 0..S-1 and applied
+// to C with the product's mode by wt_gemm_reduce_launch.
+template <typename R>
+struct wt_gemm_red {
+    cpx<R>* C;
+    int64_t ldc;
+    int64_t woff;  // offset of the first partial in the workspace
+    int32_t M, N, S, mode;
+};
+
+// Launches nred reductions (device array reds) over workspace ws; maxM and
+// maxN bound the entries' M and N.
+template <typename R>
+int wt_gemm_reduce_launch(const wt_gemm_red<R>* d_reds, int nred, int64_t maxM, int64_t maxN, const cpx<R>* ws,
+                          hipStream_t stream);
+
 // Launches nprob problems (device arrays probs[nprob], tile_start[nprob+1],
 // tile_start[0] = 0) as one grid of tile_start[nprob] blocks of variant var.
 template <typename R>

@@ -45,8 +45,23 @@ int wt_gemm_plan<R>::upload()
 {
     dfree(d_probs);
     dfree(d_tiles);
+    dfree(d_reds);
     tstride = (int64_t)tiles[0].size();
     if (dalloc(&d_probs, (int64_t)probs.size()) || dalloc(&d_tiles, WT_GEMM_NVAR * tstride)) return WT_ERR_NOMEM;
+    // the split products' partials: one workspace for the largest launch
+    int64_t need = 0;
+    for (const auto& L : launches) need = L.ws_len > need ? L.ws_len : need;
+    if (need > ws_cap) {
+        dfree(ws);
+        ws_cap = 0;
+        if (dalloc(&ws, need)) return WT_ERR_NOMEM;
+        ws_cap = need;
+    }
+    for (const auto& pt : ws_patch) probs[pt.first].C = ws + pt.second;
+    if (!reds.empty()) {
+        if (dalloc(&d_reds, (int64_t)reds.size())) return WT_ERR_NOMEM;
+        WT_HIP(rt_memcpy(d_reds, reds.data(), reds.size() * sizeof(reds[0]), hipMemcpyHostToDevice));
+    }
     if (!probs.empty())
         WT_HIP(rt_memcpy(d_probs, probs.data(), probs.size() * sizeof(probs[0]), hipMemcpyHostToDevice));
     for (int v = 0; v < WT_GEMM_NVAR && tstride > 0; ++v)
@@ -58,9 +73,14 @@ void wt_gemm_plan<R>::release()
 {
     dfree(d_probs);
     dfree(d_tiles);
+    dfree(d_reds);
+    dfree(ws);
+    ws_cap = 0;
     probs.clear();
     for (int v = 0; v < WT_GEMM_NVAR; ++v) tiles[v].clear();
     launches.clear();
+    reds.clear();
+    ws_patch.clear();
     tstride = 0;
 }
 
@@ -256,7 +276,7 @@ static int upload_symbolic(wt_solver_impl& s)
     s.hcols.resize(s.hlists.size(), 0);
     if ((rc = dalloc(&s.fronts, (int64_t)df.size())) || (rc = dalloc(&s.lists, (int64_t)s.hlists.size())) ||
         (rc = dalloc(&s.cols, (int64_t)s.hcols.size())) ||
-        (rc = dalloc(&s.dst, S.nnz)) || (rc = dalloc(&s.iperm, S.n)) || (rc = dalloc(&s.upd, (int64_t)S.upd_idx.size())) ||
+        (rc = dalloc(&s.dst, S.nnz)) || (rc = dalloc(&s.iperm, S.n)) || (rc = dalloc(&s.perm, S.n)) || (rc = dalloc(&s.upd, (int64_t)S.upd_idx.size())) ||
         (rc = dalloc(&s.relmap, (int64_t)S.relmap.size())) || (rc = dalloc(&s.d_err, 1)))
         return rc;
     WT_HIP(rt_memcpy(s.fronts, df.data(), df.size() * sizeof(wt_dfront), hipMemcpyHostToDevice));
@@ -264,6 +284,7 @@ static int upload_symbolic(wt_solver_impl& s)
     WT_HIP(rt_memcpy(s.cols, s.hcols.data(), s.hcols.size() * 4, hipMemcpyHostToDevice));
     WT_HIP(rt_memcpy(s.dst, S.dst.data(), S.nnz * 8, hipMemcpyHostToDevice));
     WT_HIP(rt_memcpy(s.iperm, S.iperm.data(), S.n * 8, hipMemcpyHostToDevice));
+    WT_HIP(rt_memcpy(s.perm, S.perm.data(), S.n * 8, hipMemcpyHostToDevice));
     if (!S.upd_idx.empty())
         WT_HIP(rt_memcpy(s.upd, S.upd_idx.data(), S.upd_idx.size() * 8, hipMemcpyHostToDevice));
     if (!S.relmap.empty())
@@ -279,7 +300,7 @@ void wt_solver_free(wt_solver_impl& s)
     s.prof.release();
     s.f32.release();
     s.f64.release();
-    dfree(s.rowptr); dfree(s.colidx); dfree(s.dst); dfree(s.iperm); dfree(s.upd); dfree(s.relmap);
+    dfree(s.rowptr); dfree(s.colidx); dfree(s.dst); dfree(s.iperm); dfree(s.perm); dfree(s.upd); dfree(s.relmap);
     dfree(s.fronts); dfree(s.lists); dfree(s.cols); dfree(s.rscale); dfree(s.lam); dfree(s.mu); dfree(s.rho); dfree(s.vals); dfree(s.d_err);
     dfree(s.xb); dfree(s.rb); dfree(s.bb); dfree(s.partial); dfree(s.norms);
     if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -660,12 +681,26 @@ static int build_solve_plan(wt_solver_impl& s, wt_numeric<R>& num, int nrhs)
         const int par = l & 1;
         if (s.lev_upd_cnt[l] > 0) {
             num.bwd.push_back({WT_STEP_SGATHER_UPD, s.lev_upd_cnt[l], 0, s.lev_upd_off[l], par, -1});
+            // y_p -= U12 x_u: few output tiles (p rows, nrhs columns) and a
+            // long K (u) at the top of the tree, so each product is split
+            // along K into pieces of at least splitk_depth until the launch
+            // has about WT_SPLITK_BLOCKS row tiles.  The rule depends on the
+            // tree only (not on nrhs): every column is solved the same in
+            // any block.
+            int64_t mt = 0;
+            for (int f : S.levels[l])
+                if (S.fronts[f].u > 0) mt += (S.fronts[f].p + 63) / 64;
             num.splan.begin();
             for (int f : S.levels[l]) {
                 const wt_front& F = S.fronts[f];
                 if (F.u == 0) continue;
-                num.splan.add(num.factor + F.Uoff, F.p, num.w[par] + F.Woff * nrhs, F.u, y + F.pstart, n, F.p, nrhs,
-                              F.u, WT_GEMM_SUB);
+                int sk = 1;
+                if (s.splitk_depth > 0 && mt > 0) {
+                    const int64_t want = (WT_SPLITK_BLOCKS + mt - 1) / mt, cap = F.u / s.splitk_depth;
+                    sk = (int)std::max<int64_t>(1, std::min<int64_t>(want, cap));
+                }
+                num.splan.add_split(num.factor + F.Uoff, F.p, num.w[par] + F.Woff * nrhs, F.u, y + F.pstart, n, F.p,
+                                    nrhs, F.u, WT_GEMM_SUB, sk);
             }
             int g = num.splan.end();
             if (g >= 0) num.bwd.push_back({WT_STEP_GEMM, 0, 0, 0, par, g});

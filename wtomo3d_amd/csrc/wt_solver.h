@@ -2,6 +2,7 @@
 // launch plans of the numeric factorization and the block solve.
 #pragma once
 #include <cstdint>
+#include <utility>
 #include <vector>
 #include <hip/hip_runtime.h>
 
@@ -18,6 +19,9 @@ struct wt_gemm_launch_rec {
     int32_t nprob;
     int32_t var;
     double flops;  // 8 M N K summed over the launch's problems
+    // products split along K: their reductions, run after the launch
+    int64_t red_off, ws_len, red_maxM, red_maxN;
+    int32_t nred;
 };
 
 template <typename R>
@@ -30,6 +34,14 @@ struct wt_gemm_plan {
     int64_t tstride = 0;
     wt_gemm_launch_rec cur{};
     bool allow_split = true;  // fp32: the bf16-split tile may be chosen
+    // split-K products: reductions, and the partials' workspace (each
+    // launch's partials start at offset 0: launches run in stream order and
+    // a launch's reduction consumes them before the next launch)
+    std::vector<wt_gemm_red<R>> reds;
+    std::vector<std::pair<int64_t, int64_t>> ws_patch;  // (problem, offset): C = ws + offset at upload
+    wt_gemm_red<R>* d_reds = nullptr;
+    cpx<R>* ws = nullptr;
+    int64_t ws_cap = 0;
     void begin()
     {
         cur.prob_off = (int64_t)probs.size();
@@ -40,6 +52,37 @@ struct wt_gemm_plan {
         }
         cur.nprob = 0;
         cur.flops = 0;
+        cur.red_off = (int64_t)reds.size();
+        cur.nred = 0;
+        cur.ws_len = cur.red_maxM = cur.red_maxN = 0;
+    }
+    // C (mode) A B as S products over consecutive K ranges (multiples of 16
+    // deep) whose partials a reduction after the launch sums in order and
+    // applies: more blocks for a product with few output tiles and a long K.
+    // S <= 1 adds the product whole.
+    void add_split(const cpx<R>* A, int64_t lda, const cpx<R>* B, int64_t ldb, cpx<R>* C, int64_t ldc, int64_t M,
+                   int64_t N, int64_t K, int mode, int S)
+    {
+        if (M <= 0 || N <= 0 || K <= 0) return;
+        const int64_t chunk = S > 1 ? ((K + S - 1) / S + 15) / 16 * 16 : K;
+        if (S <= 1 || chunk >= K) {
+            add(A, lda, B, ldb, C, ldc, M, N, K, mode);
+            return;
+        }
+        wt_gemm_red<R> r;
+        r.C = C; r.ldc = ldc; r.woff = cur.ws_len; r.M = (int32_t)M; r.N = (int32_t)N; r.mode = mode;
+        int q = 0;
+        for (int64_t k0 = 0; k0 < K; k0 += chunk, ++q) {
+            const int64_t kk = K - k0 < chunk ? K - k0 : chunk;
+            ws_patch.push_back({(int64_t)probs.size(), cur.ws_len});
+            add(A + k0 * lda, lda, B + k0, ldb, nullptr, M, M, N, kk, WT_GEMM_SET);
+            cur.ws_len += M * N;
+        }
+        r.S = q;
+        reds.push_back(r);
+        ++cur.nred;
+        cur.red_maxM = M > cur.red_maxM ? M : cur.red_maxM;
+        cur.red_maxN = N > cur.red_maxN ? N : cur.red_maxN;
     }
     void add(const cpx<R>* A, int64_t lda, const cpx<R>* B, int64_t ldb, cpx<R>* C, int64_t ldc, int64_t M,
              int64_t N, int64_t K, int mode, const int32_t* bperm = nullptr)
@@ -88,8 +131,11 @@ struct wt_gemm_plan {
     int run(int i, hipStream_t s) const
     {
         const wt_gemm_launch_rec& L = launches[i];
-        return wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.var * tstride + L.tile_off, L.nprob,
-                                 L.ntiles[L.var], L.var, s);
+        int rc = wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.var * tstride + L.tile_off, L.nprob,
+                                   L.ntiles[L.var], L.var, s);
+        if (rc == WT_OK && L.nred > 0)
+            rc = wt_gemm_reduce_launch<R>(d_reds + L.red_off, L.nred, L.red_maxM, L.red_maxN, ws, s);
+        return rc;
     }
 };
 
@@ -213,6 +259,7 @@ struct wt_solver_impl {
     int32_t* colidx = nullptr;
     int64_t* dst = nullptr;
     int64_t* iperm = nullptr;
+    int64_t* perm = nullptr;  // perm[iperm[i]] = i
     int64_t* upd = nullptr;
     int32_t* relmap = nullptr;
     wt_dfront* fronts = nullptr;
@@ -254,7 +301,11 @@ struct wt_solver_impl {
     double last_factor_ms = 0;
     bool use_graphs = false;  // capture and replay the factorization
     bool gemm_split = true;   // fp32 GEMMs may run as bf16 splits (variant WT_GEMM_SPLIT)
+    int splitk_depth = 512;   // least K of a piece of a split back-substitution product (0: no splits)
 };
+
+// Row tiles a back-substitution update launch is split along K to reach.
+#define WT_SPLITK_BLOCKS 2048
 
 struct wt_grid_dims {
     int nx, ny, nz;

@@ -16,6 +16,14 @@ template <typename Ra>
 int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* vals, const cpx<double>* x,
                 const cpx<double>* b, cpx<double>* r, double* partial, double* rnorm2, int64_t n, int nrhs,
                 hipStream_t s);
+// The residual for 17..64 columns (n a multiple of 3) written straight as
+// the next solve's input: y[perm[i] + c n] = (Ry) (scale[i] (b - A x)[i, c])
+// (scale may be null); r itself is not stored.  rnorm2 gets |r_c|^2.
+bool wt_residual_elim_ok(int64_t n, int nrhs);
+template <typename Ra, typename Ry>
+int wt_residual_elim(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* vals, const cpx<double>* x,
+                     const cpx<double>* b, const int64_t* perm, const double* scale, cpx<Ry>* y, double* partial,
+                     double* rnorm2, int64_t n, int nrhs, hipStream_t s);
 int wt_colnorm2(const cpx<double>* v, int64_t n, int nrhs, double* partial, int64_t nblk, double* out,
                 hipStream_t s);
 template <typename R>

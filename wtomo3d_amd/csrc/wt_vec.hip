@@ -72,16 +72,23 @@ __global__ __launch_bounds__(256) void from_elim_kernel(const cpx<R>* __restrict
 // the whole block and the x rows are read as contiguous 64-wide segments.
 // Per-block partial sums of |r|^2 per column go to `partial`.
 constexpr int SP_ROWS = 4;  // rows per block (one per wave)
+#ifndef RES_NPW
+#define RES_NPW 1  // consecutive nodes per wave in the node-wise residual
+#endif
 
-// acc += sg * a x with the rounding fixed by explicit fused multiply-adds,
-// so that every residual kernel forms each term identically.
-template <typename Ra>
-__device__ __forceinline__ void res_term(cpx<double>& acc, const cpx<Ra> a, const cpx<double> xv, double sg)
+// acc -= a x (SUB) or acc += a x as four fused multiply-adds in a fixed
+// order, so that every residual kernel forms each term identically.  The
+// signs are operand modifiers: a complex term costs four FP64 operations,
+// the least there is -- with 64 right-hand sides these kernels do as much
+// FP64 arithmetic per byte of matrix as the vector units can keep up with.
+template <bool SUB, typename Ra>
+__device__ __forceinline__ void res_term(cpx<double>& acc, const cpx<Ra> a, const cpx<double> xv)
 {
-    const double tr = fma((double)a.re, xv.re, -((double)a.im * xv.im));
-    const double ti = fma((double)a.re, xv.im, (double)a.im * xv.re);
-    acc.re += sg * tr;
-    acc.im += sg * ti;
+    const double ar = SUB ? -(double)a.re : (double)a.re, ai = SUB ? -(double)a.im : (double)a.im;
+    acc.re = fma(ar, xv.re, acc.re);
+    acc.re = fma(-ai, xv.im, acc.re);
+    acc.im = fma(ar, xv.im, acc.im);
+    acc.im = fma(ai, xv.re, acc.im);
 }
 
 // r = b - A x (or A x when b is null) for a row-major block of nrhs <= CW
@@ -90,7 +97,7 @@ __device__ __forceinline__ void res_term(cpx<double>& acc, const cpx<Ra> a, cons
 // elastic CSR holds its entries in triplets (the three components of one
 // neighbour node, consecutive columns), so one column index serves three
 // entries; the terms are added in the same order as one entry at a time.
-template <typename Ra, int CW>
+template <typename Ra, int CW, bool SUB>
 __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ colidx,
                                                        const cpx<Ra>* __restrict__ vals,
@@ -111,9 +118,8 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
     const int64_t row = (blk * SP_ROWS + wave) * RPW + sub;
     double nrm = 0.0;
     if (row < n && c < nrhs) {
-        cpx<double> acc = b ? b[row * nrhs + c] : cpx<double>(0.0, 0.0);
-        const double sg = b ? -1.0 : 1.0;
-        auto term = [&](const cpx<Ra> a, const cpx<double> xv) { res_term(acc, a, xv, sg); };
+        cpx<double> acc = SUB ? b[row * nrhs + c] : cpx<double>(0.0, 0.0);
+        auto term = [&](const cpx<Ra> a, const cpx<double> xv) { res_term<SUB>(acc, a, xv); };
         const int32_t e0 = rowptr[row], e1 = rowptr[row + 1];
         if ((e1 - e0) % 3 == 0) {
             for (int32_t e = e0; e < e1; e += 3) {
@@ -152,30 +158,47 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
 // loaded per lane (which also ran 32 columns at 5.2 ms in the row kernel,
 // 3.0 ms here).  A node whose rows do not share one triplet structure is
 // done row by row.
-template <typename Ra>
+//
+// Blocks are mapped to XCDs in contiguous node ranges (the hardware deals
+// workgroups to the 8 XCDs round-robin), so the x rows a node's 27
+// neighbours share are fetched into one XCD's L2 instead of all eight.
+//
+// With `y` set, the kernel writes the solve's next input instead of r:
+// y[perm[i] + c n] = (Ry) (scale[i] r[i, c]) -- the row-equilibrated
+// residual in elimination order, column-major -- and r is not stored.  The
+// three rows of a node are consecutive in elimination order, so each lane
+// stores 3 consecutive entries of its column.
+template <typename Ra, typename Ry, bool SUB, int NPW>
 __global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __restrict__ rowptr,
                                                             const int32_t* __restrict__ colidx,
                                                             const cpx<Ra>* __restrict__ vals,
                                                             const cpx<double>* __restrict__ x,
                                                             const cpx<double>* __restrict__ b,
                                                             cpx<double>* __restrict__ r,
+                                                            const int64_t* __restrict__ perm,
+                                                            const double* __restrict__ scale,
+                                                            cpx<Ry>* __restrict__ y,
                                                             double* __restrict__ partial, int64_t nnode, int nrhs)
 {
     __shared__ double s_part[4 * 64];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int c = threadIdx.x & 63;
-    const int64_t blk = blockIdx.x;
-    const int64_t node = blk * 4 + wave;
+    int64_t blk = blockIdx.x;
+    {
+        const int64_t nb = gridDim.x, q = nb / 8, rr = nb % 8, xx = blk % 8, i = blk / 8;
+        blk = xx * q + (xx < rr ? xx : rr) + i;
+    }
     double nrm = 0.0;
+    for (int np = 0; np < NPW; ++np) {
+    const int64_t node = (blk * 4 + wave) * NPW + np;
     if (node < nnode) {  // wave-uniform
         const bool lane = c < nrhs;
         const int cc = lane ? c : 0;  // idle lanes read column 0 and store nothing
         const int64_t row0 = 3 * node;
-        const double sg = b ? -1.0 : 1.0;
         cpx<double> acc[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) acc[k] = (b && lane) ? b[(row0 + k) * nrhs + cc] : cpx<double>(0.0, 0.0);
-        auto term = [&](cpx<double>& a_, const cpx<Ra> a, const cpx<double> xv) { res_term(a_, a, xv, sg); };
+        for (int k = 0; k < 3; ++k) acc[k] = (SUB && lane) ? b[(row0 + k) * nrhs + cc] : cpx<double>(0.0, 0.0);
+        auto term = [&](cpx<double>& a_, const cpx<Ra> a, const cpx<double> xv) { res_term<SUB>(a_, a, xv); };
         const int32_t e0 = rowptr[row0], e1 = rowptr[row0 + 1], e2 = rowptr[row0 + 2], e3 = rowptr[row0 + 3];
         const int32_t len = e1 - e0;
         if (len == e2 - e1 && len == e3 - e2 && len % 3 == 0) {
@@ -218,11 +241,21 @@ __global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __res
         }
         if (lane) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                r[(row0 + k) * nrhs + c] = acc[k];
-                nrm += acc[k].re * acc[k].re + acc[k].im * acc[k].im;
+            for (int k = 0; k < 3; ++k) nrm += acc[k].re * acc[k].re + acc[k].im * acc[k].im;
+            if (y) {
+                const int64_t n = 3 * nnode;
+                cpx<Ry>* yc = y + (int64_t)c * n;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double sc = scale ? scale[row0 + k] : 1.0;
+                    yc[perm[row0 + k]] = cpx<Ry>((Ry)(sc * acc[k].re), (Ry)(sc * acc[k].im));
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) r[(row0 + k) * nrhs + c] = acc[k];
             }
         }
+    }
     }
     s_part[threadIdx.x] = nrm;
     __syncthreads();
@@ -250,27 +283,30 @@ __global__ void reduce_cols_kernel(const double* __restrict__ partial, int64_t n
     if (threadIdx.x == 0) out[c] = s[0];
 }
 
-// per-column |v|^2 of a row-major block, partial sums per block
-__global__ __launch_bounds__(256) void colnorm_kernel(const cpx<double>* __restrict__ v, int64_t n, int nrhs,
+// per-column |v|^2 of a row-major block, partial sums per block: block b
+// sums its share of the rows, thread t column t % CW of every (256 / CW)-th
+// row, so each pass reads whole rows (contiguous) of the block
+__global__ __launch_bounds__(256) void colnorm_kernel(const cpx<double>* __restrict__ v, int64_t n, int nrhs, int cw,
                                                       double* __restrict__ partial)
 {
     __shared__ double s[256];
-    const int c = blockIdx.y;
+    const int c = threadIdx.x % cw, r0t = threadIdx.x / cw, rstep = 256 / cw;
     double acc = 0.0;
     const int64_t rows_per_blk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_blk;
     const int64_t r1 = r0 + rows_per_blk < n ? r0 + rows_per_blk : n;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
-        cpx<double> a = v[i * nrhs + c];
-        acc += a.re * a.re + a.im * a.im;
-    }
+    if (c < nrhs)
+        for (int64_t i = r0 + r0t; i < r1; i += rstep) {
+            const cpx<double> a = v[i * nrhs + c];
+            acc += a.re * a.re + a.im * a.im;
+        }
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = 128; off >= cw; off >>= 1) {
         if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nrhs + c] = s[0];
+    if ((int)threadIdx.x < nrhs) partial[(int64_t)blockIdx.x * nrhs + threadIdx.x] = s[threadIdx.x];
 }
 
 template <typename R>
@@ -320,29 +356,31 @@ __global__ void planewave_kernel(cpx<double>* __restrict__ x, int nx, int ny, in
 
 // Per-(segment, column) |v|^2 of a row-major block whose rows are cut into
 // contiguous segments (the unknowns of the independent systems of a batch):
-// block (x, c, g) sums rows of segment g in its x-th share.
+// block (x, g) sums rows of segment g in its x-th share, thread t column
+// t % cw of every (256 / cw)-th row (whole rows per pass, as colnorm_kernel).
 __global__ __launch_bounds__(256) void seg_colnorm_kernel(const cpx<double>* __restrict__ v,
-                                                          const int64_t* __restrict__ seg_off, int nrhs,
+                                                          const int64_t* __restrict__ seg_off, int nrhs, int cw,
                                                           double* __restrict__ partial)
 {
     __shared__ double s[256];
-    const int c = blockIdx.y, g = blockIdx.z;
+    const int g = blockIdx.y, c = threadIdx.x % cw, r0t = threadIdx.x / cw, rstep = 256 / cw;
     const int64_t a = seg_off[g], n = seg_off[g + 1] - a;
     const int64_t rows_per_blk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = a + (int64_t)blockIdx.x * rows_per_blk;
     const int64_t r1 = r0 + rows_per_blk < a + n ? r0 + rows_per_blk : a + n;
     double acc = 0.0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
-        cpx<double> x = v[i * nrhs + c];
-        acc += x.re * x.re + x.im * x.im;
-    }
+    if (c < nrhs)
+        for (int64_t i = r0 + r0t; i < r1; i += rstep) {
+            const cpx<double> x = v[i * nrhs + c];
+            acc += x.re * x.re + x.im * x.im;
+        }
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = 128; off >= cw; off >>= 1) {
         if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[((int64_t)g * gridDim.x + blockIdx.x) * nrhs + c] = s[0];
+    if ((int)threadIdx.x < nrhs) partial[((int64_t)g * gridDim.x + blockIdx.x) * nrhs + threadIdx.x] = s[threadIdx.x];
 }
 
 __global__ void seg_reduce_kernel(const double* __restrict__ partial, int nblk, int nrhs, double* __restrict__ out)
@@ -407,9 +445,13 @@ int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* val
 {
     if (nrhs < 1 || nrhs > 64) return WT_ERR_ARG;  // one wave per row covers up to 64 columns
     if (nrhs > 16 && n % 3 == 0) {
-        const int64_t nb = (n / 3 + 3) / 4;  // <= wt_residual_blocks(n)
-        hipLaunchKernelGGL((residual_node_kernel<Ra>), dim3((unsigned)nb), dim3(256), 0, s, rowptr, colidx, vals, x,
-                           b, r, partial, n / 3, nrhs);
+        const int64_t nb = (n / 3 + 4 * RES_NPW - 1) / (4 * RES_NPW);  // <= wt_residual_blocks(n)
+        if (b)
+            hipLaunchKernelGGL((residual_node_kernel<Ra, float, true, RES_NPW>), dim3((unsigned)nb), dim3(256), 0, s,
+                               rowptr, colidx, vals, x, b, r, nullptr, nullptr, nullptr, partial, n / 3, nrhs);
+        else
+            hipLaunchKernelGGL((residual_node_kernel<Ra, float, false, RES_NPW>), dim3((unsigned)nb), dim3(256), 0, s,
+                               rowptr, colidx, vals, x, b, r, nullptr, nullptr, nullptr, partial, n / 3, nrhs);
         WT_HIP(hipGetLastError());
         hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
         WT_HIP(hipGetLastError());
@@ -422,8 +464,12 @@ int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* val
     switch (cw) {
 #define WT_RES(CW)                                                                                              \
     case CW:                                                                                                    \
-        hipLaunchKernelGGL((residual_kernel<Ra, CW>), dim3((unsigned)nb), dim3(256), 0, s, rowptr, colidx, vals, \
-                           x, b, r, partial, n, nrhs);                                                          \
+        if (b)                                                                                                  \
+            hipLaunchKernelGGL((residual_kernel<Ra, CW, true>), dim3((unsigned)nb), dim3(256), 0, s, rowptr,    \
+                               colidx, vals, x, b, r, partial, n, nrhs);                                        \
+        else                                                                                                    \
+            hipLaunchKernelGGL((residual_kernel<Ra, CW, false>), dim3((unsigned)nb), dim3(256), 0, s, rowptr,   \
+                               colidx, vals, x, b, r, partial, n, nrhs);                                        \
         break;
         WT_RES(1) WT_RES(2) WT_RES(4) WT_RES(8) WT_RES(16) WT_RES(32) WT_RES(64)
 #undef WT_RES
@@ -434,9 +480,29 @@ int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* val
     return WT_OK;
 }
 
+bool wt_residual_elim_ok(int64_t n, int nrhs) { return nrhs > 16 && nrhs <= 64 && n % 3 == 0; }
+
+template <typename Ra, typename Ry>
+int wt_residual_elim(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* vals, const cpx<double>* x,
+                     const cpx<double>* b, const int64_t* perm, const double* scale, cpx<Ry>* y, double* partial,
+                     double* rnorm2, int64_t n, int nrhs, hipStream_t s)
+{
+    if (!wt_residual_elim_ok(n, nrhs) || !perm || !y || !b) return WT_ERR_ARG;
+    const int64_t nb = (n / 3 + 4 * RES_NPW - 1) / (4 * RES_NPW);  // <= wt_residual_blocks(n)
+    hipLaunchKernelGGL((residual_node_kernel<Ra, Ry, true, RES_NPW>), dim3((unsigned)nb), dim3(256), 0, s, rowptr,
+                       colidx, vals, x, b, nullptr, perm, scale, y, partial, n / 3, nrhs);
+    WT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
+    WT_HIP(hipGetLastError());
+    return WT_OK;
+}
+
 int wt_colnorm2(const cpx<double>* v, int64_t n, int nrhs, double* partial, int64_t nblk, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(colnorm_kernel, dim3((unsigned)nblk, nrhs), dim3(256), 0, s, v, n, nrhs, partial);
+    if (nrhs < 1 || nrhs > 256) return WT_ERR_ARG;
+    int cw = 1;
+    while (cw < nrhs) cw <<= 1;
+    hipLaunchKernelGGL(colnorm_kernel, dim3((unsigned)nblk), dim3(256), 0, s, v, n, nrhs, cw, partial);
     WT_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nblk, nrhs, out);
     WT_HIP(hipGetLastError());
@@ -489,14 +555,22 @@ template int wt_residual<double>(const int32_t*, const int32_t*, const cpx<doubl
                                  const cpx<double>*, cpx<double>*, double*, double*, int64_t, int, hipStream_t);
 template int wt_residual<float>(const int32_t*, const int32_t*, const cpx<float>*, const cpx<double>*,
                                 const cpx<double>*, cpx<double>*, double*, double*, int64_t, int, hipStream_t);
+template int wt_residual_elim<double, float>(const int32_t*, const int32_t*, const cpx<double>*, const cpx<double>*,
+                                             const cpx<double>*, const int64_t*, const double*, cpx<float>*, double*,
+                                             double*, int64_t, int, hipStream_t);
+template int wt_residual_elim<double, double>(const int32_t*, const int32_t*, const cpx<double>*, const cpx<double>*,
+                                              const cpx<double>*, const int64_t*, const double*, cpx<double>*,
+                                              double*, double*, int64_t, int, hipStream_t);
 template int wt_copy<double>(const cpx<double>*, cpx<double>*, int64_t, hipStream_t);
 template int wt_copy<float>(const cpx<float>*, cpx<float>*, int64_t, hipStream_t);
 
 int wt_seg_colnorm2(const cpx<double>* v, const int64_t* d_seg_off, int nseg, int nrhs, double* partial, int nblk,
                     double* out, hipStream_t s)
 {
-    if (nseg < 1 || nrhs < 1 || nblk < 1 || nseg > 65535 || nrhs > 65535) return WT_ERR_ARG;
-    hipLaunchKernelGGL(seg_colnorm_kernel, dim3((unsigned)nblk, nrhs, nseg), dim3(256), 0, s, v, d_seg_off, nrhs,
+    if (nseg < 1 || nrhs < 1 || nblk < 1 || nseg > 65535 || nrhs > 256) return WT_ERR_ARG;
+    int cw = 1;
+    while (cw < nrhs) cw <<= 1;
+    hipLaunchKernelGGL(seg_colnorm_kernel, dim3((unsigned)nblk, nseg), dim3(256), 0, s, v, d_seg_off, nrhs, cw,
                        partial);
     WT_HIP(hipGetLastError());
     hipLaunchKernelGGL(seg_reduce_kernel, dim3((nrhs + 63) / 64, nseg), dim3(64), 0, s, partial, nblk, nrhs, out);

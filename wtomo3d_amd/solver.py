@@ -121,6 +121,11 @@ class Solver:
         (default), or all on the fp32 matrix cores."""
         check(lib().wt_solver_set_gemm_split(self._h, int(bool(enable))), "wt_solver_set_gemm_split")
 
+    def set_splitk(self, min_depth=512):
+        """Split the back substitution's long-K update products into pieces
+        of at least ``min_depth`` (0: never); from the next solve."""
+        check(lib().wt_solver_set_splitk(self._h, int(min_depth)), "wt_solver_set_splitk")
+
     def set_graphs(self, enable=True):
         """Capture the factorization in a HIP graph and replay it."""
         check(lib().wt_solver_set_graphs(self._h, int(bool(enable))), "wt_solver_set_graphs")
