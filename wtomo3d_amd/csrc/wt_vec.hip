@@ -8,6 +8,8 @@
 // elimination order so the dense front solves are plain GEMMs.
 #include "wt_vec.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int TP = 32;  // transpose tile
@@ -72,9 +74,7 @@ __global__ __launch_bounds__(256) void from_elim_kernel(const cpx<R>* __restrict
 // the whole block and the x rows are read as contiguous 64-wide segments.
 // Per-block partial sums of |r|^2 per column go to `partial`.
 constexpr int SP_ROWS = 4;  // rows per block (one per wave)
-#ifndef RES_NPW
-#define RES_NPW 1  // consecutive nodes per wave in the node-wise residual
-#endif
+constexpr int RES_TB = 3;  // triplets per batch of the node-wise residual
 
 // acc -= a x (SUB) or acc += a x as four fused multiply-adds in a fixed
 // order, so that every residual kernel forms each term identically.  The
@@ -148,16 +148,12 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
 
 // The same residual for blocks of more than 16 columns, one wave per node
 // (its three rows), lane = column.  The three rows of a node have the same
-// columns, and each neighbour node's x rows (three 64-wide segments) are
-// loaded once for the node instead of once per row; each row adds its terms
-// in the same order as the row kernel.  The wave index is made uniform, so
-// the row pointers, column indices and matrix values -- the same for every
-// lane -- come through scalar loads into SGPRs instead of 64-lane vector
-// loads of one address; two triplets' x rows are in flight at a time.
-// Measured at 96x96x64, 64 columns: 3.4 ms, against 7.0 ms with the values
-// loaded per lane (which also ran 32 columns at 5.2 ms in the row kernel,
-// 3.0 ms here).  A node whose rows do not share one triplet structure is
-// done row by row.
+// columns, so each neighbour node's x rows (three 64-wide segments) are
+// loaded once for the node instead of once per row.  The wave loads its
+// node's 3 x len matrix values (up to 243) and len / 3 column indices into
+// LDS with a few coalesced vector loads, then walks the triplets TB at a
+// time, reading the values as LDS broadcasts.  Each row adds its terms in
+// the row kernel's order.
 //
 // Blocks are mapped to XCDs in contiguous node ranges (the hardware deals
 // workgroups to the 8 XCDs round-robin), so the x rows a node's 27
@@ -168,18 +164,27 @@ __global__ __launch_bounds__(256) void residual_kernel(const int32_t* __restrict
 // residual in elimination order, column-major -- and r is not stored.  The
 // three rows of a node are consecutive in elimination order, so each lane
 // stores 3 consecutive entries of its column.
-template <typename Ra, typename Ry, bool SUB, int NPW>
+//
+// Measured at 96x96x64, 64 columns (A x): 3.30 ms; the earlier form with
+// the values through scalar loads two triplets at a time took 3.43 ms, one
+// with them loaded per lane 7.0 ms.  Per node the wave reads 81 x rows of
+// 1 KB through L1 / L2 (49 GB per call, 15 TB/s at 3.3 ms) and does 972
+// FP64 fused multiply-adds per lane (0.93 ms at the FP64 peak).
+template <typename Ra, typename Ry, bool SUB, int TB>
 __global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __restrict__ rowptr,
-                                                            const int32_t* __restrict__ colidx,
-                                                            const cpx<Ra>* __restrict__ vals,
-                                                            const cpx<double>* __restrict__ x,
-                                                            const cpx<double>* __restrict__ b,
-                                                            cpx<double>* __restrict__ r,
-                                                            const int64_t* __restrict__ perm,
-                                                            const double* __restrict__ scale,
-                                                            cpx<Ry>* __restrict__ y,
-                                                            double* __restrict__ partial, int64_t nnode, int nrhs)
+                                                                const int32_t* __restrict__ colidx,
+                                                                const cpx<Ra>* __restrict__ vals,
+                                                                const cpx<double>* __restrict__ x,
+                                                                const cpx<double>* __restrict__ b,
+                                                                cpx<double>* __restrict__ r,
+                                                                const int64_t* __restrict__ perm,
+                                                                const double* __restrict__ scale,
+                                                                cpx<Ry>* __restrict__ y,
+                                                                double* __restrict__ partial, int64_t nnode, int nrhs)
 {
+    constexpr int MAXE = 3 * 81;
+    __shared__ cpx<Ra> s_val[4][MAXE];
+    __shared__ int32_t s_col[4][27];
     __shared__ double s_part[4 * 64];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int c = threadIdx.x & 63;
@@ -188,9 +193,8 @@ __global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __res
         const int64_t nb = gridDim.x, q = nb / 8, rr = nb % 8, xx = blk % 8, i = blk / 8;
         blk = xx * q + (xx < rr ? xx : rr) + i;
     }
+    const int64_t node = blk * 4 + wave;
     double nrm = 0.0;
-    for (int np = 0; np < NPW; ++np) {
-    const int64_t node = (blk * 4 + wave) * NPW + np;
     if (node < nnode) {  // wave-uniform
         const bool lane = c < nrhs;
         const int cc = lane ? c : 0;  // idle lanes read column 0 and store nothing
@@ -201,38 +205,39 @@ __global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __res
         auto term = [&](cpx<double>& a_, const cpx<Ra> a, const cpx<double> xv) { res_term<SUB>(a_, a, xv); };
         const int32_t e0 = rowptr[row0], e1 = rowptr[row0 + 1], e2 = rowptr[row0 + 2], e3 = rowptr[row0 + 3];
         const int32_t len = e1 - e0;
-        if (len == e2 - e1 && len == e3 - e2 && len % 3 == 0) {
-            int32_t t = 0;
-            for (; t + 6 <= len; t += 6) {
-                cpx<double> xv[2][3];
+        if (len == e2 - e1 && len == e3 - e2 && len % 3 == 0 && len <= 81) {
+            const int nt = len / 3;
+            for (int i = c; i < 3 * len; i += 64) s_val[wave][i] = vals[e0 + i];
+            if (c < nt) s_col[wave][c] = colidx[e0 + 3 * c];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const cpx<Ra>* sv = s_val[wave];
+            int t0 = 0;
+            auto batch = [&](auto tb_c) {
+                constexpr int B = decltype(tb_c)::value;
+                cpx<double> xv[B][3];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const cpx<double>* xp = x + (int64_t)colidx[e0 + t + 3 * u] * nrhs + cc;
+                for (int u = 0; u < B; ++u) {
+                    const cpx<double>* xp = x + (int64_t)s_col[wave][t0 + u] * nrhs + cc;
                     xv[u][0] = xp[0];
                     xv[u][1] = xp[nrhs];
                     xv[u][2] = xp[2 * nrhs];
                 }
 #pragma unroll
-                for (int u = 0; u < 2; ++u)
+                for (int u = 0; u < B; ++u) {
+                    const int e = 3 * (t0 + u);
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const int32_t e = e0 + k * len + t + 3 * u;
-                        term(acc[k], vals[e], xv[u][0]);
-                        term(acc[k], vals[e + 1], xv[u][1]);
-                        term(acc[k], vals[e + 2], xv[u][2]);
+                        term(acc[k], sv[k * len + e], xv[u][0]);
+                        term(acc[k], sv[k * len + e + 1], xv[u][1]);
+                        term(acc[k], sv[k * len + e + 2], xv[u][2]);
                     }
-            }
-            for (; t < len; t += 3) {
-                const cpx<double>* xp = x + (int64_t)colidx[e0 + t] * nrhs + cc;
-                const cpx<double> x0 = xp[0], x1 = xp[nrhs], x2 = xp[2 * nrhs];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int32_t e = e0 + k * len + t;
-                    term(acc[k], vals[e], x0);
-                    term(acc[k], vals[e + 1], x1);
-                    term(acc[k], vals[e + 2], x2);
                 }
-            }
+                t0 += B;
+            };
+            while (t0 + TB <= nt) batch(std::integral_constant<int, TB>());
+            while (t0 < nt) batch(std::integral_constant<int, 1>());
         } else {
 #pragma unroll
             for (int k = 0; k < 3; ++k)
@@ -255,7 +260,6 @@ __global__ __launch_bounds__(256) void residual_node_kernel(const int32_t* __res
                 for (int k = 0; k < 3; ++k) r[(row0 + k) * nrhs + c] = acc[k];
             }
         }
-    }
     }
     s_part[threadIdx.x] = nrm;
     __syncthreads();
@@ -445,12 +449,12 @@ int wt_residual(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>* val
 {
     if (nrhs < 1 || nrhs > 64) return WT_ERR_ARG;  // one wave per row covers up to 64 columns
     if (nrhs > 16 && n % 3 == 0) {
-        const int64_t nb = (n / 3 + 4 * RES_NPW - 1) / (4 * RES_NPW);  // <= wt_residual_blocks(n)
+        const int64_t nb = (n / 3 + 3) / 4;  // <= wt_residual_blocks(n)
         if (b)
-            hipLaunchKernelGGL((residual_node_kernel<Ra, float, true, RES_NPW>), dim3((unsigned)nb), dim3(256), 0, s,
+            hipLaunchKernelGGL((residual_node_kernel<Ra, float, true, RES_TB>), dim3((unsigned)nb), dim3(256), 0, s,
                                rowptr, colidx, vals, x, b, r, nullptr, nullptr, nullptr, partial, n / 3, nrhs);
         else
-            hipLaunchKernelGGL((residual_node_kernel<Ra, float, false, RES_NPW>), dim3((unsigned)nb), dim3(256), 0, s,
+            hipLaunchKernelGGL((residual_node_kernel<Ra, float, false, RES_TB>), dim3((unsigned)nb), dim3(256), 0, s,
                                rowptr, colidx, vals, x, b, r, nullptr, nullptr, nullptr, partial, n / 3, nrhs);
         WT_HIP(hipGetLastError());
         hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
@@ -488,8 +492,8 @@ int wt_residual_elim(const int32_t* rowptr, const int32_t* colidx, const cpx<Ra>
                      double* rnorm2, int64_t n, int nrhs, hipStream_t s)
 {
     if (!wt_residual_elim_ok(n, nrhs) || !perm || !y || !b) return WT_ERR_ARG;
-    const int64_t nb = (n / 3 + 4 * RES_NPW - 1) / (4 * RES_NPW);  // <= wt_residual_blocks(n)
-    hipLaunchKernelGGL((residual_node_kernel<Ra, Ry, true, RES_NPW>), dim3((unsigned)nb), dim3(256), 0, s, rowptr,
+    const int64_t nb = (n / 3 + 3) / 4;  // <= wt_residual_blocks(n)
+    hipLaunchKernelGGL((residual_node_kernel<Ra, Ry, true, RES_TB>), dim3((unsigned)nb), dim3(256), 0, s, rowptr,
                        colidx, vals, x, b, nullptr, perm, scale, y, partial, n / 3, nrhs);
     WT_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_cols_kernel, dim3(nrhs), dim3(256), 0, s, partial, nb, nrhs, rnorm2);
