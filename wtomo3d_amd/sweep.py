@@ -65,7 +65,11 @@ def sweep(w, prec=32, tol=1e-8, max_iter=30, compare_fp64=False, leaf_nodes=64):
         nb = n * nsrc * 16
         U, B, X = rhs.DeviceBuffer(nb), rhs.DeviceBuffer(nb), rhs.DeviceBuffer(nb)
         prof_bufs = None
-        for f, src in zip(w["freqs"], w["sources"]):
+        # the first frequency runs once untimed on this handle first: the
+        # numeric arenas of both precisions, the launch plans and the solve
+        # blocks are allocated there, not inside a timed frequency
+        runs = [(w["freqs"][0], w["sources"][0], False)] + [(f, src, True) for f, src in zip(w["freqs"], w["sources"])]
+        for f, src, timed in runs:
             omega = 2 * np.pi * f
             p = grid_params(nx, ny, nz, dx, omega, pmld=w["pmld"])
             L.wt_dev_sync()
@@ -103,8 +107,8 @@ def sweep(w, prec=32, tol=1e-8, max_iter=30, compare_fp64=False, leaf_nodes=64):
                 rec["fp64_rel_residual"] = st64.rel_residual
                 rec["vs_fp64_rel_diff"] = float(np.max(np.linalg.norm(Xh - X64, axis=0) /
                                                        np.linalg.norm(X64, axis=0)))
-                s.release(64)
-            out.append(rec)
+            if timed:
+                out.append(rec)
     return out
 
 
@@ -179,7 +183,6 @@ def benchmark(nx=64, ny=64, nz=48, nfreq=8):
     refinement to 1e-8; then the complex128 factor + one solve of the same
     block for comparison."""
     w = layered_workload(nx, ny, nz, freqs=np.round(np.arange(nfreq) * 0.05 + 0.10, 4))
-    sweep(layered_workload(nx, ny, nz, freqs=[0.3], nsrc=32), prec=32)  # plans and buffers (untimed)
     recs = sweep(w, prec=32, tol=1e-8, max_iter=30, compare_fp64=True)
     t32 = sum(r["factor_s"] + r["solve_s"] for r in recs)
     t64 = sum(r["fp64_s"] for r in recs)
