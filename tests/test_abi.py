@@ -47,3 +47,12 @@ def test_gemm_entry_rejects_unknown_modes_before_any_device_call():
     for mode in (-1, 4, 7):
         rc = L.wt_gemm_host_var(32, -1, 4, 4, 4, None, 4, None, 4, None, 4, mode, None, 1, ctypes.byref(ms))
         assert rc == 1  # WT_ERR_ARG
+
+
+def test_parallel_interface_is_the_empty_stub():
+    """strumpack_parallel_finter: the reference's MPI entry is an empty stub
+    (strumpack_finter.c:123-126); here it does nothing and returns 0."""
+    L = _lib.lib()
+    L.strumpack_parallel_finter.argtypes = []
+    L.strumpack_parallel_finter.restype = __import__("ctypes").c_int
+    assert L.strumpack_parallel_finter() == 0
