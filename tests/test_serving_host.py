@@ -103,6 +103,16 @@ def test_overlapping_same_key_factor_once():
     assert e.used == 0 and not e.cache and e.log.count("close") == 1
 
 
+def test_progress_lines_go_to_stderr(capfd):
+    """The engine's progress report stays off stdout, which carries the
+    benchmark's single result line."""
+    e = MockEngine(slots=2, hbm_budget=10_000, report_every=1, factor_s=0.0, solve_s=0.0)
+    e.run([req(i, i % 2) for i in range(4)], realtime=False)
+    e.close()
+    out, err = capfd.readouterr()
+    assert out == "" and err.count("serving: ") == 4
+
+
 def test_many_overlapping_keys():
     e = MockEngine(slots=4, hbm_budget=100_000, factor_s=0.02, solve_s=0.005)
     rs = [req(i, i % 3, freq=0.1 * (1 + i % 3)) for i in range(12)]

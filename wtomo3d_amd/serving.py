@@ -45,6 +45,7 @@ model's lateral average (``media.background_1d``).
 """
 import collections
 import ctypes
+import sys
 import threading
 import time
 
@@ -466,7 +467,7 @@ class Engine:
                 self.done.extend(rs)
                 if self.report_every and len(self.done) // self.report_every > before // self.report_every:
                     print("serving: %d requests finished, %d pending, %.0f GB reserved"
-                          % (len(self.done), len(self.pending), self.used / 1e9), flush=True)
+                          % (len(self.done), len(self.pending), self.used / 1e9), file=sys.stderr, flush=True)
                 self.cv.notify_all()
 
     def run(self, requests, realtime=True):
@@ -602,7 +603,7 @@ def sustained_rate(rates, seconds, **kw):
         reps.append(rep)
         print("serving: %.2f req/s -> p50 %.0f ms, p99 %.0f ms, backlog %s, sustained %s"
               % (rate, rep["p50_latency_ms"] or -1, rep["p99_latency_ms"] or -1, rep["backlog_at_last_arrival"],
-                 rep["sustained"]), flush=True)
+                 rep["sustained"]), file=sys.stderr, flush=True)
         if not rep["sustained"]:
             break
         best = rate
@@ -625,7 +626,7 @@ if __name__ == "__main__":
     if a.burst_seconds > 0:
         out["burst"] = run_stream(a.burst_rate, a.burst_seconds, group_max=a.group_max)
         print("serving: burst %.1f req/s for %.1f s -> %s" % (a.burst_rate, a.burst_seconds,
-                                                            json.dumps(out["burst"])), flush=True)
+                                                            json.dumps(out["burst"])), file=sys.stderr, flush=True)
     if a.out:
         with open(a.out, "w") as f:
             json.dump(out, f, indent=1)
