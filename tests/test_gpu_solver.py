@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import wtomo3d_amd as W
+from wtomo3d_amd.rhs import DeviceBuffer
 from wtomo3d_amd.solver import Solver
 from solve_cases import problem, rhs_block
 from helpers import rel_err_entries
@@ -83,6 +84,27 @@ def test_fp32_factor_with_fp64_refinement(case):
     err_gpu = np.max(np.abs(X - Xt)) / np.max(np.abs(Xt))
     assert err_gpu <= 10 * err_oracle + 1e-12
     assert st["rel_residual_first"] < 1e-2  # the complex64 factor alone is a good preconditioner
+    s.close()
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (2, 3, 4), (3, 2, 5)])
+def test_tiny_grids_against_dense_lapack(dims):
+    """The smallest grids (one or a few fronts, pivot blocks narrower than a
+    64-row tile, PML over every node): complex128 factor + one solve and
+    complex64 factor + refinement against numpy's dense LAPACK solve."""
+    s, A, vals = _setup(*dims, 64, seed=3)
+    B, Xt = rhs_block(A, 3, 17)
+    Ad = A.toarray()
+    Xd = np.linalg.solve(Ad, B)
+    scale = np.max(np.abs(Xd))
+    s.factor(64)
+    X, st = s.solve(B, prec=64, tol=0.0, max_iter=1)
+    assert SO.rel_residual(A, X, B) <= 1e-12
+    assert np.max(np.abs(X - Xd)) <= 1e-10 * scale
+    s.factor(32)
+    X, st = s.solve(B, prec=32, tol=1e-12, max_iter=30)
+    assert st["converged"] == 1 and SO.rel_residual(A, X, B) <= 1e-12
+    assert np.max(np.abs(X - Xd)) <= 1e-10 * scale
     s.close()
 
 
@@ -354,6 +376,32 @@ def test_wide_block_residual_matches_numpy(nrhs):
     assert abs(st["rel_residual"] - r_np) <= 1e-6 * r_np + 1e-15
     x3, _ = s.solve(B[:, 3], prec=32, tol=0.0, max_iter=2)
     assert np.array_equal(x3, X[:, 3])
+    s.close()
+
+
+@pytest.mark.parametrize("nrhs", [130, 65])
+def test_more_than_64_columns_refined_in_blocks(nrhs):
+    """More than 64 right-hand sides are refined in blocks of at most 64, the
+    last one ragged: with a fixed number of solves every column comes out as
+    it does alone, host and device buffers give the same block, and refined
+    to a tolerance every column meets it (the statistics are the worst
+    block's)."""
+    s, A, vals = _setup(6, 5, 7, 64, seed=9)
+    B, Xt = rhs_block(A, nrhs, 43)
+    s.factor(32)
+    X3, st3 = s.solve(B, prec=32, tol=0.0, max_iter=3)
+    assert st3["nrhs"] == nrhs and st3["iterations"] == 3
+    for j in (0, 63, 64, nrhs - 1):
+        xj, _ = s.solve(B[:, j], prec=32, tol=0.0, max_iter=3)
+        assert np.array_equal(xj, X3[:, j]), j
+    L = W.lib()
+    dB, dX = DeviceBuffer.from_host(np.ascontiguousarray(B)), DeviceBuffer(B.nbytes)
+    st = W._lib.WtSolveStats()
+    assert L.wt_solver_solve(s._h, 32, nrhs, dB.ptr, dX.ptr, 1, 0.0, 3, ctypes.byref(st)) == 0
+    assert np.array_equal(dX.to_host(np.complex128, B.shape), X3)
+    X, st = s.solve(B, prec=32, tol=1e-10, max_iter=30)
+    assert st["converged"] == 1 and st["rel_residual"] <= 1e-10
+    assert np.max(SO.rel_residual(A, X, B)) <= 1e-10
     s.close()
 
 
