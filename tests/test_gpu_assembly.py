@@ -190,3 +190,30 @@ def test_gpu_values_heterogeneous_96x96x64_vs_reference(prec):
     rel, zm = rel_err_entries(a, ref)
     assert zm == 0
     assert rel <= (1e-12 if prec == 64 else 3e-5), rel
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build absent")
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("tau", [4.0, 0.8])
+def test_gpu_values_complex_frequency_vs_reference(prec, tau):
+    """The anti-aliasing damping (tau set): the GPU assembly at the complex
+    frequency omega = 2 pi f + i / tau against the reference's ASMBLE, which
+    forms it at elwave.f90:164, on a layered + random model with PML on every
+    side and the free surface."""
+    nx, ny, nz = 22, 19, 17
+    meta = dict(dx=250.0, freq=0.25, pmld=1250.0, pmlr=1e-3, tau=tau)
+    ft = np.float32 if prec == 32 else np.float64
+    rng = np.random.default_rng(91)
+    vp = (4000 + 3000 * rng.random((ny, nx, nz))).astype(ft)
+    vs = (vp / ft(1.8)).astype(ft)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(ft)
+    dx_, om, pmld, pmlf, ipml = ref_params(meta, prec)
+    assert om.imag != 0.0
+    lam, mu, rho1 = lame_ref(vp, vs, rho, prec)
+    a = W.assemble(_params(nx, ny, nz, dx_, om, pmld, pmlf, ipml), lam, mu, rho1, prec=prec)
+    ierr, ir, jc, ref = O.ref_assemble(nx, ny, nz, meta["dx"], meta["freq"], vp, vs, rho, meta["pmld"],
+                                       meta["pmlr"], tau=tau, prec=prec)
+    assert ierr == 0 and ref.size == a.size
+    rel, zm = rel_err_entries(a, ref)
+    assert zm == 0
+    assert rel <= (1e-12 if prec == 64 else 3e-5), rel

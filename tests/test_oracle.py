@@ -122,6 +122,33 @@ def test_live_reference_random_model(prec):
     assert rc == 0 and zm == 0 and rel <= (1e-12 if prec == 64 else 3e-5)
 
 
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built here")
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("tau", [4.0, 0.8])
+def test_live_reference_complex_frequency(prec, tau):
+    """The anti-aliasing damping: with tau set the reference assembles at the
+    complex frequency omega = 2 pi f + i / tau (elwave.f90:164), which enters
+    the mass terms and every PML stretch factor."""
+    nx, ny, nz, dx = 12, 10, 9, 250.0
+    rng = np.random.default_rng(78)
+    ft = np.float32 if prec == 32 else np.float64
+    vp = (4000 + 3000 * rng.random((ny, nx, nz))).astype(ft)
+    vs = (vp / ft(1.8)).astype(ft)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(ft)
+    meta = dict(dx=dx, freq=0.25, pmld=750.0, pmlr=1e-3, tau=tau)
+    ierr, ir, jc, ref = O.ref_assemble(nx, ny, nz, dx, 0.25, vp, vs, rho, 750.0, 1e-3, tau=tau, prec=prec)
+    assert ierr == 0
+    dx_, om, pmld, pmlf, ipml = ref_params(meta, prec)
+    assert om.imag != 0.0
+    lam, mu, rho1 = lame_ref(vp, vs, rho, prec)
+    rc, a = O.assemble(nx, ny, nz, dx_, om, lam, mu, rho1, pmld, pmlf, ipml, prec=prec)
+    rel, zm = rel_err_entries(a, ref)
+    assert rc == 0 and zm == 0 and rel <= (1e-12 if prec == 64 else 3e-5)
+    # and it is not the real-frequency operator
+    _, _, _, ref_real = O.ref_assemble(nx, ny, nz, dx, 0.25, vp, vs, rho, 750.0, 1e-3, prec=prec)
+    assert rel_err_entries(ref_real, ref)[0] > 1e-3
+
+
 # ---- physics invariants: pins that do not rely on the reference ----
 
 def _interior_rows(nx, ny, nz, ipml):
