@@ -113,9 +113,9 @@ def test_expansion_converges_on_the_device_operator(phi_deg, is_s):
         assert 3.0 <= a / b <= 6.0, r
 
 
-def _golden_solver():
+def _golden_solver(tau=999.999):
     g, nx, ny, nz, dx, freq, pmld, pmlr, srcfn = _golden_scatter(64)
-    meta = dict(dx=dx, freq=freq, pmld=pmld, pmlr=pmlr)
+    meta = dict(dx=dx, freq=freq, pmld=pmld, pmlr=pmlr, tau=tau)
     dxp, om, pmldp, pmlf, ipml = ref_params(meta, 64)
     s = Solver(nx, ny, nz)
     lam, mu, rho = media.lame(g["vp"], g["vs"], g["rho"])
@@ -134,6 +134,23 @@ def test_scattered_source_matches_reference_fixture():
     ref = g["sfld64"]
     assert np.abs(sf - ref).max() <= 1e-10 * np.abs(ref).max()
     s.close()
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build absent")
+@pytest.mark.parametrize("tau", [3.0, 0.7])
+def test_scattered_source_complex_frequency_vs_reference(tau):
+    """With the anti-aliasing damping (omega = 2 pi f + i / tau, elwave.f90:164)
+    the handle assembled at the complex frequency gives the reference's
+    HETFLDSRC at that frequency."""
+    g, s, u, srcfn = _golden_solver(tau)
+    g2, nx, ny, nz, dx, freq, pmld, pmlr, _ = _golden_scatter(64)
+    lb, mb, rb = media.lame(g["vpb"], g["vsb"], g["rhob"])
+    sf = s.scattered_source(lb, mb, rb, u[:, None], srcfn)[:, 0]
+    s.close()
+    ierr, ref = O.ref_hetfldsrc(nx, ny, nz, dx, freq, g["vp"], g["vs"], g["rho"], g["vpb"], g["vsb"], g["rhob"],
+                                g["ub"], g["vb"], g["wb"], srcfn, pmld, pmlr, tau=tau, prec=64)
+    assert ierr == 0
+    assert np.abs(sf - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 def test_scattered_source_invariants_and_wide_blocks():

@@ -130,9 +130,9 @@ def _golden_scatter(prec):
     return g, nx, ny, nz, dx, freq, pmld, pmlr, complex(sr, si)
 
 
-def _oracle_ops(g, nx, ny, nz, dx, freq, pmld, pmlr, prec):
+def _oracle_ops(g, nx, ny, nz, dx, freq, pmld, pmlr, prec, tau=999.999):
     from helpers import ref_params, lame_ref
-    meta = dict(dx=dx, freq=freq, pmld=pmld, pmlr=pmlr)
+    meta = dict(dx=dx, freq=freq, pmld=pmld, pmlr=pmlr, tau=tau)
     dxp, om, pmldp, pmlf, ipml = ref_params(meta, prec)
     ft = np.float32 if prec == 32 else np.float64
     ops = []
@@ -184,3 +184,21 @@ def test_scattered_source_background_equal_live_reference():
     ierr, sf = O.ref_hetfldsrc(nx, ny, nz, dx, freq, g["vpb"], g["vsb"], g["rhob"], g["vpb"], g["vsb"], g["rhob"],
                                g["ub"], g["vb"], g["wb"], srcfn, pmld, pmlr, prec=64)
     assert ierr == 0 and not np.any(sf)
+
+
+@needs_ref
+@pytest.mark.parametrize("tau", [3.0, 0.7])
+def test_scattered_source_complex_frequency_live_reference(tau):
+    """With the anti-aliasing damping set the reference forms both operators
+    at omega = 2 pi f + i / tau (elwave.f90:164): the oracle's (A_bg - A) u_bg
+    at that frequency against the compiled HETFLDSRC."""
+    g, nx, ny, nz, dx, freq, pmld, pmlr, srcfn = _golden_scatter(64)
+    ir, jc, a, abg = _oracle_ops(g, nx, ny, nz, dx, freq, pmld, pmlr, 64, tau=tau)
+    u = RO.nodes_to_unknowns(g["ub"], g["vb"], g["wb"])
+    sf = RO.scattered_source(ir, jc, a, abg, u, srcfn)
+    ierr, ref = O.ref_hetfldsrc(nx, ny, nz, dx, freq, g["vp"], g["vs"], g["rho"], g["vpb"], g["vsb"], g["rhob"],
+                                g["ub"], g["vb"], g["wb"], srcfn, pmld, pmlr, tau=tau, prec=64)
+    assert ierr == 0
+    assert np.abs(sf - ref).max() <= 1e-11 * np.abs(ref).max()
+    # the damping changes the source
+    assert np.abs(ref - g["sfld64"]).max() > 1e-6 * np.abs(ref).max()
