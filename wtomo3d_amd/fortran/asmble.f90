@@ -5,8 +5,8 @@
 ! assembly in libwtomo3d_hip.so through the C ABI entry wt_asmble
 ! (include/wtomo3d.h).  Link this file and libwtomo3d_hip.so in place of
 ! asmble.f90, setcoefs3d.f, calcwts.f, fdfd3d.f, pfdfd3d.f, pml3d.f,
-! fixed3d.f, tpfree.f and zeroas.f; compile it against the driver's own
-! modules.F90.
+! fixed3d.f, tpfree.f and zeroas.f; compile it against the reference
+! program's own modules.F90.
       SUBROUTINE ASMBLE(lunlog, nzero, nx, ny, nz, irptr, jcptr, a, ierr)
       USE ISO_C_BINDING
       USE MODEL_MODULE, ONLY : da, mu, rho, omega, dx, freesurf
