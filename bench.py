@@ -25,7 +25,7 @@ Haskell plane waves each (``sweep``, ``--no-sweep`` skips it), and the
 batched small-model workload (256 models of uneven size, each its own
 frequency and 1-16 right-hand sides; ``batched``, ``--no-batch``).
 
-Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on
+Prints one JSON line.  ``--gpus`` above 1 is refused: the library runs on
 one GPU (frequencies, models and right-hand-side blocks are independent and
 would shard across GPUs without communication).
 """
@@ -226,7 +226,7 @@ def batch_sample():
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding "
+        print("bench.py: the library runs on one GPU only (--gpus 1); multi-GPU sharding "
               "of frequencies / models / right-hand-side blocks is not built", file=sys.stderr)
         sys.exit(2)
     import wtomo3d_amd as W

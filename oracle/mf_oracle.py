@@ -104,7 +104,7 @@ def cpu_rate_sample(fronts, dtype=np.complex64, budget_s=20.0, seed=0):
 
 
 # ---------------------------------------------------------------------------
-# FP64 multifrontal direct solve (the checker of full-size solves)
+# FP64 multifrontal direct solve (the reference solution of full-size solve tests)
 
 def nd_tree(nx, ny, nz, leaf=64):
     """The tree of ``nd_fronts`` with node sets: list of fronts in post-order,

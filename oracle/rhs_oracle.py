@@ -39,8 +39,8 @@ def nodes_to_unknowns(ub, vb, wb):
 
 def planewave_expand(nx, ny, nz, dx, U, W, kh, phi):
     """The 3-D background field of plane waves from their depth profiles (the
-    expansion the reference's missing planewvexp.f90 would do, defined by
-    this build; interface.f90:43-47):
+    expansion the reference's missing planewvexp.f90 would do, defined here
+    from its interface, interface.f90:43-47):
 
         u_j = (U_j(z) cos phi_j, U_j(z) sin phi_j, W_j(z))
               * exp(-i kh_j (x cos phi_j + y sin phi_j)),
