@@ -72,8 +72,10 @@ def parse():
     ap.add_argument("--no-split", action="store_true",
                     help="keep every complex64 product on the fp32 matrix cores (no bf16-split GEMMs)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the multi-frequency layered sample")
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="requests/s of the serving sample")
-    ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the serving sample")
+    ap.add_argument("--serve-rate", type=float, default=0.7,
+                    help="requests/s of the serving sample (default: the highest rate of the mix one GPU "
+                         "was measured to sustain, so p50/p99 are steady-state latencies)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving sample")
     ap.add_argument("--burst-seconds", type=float, default=3.0,
                     help="arrival window of the 20 requests/s serving burst (0: none)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
