@@ -83,10 +83,13 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(nx, ny, nz, nrhs, budget):
+def cpu_baseline(nx, ny, nz, nrhs, budget, solves=4):
     """The oracle's dense front factorization (LAPACK via numpy/scipy) timed
     on the host on the largest fronts of the same elimination tree, within
-    `budget` seconds; the tree's total work at that rate gives RHS/s."""
+    `budget` seconds; the tree's total work at that rate -- the
+    factorization plus `solves` block solves of `nrhs` columns (8 flops per
+    factor entry and column each, the same count as the GPU step's) --
+    gives RHS/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import mf_oracle as MF
     try:
@@ -105,12 +108,16 @@ def cpu_baseline(nx, ny, nz, nrhs, budget):
         limiter.unregister() if hasattr(limiter, "unregister") else None
     rate = fl / t
     total = MF.tree_flops(fronts)
-    t_tree = total / rate
+    entries = float(sum(p * p + 2 * p * u for p, u, _ in fronts))  # [L11\U11; L21] and U12
+    solve_fl = solves * 8.0 * nrhs * entries
+    t_tree = (total + solve_fl) / rate
     return {"value": nrhs / t_tree, "unit": "RHS/s", "cores": cores, "kind": "port",
             "sample": "complex64 dense partial LU (LAPACK getrf/trsm + gemm) of the %d largest fronts "
-                      "(p,u)=%s of the 96x96x64 elimination tree, %.1f s at %.3g GFLOP/s; whole-tree "
-                      "%.3g flop extrapolated to %.1f s per 64-RHS factorization (solve not counted)"
-                      % (len(sample), sample[:3], t, rate / 1e9, total, t_tree)}
+                      "(p,u)=%s of the %dx%dx%d elimination tree, %.1f s at %.3g GFLOP/s; the whole tree's "
+                      "%.3g flop plus %d block solves of %d columns (%.3g flop) extrapolated at that rate to "
+                      "%.1f s per step"
+                      % (len(sample), sample[:3], nx, ny, nz, t, rate / 1e9, total, solves, nrhs, solve_fl,
+                         t_tree)}
 
 
 def reference_assembly(nx, ny, nz, dx, freq, vp, vs, rho, gpu_ms):
@@ -412,7 +419,7 @@ def main():
     asm = prof["assemble"]
     asm_gbs = asm["bytes"] / (asm["ms"] * 1e-3) / 1e9 if asm["ms"] > 0 else 0.0
     progress("CPU baseline")
-    cpu = None if a.no_cpu else cpu_baseline(nx, ny, nz, nrhs, a.cpu_budget)
+    cpu = None if a.no_cpu else cpu_baseline(nx, ny, nz, nrhs, a.cpu_budget, max(int(round(n_solves / a.steps)), 1))
     if cpu is not None:
         progress("CPU reference assembly")
         cpu["assembly_reference"] = reference_assembly(nx, ny, nz, dx, freq, vp, vs, rho,
