@@ -39,7 +39,7 @@ int run_gemm(int var, int M, int N, int K, const void* A, int64_t lda, const voi
             for (int r = 0; r < reps && rc == WT_OK; ++r) {
                 if (hipMemcpy(dC, C, nc * cs, hipMemcpyHostToDevice)) { rc = WT_ERR_HIP; break; }
                 (void)hipEventRecord(e0, 0);
-                rc = wt_gemm_launch<R>(dprob, dts, 1, ts[1], var, 0);
+                rc = wt_gemm_launch<R>(dprob, dts, 1, ts[1], var, 0, mode == WT_GEMM_SUB || mode == WT_GEMM_ADD);
                 (void)hipEventRecord(e1, 0);
             }
             float t = 0;

@@ -110,6 +110,8 @@ int wt_gemm_reduce_launch(const wt_gemm_red<R>* d_reds, int nred, int64_t maxM, 
 
 // Launches nprob problems (device arrays probs[nprob], tile_start[nprob+1],
 // tile_start[0] = 0) as one grid of tile_start[nprob] blocks of variant var.
+// rmw != 0 declares that every problem reads C (WT_GEMM_SUB / WT_GEMM_ADD):
+// the 64 x 64 tile then loads C ahead, with identical results.
 template <typename R>
 int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, int nprob,
-                   int64_t ntiles, int var, hipStream_t stream);
+                   int64_t ntiles, int var, hipStream_t stream, int rmw = 0);

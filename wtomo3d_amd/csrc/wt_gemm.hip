@@ -68,6 +68,9 @@ using gv2 = __attribute__((address_space(1))) v2<R>;
 typedef __attribute__((address_space(1))) const int32_t gi32;
 
 constexpr int NT = 256;  // threads per block
+#ifndef GEMM_CPREF
+#define GEMM_CPREF 1
+#endif
 // complex k per slice.  The slice depth does not change the order in which
 // any accumulator sums its k terms (the MFMA sequence over k is the same),
 // so results do not depend on it; 16 halves the barriers per k of 8 and
@@ -85,7 +88,7 @@ constexpr int gemm_occupancy()
     return sizeof(R) == 4 ? 2 : (V == 2 ? 2 : 1);
 }
 
-template <typename R, int V>
+template <typename R, int V, bool CP = false>
 __global__ __launch_bounds__(NT, (gemm_occupancy<R, V>())) void gemm_batched_kernel(const wt_gemm_prob<R>* __restrict__ probs,
                                                             const int64_t* __restrict__ tile_start,
                                                             int nprob, int64_t ntiles)
@@ -190,6 +193,28 @@ __global__ __launch_bounds__(NT, (gemm_occupancy<R, V>())) void gemm_batched_ker
         for (int e = 0; e < EB; ++e) { sBr[buf][kb + e][nbb] = rb[e].x; sBi[buf][kb + e][nbb] = rb[e].y; }
     };
 
+    // CP (launches whose products all read C: C -= A B, C += A B): C loaded
+    // with the first slice, so its latency overlaps the K loop -- an instance
+    // of its own, as the 32 registers it holds per lane would cost the other
+    // launches occupancy (tiles whose lanes hold at most 16 outputs)
+    const int mode = P.mode;
+    constexpr bool CPRE = CP && GEMM_CPREF && SM * SN * M_::NACC <= 16;
+    const bool rmw = mode == WT_GEMM_SUB || mode == WT_GEMM_ADD;
+    v2<R> cpre[CPRE ? SM : 1][CPRE ? SN : 1][CPRE ? M_::NACC : 1];
+    if constexpr (CPRE) {
+        if (rmw) {
+#pragma unroll
+            for (int i = 0; i < SM; ++i)
+#pragma unroll
+                for (int j = 0; j < SN; ++j)
+#pragma unroll
+                    for (int v = 0; v < M_::NACC; ++v) {
+                        const int m = m0 + wm0 + i * M_::MT + M_::dcol(lane);
+                        const int n = n0 + wn0 + j * M_::MT + M_::drow(lane, v);
+                        cpre[i][j][v] = (m < P.M && n < P.N) ? gC[(int64_t)n * P.ldc + m] : v2<R>{R(0), R(0)};
+                    }
+        }
+    }
     if (nkt > 0) {
         load(0);
         store(0);
@@ -228,7 +253,6 @@ __global__ __launch_bounds__(NT, (gemm_occupancy<R, V>())) void gemm_batched_ker
     }
 
     // epilogue
-    const int mode = P.mode;
 #pragma unroll
     for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -246,7 +270,9 @@ __global__ __launch_bounds__(NT, (gemm_occupancy<R, V>())) void gemm_batched_ker
                     } else if (mode == WT_GEMM_NSET) {
                         *pc = v2<R>{-re, -im};
                     } else {
-                        const v2<R> c = *pc;
+                        v2<R> c;
+                        if constexpr (CPRE) c = cpre[i][j][v];
+                        else c = *pc;
                         if (mode == WT_GEMM_SUB) *pc = v2<R>{c.x - re, c.y - im};
                         else *pc = v2<R>{c.x + re, c.y + im};
                     }
@@ -505,18 +531,26 @@ __global__ __launch_bounds__(NT, T == 128 ? 1 : 2) void gemm_split_kernel(const 
 
 template <typename R>
 int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, int nprob, int64_t ntiles, int var,
-                   hipStream_t stream)
+                   hipStream_t stream, int rmw)
 {
     if (nprob <= 0 || ntiles <= 0) return WT_OK;
     if (ntiles > 0x7fffffff) return WT_ERR_ARG;
+    // the 64 x 64 tile with C loaded ahead, for launches that read C
+    auto tile64 = [&]() {
+        if (rmw)
+            hipLaunchKernelGGL((gemm_batched_kernel<R, 2, true>), dim3((unsigned)ntiles), dim3(NT), 0, stream,
+                               d_probs, d_tile_start, nprob, ntiles);
+        else
+            hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
+                               d_tile_start, nprob, ntiles);
+    };
     switch (var) {
     case 1:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 1>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                            d_tile_start, nprob, ntiles);
         break;
     case 2:
-        hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
-                           d_tile_start, nprob, ntiles);
+        tile64();
         break;
     case 3:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 3>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
@@ -527,16 +561,14 @@ int wt_gemm_launch(const wt_gemm_prob<R>* d_probs, const int64_t* d_tile_start, 
             hipLaunchKernelGGL((gemm_split_kernel<128>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                                d_tile_start, nprob, ntiles);
         else
-            hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
-                               d_tile_start, nprob, ntiles);
+            tile64();
         break;
     case WT_GEMM_SPLIT64:
         if constexpr (sizeof(R) == 4)
             hipLaunchKernelGGL((gemm_split_kernel<64>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
                                d_tile_start, nprob, ntiles);
         else
-            hipLaunchKernelGGL((gemm_batched_kernel<R, 2>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
-                               d_tile_start, nprob, ntiles);
+            tile64();
         break;
     default:
         hipLaunchKernelGGL((gemm_batched_kernel<R, 0>), dim3((unsigned)ntiles), dim3(NT), 0, stream, d_probs,
@@ -584,5 +616,5 @@ template int wt_gemm_reduce_launch<float>(const wt_gemm_red<float>*, int, int64_
                                           hipStream_t);
 template int wt_gemm_reduce_launch<double>(const wt_gemm_red<double>*, int, int64_t, int64_t, const cpx<double>*,
                                            hipStream_t);
-template int wt_gemm_launch<float>(const wt_gemm_prob<float>*, const int64_t*, int, int64_t, int, hipStream_t);
-template int wt_gemm_launch<double>(const wt_gemm_prob<double>*, const int64_t*, int, int64_t, int, hipStream_t);
+template int wt_gemm_launch<float>(const wt_gemm_prob<float>*, const int64_t*, int, int64_t, int, hipStream_t, int);
+template int wt_gemm_launch<double>(const wt_gemm_prob<double>*, const int64_t*, int, int64_t, int, hipStream_t, int);

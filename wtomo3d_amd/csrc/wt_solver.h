@@ -22,6 +22,7 @@ struct wt_gemm_launch_rec {
     // products split along K: their reductions, run after the launch
     int64_t red_off, ws_len, red_maxM, red_maxN;
     int32_t nred;
+    int32_t rmw;  // every problem reads C (WT_GEMM_SUB / WT_GEMM_ADD)
 };
 
 template <typename R>
@@ -123,6 +124,9 @@ struct wt_gemm_plan {
         // fp64: the 64 x 64 tile throughout (two blocks per CU; measured
         // fastest on large and narrow launches alike)
         if (sizeof(R) == 8) cur.var = 2;
+        cur.rmw = 1;
+        for (int64_t i = cur.prob_off; i < (int64_t)probs.size(); ++i)
+            if (probs[i].mode != WT_GEMM_SUB && probs[i].mode != WT_GEMM_ADD) cur.rmw = 0;
         launches.push_back(cur);
         return (int)launches.size() - 1;
     }
@@ -132,7 +136,7 @@ struct wt_gemm_plan {
     {
         const wt_gemm_launch_rec& L = launches[i];
         int rc = wt_gemm_launch<R>(d_probs + L.prob_off, d_tiles + L.var * tstride + L.tile_off, L.nprob,
-                                   L.ntiles[L.var], L.var, s);
+                                   L.ntiles[L.var], L.var, s, L.rmw);
         if (rc == WT_OK && L.nred > 0)
             rc = wt_gemm_reduce_launch<R>(d_reds + L.red_off, L.nred, L.red_maxM, L.red_maxN, ws, s);
         return rc;
