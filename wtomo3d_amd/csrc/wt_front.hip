@@ -74,6 +74,10 @@ __global__ __launch_bounds__(256) void scatter_rows_kernel(const int32_t* __rest
 }
 
 
+#ifndef EA_UNROLL
+#define EA_UNROLL 4
+#endif
+
 // Adds the contribution blocks of the listed children into their parents.
 // At most one child per parent is listed, so the adds do not race.  The
 // child-to-parent map ascends, so the child's indices below q (the first
@@ -120,17 +124,36 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const wt_dfront* __rest
     cpx<R>* Lp = factor + P.Loff;
     cpx<R>* Up = factor + P.Uoff;
     cpx<R>* Cp = cb_parent + P.CBoff;
+    // EA_UNROLL entries per thread and pass: their index, child and target
+    // loads are all issued before the first store.  The map is injective, so
+    // the targets of one pass are distinct and every entry still receives
+    // exactly one add.
+    constexpr int UR = EA_UNROLL;
+    const int64_t nt = blockDim.x;
     for (int64_t j = (part == 2 && j0 < q) ? q : j0; j < j1; ++j) {
         const int64_t s = map[j];
         const int64_t i0 = part == 2 ? q : 0, i1 = (part == 1 && j >= q) ? q : uc;
-        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            const int64_t r = map[i];
-            const cpx<R> v = cb[j * uc + i];
-            cpx<R>* t;
-            if (s < pp) t = Lp + s * ld + r;
-            else if (r < pp) t = Up + (s - pp) * pp + r;
-            else t = Cp + (s - pp) * pu + (r - pp);
-            *t = *t + v;
+        const cpx<R>* cbj = cb + j * uc;
+        for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += UR * nt) {
+            int64_t r[UR];
+            cpx<R> v[UR], o[UR];
+            cpx<R>* t[UR];
+#pragma unroll
+            for (int k = 0; k < UR; ++k) {
+                const int64_t i = ib + k * nt;
+                r[k] = i < i1 ? map[i] : 0;
+                v[k] = i < i1 ? cbj[i] : cpx<R>(0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < UR; ++k) {
+                if (s < pp) t[k] = Lp + s * ld + r[k];
+                else if (r[k] < pp) t[k] = Up + (s - pp) * pp + r[k];
+                else t[k] = Cp + (s - pp) * pu + (r[k] - pp);
+                if (ib + k * nt < i1) o[k] = *t[k];
+            }
+#pragma unroll
+            for (int k = 0; k < UR; ++k)
+                if (ib + k * nt < i1) *t[k] = o[k] + v[k];
         }
     }
 }
