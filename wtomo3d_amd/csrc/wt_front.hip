@@ -9,6 +9,10 @@
 // solves and the forward/back substitutions run as MFMA GEMMs.
 #include "wt_front.h"
 
+#ifndef FILL16
+#define FILL16 1
+#endif
+
 namespace {
 
 template <typename R>
@@ -17,6 +21,19 @@ __global__ void fill_zero_kernel(cpx<R>* __restrict__ p, int64_t n)
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = cpx<R>(R(0), R(0));
+}
+
+// Zeroes n16 16-byte units, four stores per thread and pass (consecutive
+// lanes on consecutive units in each store).
+__global__ __launch_bounds__(256) void fill_zero16_kernel(uint4* __restrict__ p, int64_t n16)
+{
+    const int64_t nt = (int64_t)gridDim.x * blockDim.x;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += 4 * nt) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * nt < n16) p[i + k * nt] = z;
+    }
 }
 
 template <typename Ri, typename R>
@@ -890,6 +907,19 @@ template <typename R>
 int wt_fill_zero(cpx<R>* p, int64_t n, hipStream_t s)
 {
     if (n <= 0) return WT_OK;
+    const int64_t bytes = n * (int64_t)sizeof(cpx<R>);
+    if (FILL16 && ((uintptr_t)p & 15) == 0 && bytes >= 16) {
+        // the 16-byte units, then an odd complex64 entry at the end
+        const int64_t n16 = bytes / 16;
+        int64_t blocks = (n16 + 1023) / 1024;
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(fill_zero16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)p, n16);
+        WT_HIP(hipGetLastError());
+        const int64_t done = n16 * 16 / (int64_t)sizeof(cpx<R>);
+        if (done == n) return WT_OK;
+        p += done;
+        n -= done;
+    }
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(fill_zero_kernel<R>, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
